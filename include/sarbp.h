@@ -1,0 +1,216 @@
+/*
+ * sarbp.h -- C ABI of the MI355X (gfx950) time-domain back-projection imager.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns a
+ * sarbp_status and never exits or throws.  Device pointers are caller-owned
+ * HIP device allocations; `stream` is a hipStream_t passed as void* (NULL is
+ * the null stream).  Work is enqueued asynchronously on that stream unless the
+ * function says otherwise.
+ *
+ * Reference interface replaced (reference repository wenj-yan/
+ * SAR_CambrianAccelerationCard, code/SAR_BP/CUDA):
+ *   void launchBackProjection(cufftComplex* Srnm, cufftComplex* Image,
+ *                             const SARParams params, int Na, int Nr);
+ *     -- BPA_CUDA_kernels.cu:91-99, declared at BPA_CUDA.cpp:8-9, called from
+ *        SAR_Processor::applyBPAlgorithm (BPA_CUDA.cpp:83-90).
+ *   struct SARParams -- BPA_CUDA.h:28-50.
+ *   CHECK_CUDA (print + exit) -- BPA_CUDA.h:11-17, replaced by status codes.
+ * The arithmetic follows the BP loops of code/SAR_BP/matlab/BPA_SAR.m:133-153
+ * and :178-197 (pixel value = sum over pulses of the range-compressed sample at
+ * the pixel's slant range times exp(+j 4 pi fc R / c)), not the CUDA kernel,
+ * whose range does not depend on the pulse (BPA_CUDA_kernels.cu:56-64).
+ */
+#ifndef SARBP_H
+#define SARBP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SARBP_ABI_VERSION 1
+
+typedef enum sarbp_status {
+  SARBP_OK = 0,
+  SARBP_ERR_INVALID_ARGUMENT = 1, /* null pointer, bad size or enum value    */
+  SARBP_ERR_OUT_OF_MEMORY = 2,    /* a device or host allocation failed      */
+  SARBP_ERR_OVER_BUDGET = 3,      /* job can never fit the server HBM budget */
+  SARBP_ERR_DEVICE = 4,           /* a HIP runtime call failed               */
+  SARBP_ERR_UNSUPPORTED = 5,      /* valid request this build cannot run     */
+  SARBP_ERR_NOT_READY = 6,        /* job still queued or running             */
+  SARBP_ERR_NOT_FOUND = 7,        /* unknown job id                          */
+  SARBP_ERR_CANCELLED = 8,        /* job was cancelled                       */
+  SARBP_ERR_TIMEOUT = 9,          /* wait or watchdog expired                */
+  SARBP_ERR_SHUTDOWN = 10         /* server is shutting down                 */
+} sarbp_status;
+
+/* Interleaved complex64: the layout of cufftComplex / std::complex<float>. */
+typedef struct sarbp_cf32 {
+  float re, im;
+} sarbp_cf32;
+
+typedef enum sarbp_interp {
+  SARBP_INTERP_NEAREST = 0, /* MATLAB round() of the bin (BPA_SAR.m:141,184) */
+  SARBP_INTERP_LINEAR = 1,  /* two-tap linear                                */
+  SARBP_INTERP_SINC8 = 2    /* 8-tap Kaiser-windowed sinc                    */
+} sarbp_interp;
+
+typedef enum sarbp_precision {
+  /* FP64 per-(tile, pulse) reference range and phase, FP32 per-pixel
+   * differential with exact modular phase reduction.  Default. */
+  SARBP_PREC_REF = 0,
+  SARBP_PREC_F64 = 1, /* FP64 slant range and phase for every pixel-pulse    */
+  SARBP_PREC_F32 = 2  /* FP32 positions, range and phase (inaccurate)         */
+} sarbp_precision;
+
+typedef enum sarbp_dtype {
+  SARBP_DTYPE_C64 = 0,   /* profiles as complex64 (8 B / sample)             */
+  SARBP_DTYPE_BF16X2 = 1 /* profiles as bf16 re, bf16 im (4 B / sample; re in
+                            the low half-word)                               */
+} sarbp_dtype;
+
+/* Image grid: pixel (ix, iy) sits at origin + ix*ux + iy*uy (metres, FP64).
+ * The image is stored row-major [ny][nx], ix fastest.  In the reference the
+ * rows are azimuth and the columns range (BPA_CUDA_kernels.cu:53-54). */
+typedef struct sarbp_grid {
+  double origin[3];
+  double ux[3];
+  double uy[3];
+  int32_t nx, ny;
+} sarbp_grid;
+
+/* Radar / processing parameters (the SARParams of BPA_CUDA.h:28-50, reduced to
+ * what BP reads, in FP64). */
+typedef struct sarbp_params {
+  double c;           /* propagation speed, m/s                               */
+  double fc;          /* carrier frequency, Hz (phase = 4 pi fc R / c)        */
+  double r0;          /* slant range of range sample 0, m                     */
+  double dr;          /* range sample spacing, m                              */
+  double kaiser_beta; /* Kaiser window shape for SARBP_INTERP_SINC8           */
+  int32_t n_pulses;   /* profiles rows, positions rows                        */
+  int32_t n_samples;  /* samples per profile                                  */
+  int32_t interp;     /* sarbp_interp                                         */
+  int32_t precision;  /* sarbp_precision                                      */
+  int32_t dtype;      /* sarbp_dtype                                          */
+  int32_t accumulate; /* 0: image overwritten; 1: sum added into the image    */
+} sarbp_params;
+
+typedef struct sarbp_context sarbp_context;
+
+/* Human-readable text for a status code (static storage, never NULL). */
+const char* sarbp_strerror(sarbp_status status);
+
+/* ABI version of the loaded library (SARBP_ABI_VERSION it was built with). */
+int32_t sarbp_abi_version(void);
+
+/* Creates a context bound to HIP device `device`.  Kernel tables are uploaded
+ * lazily on that device.  A context may be used from several host threads; calls
+ * are serialised internally. */
+sarbp_status sarbp_create(sarbp_context** ctx, int32_t device);
+void sarbp_destroy(sarbp_context* ctx);
+
+/*
+ * Back-projects one image.  Replaces launchBackProjection (BPA_CUDA_kernels.cu:91).
+ *   d_profiles : [n_pulses][n_samples] range-compressed pulses (dtype), device
+ *   d_pos      : [n_pulses][3] FP64 antenna phase-centre positions, device
+ *   grid       : host struct
+ *   params     : host struct
+ *   d_image    : [ny][nx] complex64, device; overwritten or accumulated into
+ * Range sample k of pulse m is at slant range r0 + k*dr; samples outside
+ * [0, n_samples) read as zero.  Asynchronous on `stream`.
+ */
+sarbp_status sarbp_backproject(sarbp_context* ctx, const void* d_profiles,
+                               const double* d_pos, const sarbp_grid* grid,
+                               const sarbp_params* params, sarbp_cf32* d_image,
+                               void* stream);
+
+/* Same, then waits for the stream (the synchronous behaviour of
+ * launchBackProjection, BPA_CUDA_kernels.cu:97-98). */
+sarbp_status sarbp_backproject_sync(sarbp_context* ctx, const void* d_profiles,
+                                    const double* d_pos, const sarbp_grid* grid,
+                                    const sarbp_params* params,
+                                    sarbp_cf32* d_image, void* stream);
+
+/* ---- plans ------------------------------------------------------------------ */
+
+/*
+ * A plan fixes the launch configuration for one geometry.  sarbp_backproject
+ * plans from the grid alone (range window sized by the triangle inequality,
+ * second-order range expansion); a plan made from the host copy of the
+ * antenna positions bounds the line-of-sight directions and the minimum slant
+ * range over the whole scene, which gives shorter LDS windows and, where the
+ * bound allows it, the cheaper first-order range expansion.  The plan stays
+ * valid for any positions inside the bounds computed at creation (the device
+ * positions passed to sarbp_plan_execute must be the ones planned for).
+ */
+typedef struct sarbp_plan sarbp_plan;
+
+typedef struct sarbp_plan_info {
+  int32_t tiles_x, tiles_y;     /* workgroups of the plain launch            */
+  int32_t tile_nx, tile_ny;     /* pixels per tile                           */
+  int32_t window;               /* LDS range window per pulse, samples       */
+  int32_t pulses_per_stage;     /* pulses staged per barrier interval        */
+  int32_t pixels_per_lane;
+  int32_t range_order;          /* 1 or 2: order of the per-pixel expansion  */
+  int32_t lds_bytes;            /* dynamic LDS per workgroup                 */
+  double max_phase_err_rad;     /* bound on the expansion truncation error   */
+} sarbp_plan_info;
+
+sarbp_status sarbp_plan_create(sarbp_context* ctx, const double* h_pos,
+                               const sarbp_grid* grid,
+                               const sarbp_params* params, sarbp_plan** plan);
+sarbp_status sarbp_plan_execute(sarbp_plan* plan, const void* d_profiles,
+                                const double* d_pos, sarbp_cf32* d_image,
+                                void* stream);
+sarbp_status sarbp_plan_get_info(const sarbp_plan* plan, sarbp_plan_info* info);
+void sarbp_plan_destroy(sarbp_plan* plan);
+
+/* Detail text of the last error raised on the calling thread ("" if none). */
+const char* sarbp_last_error_detail(void);
+
+/* ---- launch tuning -------------------------------------------------------- */
+
+/* Kernel configuration knobs chosen by the autotuner.  pixels_per_lane is the
+ * number of image rows each lane accumulates (tile = 64 x (4*pixels_per_lane)
+ * pixels), pulses_per_stage the pulses staged in LDS per barrier interval. */
+typedef struct sarbp_tuning {
+  int32_t pixels_per_lane;   /* 1, 2, 4 or 8                                 */
+  int32_t pulses_per_stage;  /* 8, 16, 32 or 64                              */
+} sarbp_tuning;
+
+/* Sets the configuration used by later calls with this (interp, precision,
+ * dtype) triple on this context; {0,0} restores the built-in default. */
+sarbp_status sarbp_set_tuning(sarbp_context* ctx, int32_t interp,
+                              int32_t precision, int32_t dtype,
+                              const sarbp_tuning* tuning);
+sarbp_status sarbp_get_tuning(sarbp_context* ctx, int32_t interp,
+                              int32_t precision, int32_t dtype,
+                              sarbp_tuning* tuning);
+
+/* ---- batched launch --------------------------------------------------------- */
+
+typedef struct sarbp_job {
+  const void* d_profiles;
+  const double* d_pos;
+  sarbp_grid grid;
+  sarbp_params params;
+  sarbp_cf32* d_image;
+} sarbp_job;
+
+/*
+ * Back-projects n_jobs independent images of arbitrary, uneven sizes in one
+ * persistent launch: a device work list of (job, tile, pulse-chunk) items is
+ * drained by a fixed grid of workgroups.  All jobs must share interp,
+ * precision and dtype.  Each job's pixel sums are formed in a fixed pulse
+ * order, so a job's image is bitwise identical whatever else is in the batch.
+ * The work list lives in context-owned device memory.
+ */
+sarbp_status sarbp_backproject_batch(sarbp_context* ctx, const sarbp_job* jobs,
+                                     int32_t n_jobs, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SARBP_H */

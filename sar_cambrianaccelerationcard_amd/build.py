@@ -1,0 +1,100 @@
+"""Builds the native libraries of the package in-tree.
+
+    python -m sar_cambrianaccelerationcard_amd.build        # everything
+    python -m sar_cambrianaccelerationcard_amd.build sarbp  # one target
+
+Targets (all written to sar_cambrianaccelerationcard_amd/_lib/):
+  libsarbp.so   gfx950 HIP back-projection kernels + C ABI (include/sarbp.h)
+  libsarsim.so  FP64 CPU scene simulator (csrc/sarsim.h)
+  sarbp_run     C++ command-line driver (mirror of the reference's main.cpp)
+
+Every HIP file is compiled with packed FP32 instructions disabled
+(-target-feature -packed-fp32-ops) for the device.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+LIB = os.path.join(PKG, "_lib")
+INCLUDE = os.path.join(ROOT, "include")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-command-line-argument",
+    f"-I{INCLUDE}", f"-I{CSRC}",
+]
+
+
+def _run(cmd: list[str]) -> None:
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    # hipcc's host pass does not know the packed-fp32 feature and says so.
+    err = "\n".join(l for l in r.stderr.splitlines()
+                    if "packed-fp32-ops" not in l and l.strip())
+    if r.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{err}")
+
+
+def _newer(target: str, sources: list[str]) -> bool:
+    if not os.path.exists(target):
+        return False
+    t = os.path.getmtime(target)
+    return all(os.path.getmtime(s) <= t for s in sources)
+
+
+def build_sarbp(force: bool = True) -> str:
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libsarbp.so")
+    srcs = [os.path.join(CSRC, f) for f in ("sarbp_kernels.hip", "sarbp_api.hip")]
+    deps = srcs + [os.path.join(CSRC, "sarbp_internal.h"), os.path.join(INCLUDE, "sarbp.h")]
+    if not force and _newer(out, deps):
+        return out
+    objs = []
+    for s in srcs:
+        o = os.path.join(LIB, os.path.basename(s) + ".o")
+        _run([HIPCC, *HIP_FLAGS, "-c", s, "-o", o])
+        objs.append(o)
+    tmp = out + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, out)
+    for o in objs:
+        os.remove(o)
+    return out
+
+
+def build_sarsim(force: bool = True) -> str:
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libsarsim.so")
+    src = os.path.join(CSRC, "sarsim.cpp")
+    if not force and _newer(out, [src, os.path.join(CSRC, "sarsim.h")]):
+        return out
+    tmp = out + ".tmp"
+    _run(["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-Wall",
+          "-Wextra", src, "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_all(force: bool = True) -> list[str]:
+    return [build_sarbp(force), build_sarsim(force)]
+
+
+if __name__ == "__main__":
+    which = sys.argv[1:] or ["all"]
+    for w in which:
+        if w == "all":
+            print("\n".join(build_all()))
+        elif w == "sarbp":
+            print(build_sarbp())
+        elif w == "sarsim":
+            print(build_sarsim())
+        else:
+            raise SystemExit(f"unknown target {w}")

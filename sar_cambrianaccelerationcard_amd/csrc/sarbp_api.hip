@@ -1,0 +1,498 @@
+// sarbp_api.hip -- host side of the C ABI declared in include/sarbp.h:
+// argument validation, launch planning (tile shape, LDS range window, stage
+// depth), per-context Kaiser-sinc tables, and the plain / planned / batched
+// launches.  Nothing here exits or throws across the ABI.
+#include "sarbp.h"
+#include "sarbp_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+using namespace sarbp;
+
+namespace {
+
+thread_local std::string g_detail;
+
+sarbp_status fail(sarbp_status st, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+sarbp_status fail(sarbp_status st, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_detail = buf;
+  return st;
+}
+
+sarbp_status hip_fail(hipError_t e, const char* what) {
+  return fail(e == hipErrorOutOfMemory ? SARBP_ERR_OUT_OF_MEMORY : SARBP_ERR_DEVICE,
+              "%s: %s", what, hipGetErrorString(e));
+}
+
+// Restores the caller's current device on scope exit.
+struct DeviceGuard {
+  int prev = -1;
+  hipError_t err = hipSuccess;
+  explicit DeviceGuard(int dev) {
+    if ((err = hipGetDevice(&prev)) == hipSuccess && prev != dev) err = hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+constexpr double kPi = 3.14159265358979323846;
+
+double bessel_i0(double x) {
+  double q = 0.25 * x * x, term = 1.0, sum = 1.0;
+  for (int k = 1; k < 500; ++k) {
+    term *= q / ((double)k * (double)k);
+    sum += term;
+    if (term < 1e-18 * sum) break;
+  }
+  return sum;
+}
+
+double kaiser_sinc8(double x, double beta, double inv_i0b) {
+  const double ax = std::fabs(x);
+  if (ax >= 4.0) return 0.0;
+  const double t = x / 4.0;
+  const double w = bessel_i0(beta * std::sqrt(1.0 - t * t)) * inv_i0b;
+  const double s = (x == 0.0) ? 1.0 : std::sin(kPi * x) / (kPi * x);
+  return s * w;
+}
+
+// Table row i, tap t: (K(i/Q + 3 - t), K((i+1)/Q + 3 - t) - K(i/Q + 3 - t)).
+std::vector<float> make_sinc_table(double beta) {
+  std::vector<float> tab((size_t)kSincQ * 16);
+  const double inv_i0b = 1.0 / bessel_i0(beta);
+  for (int i = 0; i < kSincQ; ++i) {
+    for (int t = 0; t < 8; ++t) {
+      const double k0 = kaiser_sinc8((double)i / kSincQ + 3.0 - t, beta, inv_i0b);
+      const double k1 = kaiser_sinc8((double)(i + 1) / kSincQ + 3.0 - t, beta, inv_i0b);
+      tab[(size_t)i * 16 + 2 * t] = (float)k0;
+      tab[(size_t)i * 16 + 2 * t + 1] = (float)(k1 - k0);
+    }
+  }
+  return tab;
+}
+
+int tuning_key(int interp, int prec, int dtype) { return interp * 100 + prec * 10 + dtype; }
+
+bool finite_all(const double* v, int n) {
+  for (int i = 0; i < n; ++i)
+    if (!std::isfinite(v[i])) return false;
+  return true;
+}
+
+double norm3(const double* v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+
+}  // namespace
+
+struct sarbp_context {
+  int device = 0;
+  int n_cu = 256;
+  std::mutex mu;
+  std::map<double, float4*> sinc_tabs;
+  std::map<int, sarbp_tuning> tuning;
+  // batched launches
+  JobDesc* d_jobs = nullptr;
+  int32_t jobs_cap = 0;
+  JobDesc* h_jobs = nullptr;  // pinned staging copy of the descriptor table
+  unsigned long long* d_counter = nullptr;
+};
+
+struct sarbp_plan {
+  sarbp_context* ctx;
+  BpArgs args;
+  int interp, precision, dtype, ppl;
+  sarbp_plan_info info;
+};
+
+namespace {
+
+sarbp_status validate(const sarbp_grid* g, const sarbp_params* p) {
+  if (!g || !p) return fail(SARBP_ERR_INVALID_ARGUMENT, "grid or params is NULL");
+  if (g->nx < 0 || g->ny < 0) return fail(SARBP_ERR_INVALID_ARGUMENT, "negative image size %d x %d", g->nx, g->ny);
+  if (!finite_all(g->origin, 3) || !finite_all(g->ux, 3) || !finite_all(g->uy, 3))
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "grid vectors must be finite");
+  if (p->n_pulses < 0) return fail(SARBP_ERR_INVALID_ARGUMENT, "n_pulses < 0");
+  if (p->n_samples < 1) return fail(SARBP_ERR_INVALID_ARGUMENT, "n_samples < 1");
+  if (p->interp < 0 || p->interp > 2) return fail(SARBP_ERR_INVALID_ARGUMENT, "interp %d", p->interp);
+  if (p->precision < 0 || p->precision > 2) return fail(SARBP_ERR_INVALID_ARGUMENT, "precision %d", p->precision);
+  if (p->dtype < 0 || p->dtype > 1) return fail(SARBP_ERR_INVALID_ARGUMENT, "dtype %d", p->dtype);
+  if (p->accumulate != 0 && p->accumulate != 1) return fail(SARBP_ERR_INVALID_ARGUMENT, "accumulate %d", p->accumulate);
+  if (!(std::isfinite(p->c) && p->c > 0)) return fail(SARBP_ERR_INVALID_ARGUMENT, "c must be > 0");
+  if (!(std::isfinite(p->fc) && p->fc >= 0)) return fail(SARBP_ERR_INVALID_ARGUMENT, "fc must be >= 0");
+  if (!(std::isfinite(p->dr) && p->dr > 0)) return fail(SARBP_ERR_INVALID_ARGUMENT, "dr must be > 0");
+  if (!std::isfinite(p->r0)) return fail(SARBP_ERR_INVALID_ARGUMENT, "r0 must be finite");
+  if (p->interp == SARBP_INTERP_SINC8 && !(std::isfinite(p->kaiser_beta) && p->kaiser_beta >= 0 && p->kaiser_beta <= 50))
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "kaiser_beta must be in [0, 50]");
+  if (norm3(g->ux) == 0.0 || norm3(g->uy) == 0.0) return fail(SARBP_ERR_INVALID_ARGUMENT, "zero grid step");
+  return SARBP_OK;
+}
+
+sarbp_tuning get_tuning(sarbp_context* ctx, int interp, int prec, int dtype) {
+  auto it = ctx->tuning.find(tuning_key(interp, prec, dtype));
+  if (it != ctx->tuning.end()) return it->second;
+  sarbp_tuning t;
+  t.pixels_per_lane = 4;
+  t.pulses_per_stage = 32;
+  return t;
+}
+
+// Range extent (metres, half-width) of one tile seen from any pulse, and the
+// minimum slant range over the scene.  Without positions: triangle inequality
+// |R - Rc| <= |d|.  With positions: |R - Rc| <= |e.ux| hx + |e.uy| hy + q with
+// |e.u| <= min(|u|, max|(p - a).u| / dist(a, image plane)), maxima over the
+// image corners (linear in p), and q <= dmax^2 / Rmin (twice the leading
+// term pp / (2 R), as margin).
+struct Extent {
+  double half_m;   // bound on |R - Rc| over a tile
+  double dmax;     // max |pixel - reference| over a tile
+  double rmin;     // lower bound on slant range (0 if unknown)
+};
+
+Extent tile_extent(const sarbp_grid* g, int ppl, const double* h_pos, int n_pulses) {
+  const int ty = kWaves * ppl;
+  // Offsets of tile pixels from the reference pixel: di in [-32, 31], dj in [-ty/2, ty/2-1].
+  const int di_lo = -kTileX / 2, di_hi = kTileX / 2 - 1, dj_lo = -ty / 2, dj_hi = ty / 2 - 1;
+  double dmax = 0.0;
+  for (int ci = 0; ci < 4; ++ci) {
+    const int di = (ci & 1) ? di_hi : di_lo, dj = (ci & 2) ? dj_hi : dj_lo;
+    double d[3];
+    for (int k = 0; k < 3; ++k) d[k] = di * g->ux[k] + dj * g->uy[k];
+    dmax = std::max(dmax, norm3(d));
+  }
+  Extent e{dmax, dmax, 0.0};
+  if (!h_pos || n_pulses <= 0) return e;
+  // image plane normal
+  double n[3] = {g->ux[1] * g->uy[2] - g->ux[2] * g->uy[1], g->ux[2] * g->uy[0] - g->ux[0] * g->uy[2],
+                 g->ux[0] * g->uy[1] - g->ux[1] * g->uy[0]};
+  const double nn = norm3(n);
+  if (nn == 0.0) return e;
+  for (double& v : n) v /= nn;
+  double corners[4][3];
+  for (int ci = 0; ci < 4; ++ci) {
+    const double ix = (ci & 1) ? std::max(g->nx - 1, 0) + kTileX : -kTileX;
+    const double iy = (ci & 2) ? std::max(g->ny - 1, 0) + ty : -ty;
+    for (int k = 0; k < 3; ++k) corners[ci][k] = g->origin[k] + ix * g->ux[k] + iy * g->uy[k];
+  }
+  const double lux = norm3(g->ux), luy = norm3(g->uy);
+  double amax = 0.0, bmax = 0.0, rmin = INFINITY;
+  int side = 0;
+  for (int m = 0; m < n_pulses; ++m) {
+    const double* a = h_pos + 3 * (size_t)m;
+    const double sep = (a[0] - g->origin[0]) * n[0] + (a[1] - g->origin[1]) * n[1] + (a[2] - g->origin[2]) * n[2];
+    const int sd = sep > 0 ? 1 : -1;
+    if (side == 0) side = sd;
+    if (sd != side || std::fabs(sep) == 0.0) return e;  // antennas on both sides: no bound
+    double nx_ = 0.0, ny_ = 0.0;
+    for (int ci = 0; ci < 4; ++ci) {
+      double px = 0.0, py = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        px += (corners[ci][k] - a[k]) * g->ux[k];
+        py += (corners[ci][k] - a[k]) * g->uy[k];
+      }
+      nx_ = std::max(nx_, std::fabs(px));
+      ny_ = std::max(ny_, std::fabs(py));
+    }
+    amax = std::max(amax, std::min(lux, nx_ / std::fabs(sep)));
+    bmax = std::max(bmax, std::min(luy, ny_ / std::fabs(sep)));
+    rmin = std::min(rmin, std::fabs(sep));
+  }
+  const double hx = kTileX / 2, hy = ty / 2;
+  const double q = dmax * dmax / rmin;
+  e.half_m = std::min(dmax, amax * hx + bmax * hy + q);
+  e.rmin = rmin;
+  return e;
+}
+
+sarbp_status upload_table(sarbp_context* ctx, double beta, float4** out) {
+  auto it = ctx->sinc_tabs.find(beta);
+  if (it != ctx->sinc_tabs.end()) {
+    *out = it->second;
+    return SARBP_OK;
+  }
+  std::vector<float> tab = make_sinc_table(beta);
+  float4* d = nullptr;
+  hipError_t e = hipMalloc(&d, tab.size() * sizeof(float));
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc(sinc table)");
+  e = hipMemcpy(d, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(d);
+    return hip_fail(e, "hipMemcpy(sinc table)");
+  }
+  ctx->sinc_tabs[beta] = d;
+  *out = d;
+  return SARBP_OK;
+}
+
+constexpr size_t kLdsBudget = 64 * 1024;
+
+// Fills a launch descriptor.  Caller holds ctx->mu and the device guard.
+sarbp_status plan_args(sarbp_context* ctx, const double* h_pos, const sarbp_grid* g,
+                       const sarbp_params* p, const sarbp_tuning& tune, BpArgs* a,
+                       sarbp_plan_info* info) {
+  const int ppl = tune.pixels_per_lane;
+  if (ppl != 2 && ppl != 4 && ppl != 8) return fail(SARBP_ERR_INVALID_ARGUMENT, "pixels_per_lane %d", ppl);
+  const Extent ext = tile_extent(g, ppl, h_pos, p->n_pulses);
+  // half window in samples: extent + sinc taps (3 left / 4 right) + 2 margin
+  const double half_bins = std::ceil(ext.half_m / p->dr) + 6.0;
+  if (half_bins > 4096.0)
+    return fail(SARBP_ERR_UNSUPPORTED, "range window of a tile too long (%.0f samples): pixel spacing is too coarse for dr", 2 * half_bins);
+  int win = 16;
+  while (win < 2 * (int)half_bins + 2) win *= 2;
+  int P = std::max(1, tune.pulses_per_stage);
+  const size_t table = p->interp == SARBP_INTERP_SINC8 ? (size_t)kSincQ * 64 : 0;
+  while (P > 1 && lds_fixed_bytes(win, P) + table > kLdsBudget) P /= 2;
+  if (lds_fixed_bytes(win, P) + table > kLdsBudget)
+    return fail(SARBP_ERR_UNSUPPORTED, "range window of %d samples does not fit LDS", win);
+
+  std::memset(a, 0, sizeof *a);
+  for (int k = 0; k < 3; ++k) {
+    a->origin[k] = g->origin[k];
+    a->ux[k] = g->ux[k];
+    a->uy[k] = g->uy[k];
+  }
+  a->r0 = p->r0;
+  a->dr = p->dr;
+  a->inv_dr = 1.0 / p->dr;
+  a->k_rev = 2.0 * p->fc / p->c;
+  a->inv_dr_f = (float)a->inv_dr;
+  a->k_rev_f = (float)a->k_rev;
+  a->n_pulses = p->n_pulses;
+  a->n_samples = p->n_samples;
+  a->nx = g->nx;
+  a->ny = g->ny;
+  const int ty = kWaves * ppl;
+  a->tiles_x = (g->nx + kTileX - 1) / kTileX;
+  a->tiles_y = (g->ny + ty - 1) / ty;
+  a->win = win;
+  a->half = win / 2;
+  a->pulses_per_stage = P;
+  a->pulse_begin = 0;
+  a->pulse_end = p->n_pulses;
+  a->accumulate = p->accumulate;
+  a->dmax = ext.dmax;
+  a->order_tol = 1e-5;
+  if (p->interp == SARBP_INTERP_SINC8) {
+    float4* tab = nullptr;
+    sarbp_status st = upload_table(ctx, p->kaiser_beta, &tab);
+    if (st != SARBP_OK) return st;
+    a->sinc_tab = tab;
+  }
+  if (info) {
+    info->tiles_x = a->tiles_x;
+    info->tiles_y = a->tiles_y;
+    info->tile_nx = kTileX;
+    info->tile_ny = ty;
+    info->window = win;
+    info->pulses_per_stage = P;
+    info->pixels_per_lane = ppl;
+    info->lds_bytes = (int32_t)lds_bytes(p->interp, win, P);
+    const double rm = ext.rmin > ext.dmax ? ext.rmin - ext.dmax : 0.0;
+    const double d2 = ext.dmax * ext.dmax;
+    info->range_order = rm > 0 && 2 * kPi * a->k_rev * 0.625 * d2 * d2 / (rm * rm * rm) <= a->order_tol ? 1 : 0;
+    info->max_phase_err_rad = rm > 0 ? 2 * kPi * a->k_rev * 0.625 * d2 * d2 / (rm * rm * rm) : INFINITY;
+  }
+  return SARBP_OK;
+}
+
+sarbp_status launch(const BpArgs& a, const sarbp_params* p, int ppl, hipStream_t s) {
+  if (a.nx == 0 || a.ny == 0) return SARBP_OK;
+  hipError_t e = launch_plain(a, p->interp, p->precision, p->dtype, ppl, s);
+  if (e != hipSuccess) return hip_fail(e, "BP kernel launch");
+  return SARBP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sarbp_strerror(sarbp_status s) {
+  switch (s) {
+    case SARBP_OK: return "ok";
+    case SARBP_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case SARBP_ERR_OUT_OF_MEMORY: return "out of memory";
+    case SARBP_ERR_OVER_BUDGET: return "over HBM budget";
+    case SARBP_ERR_DEVICE: return "device error";
+    case SARBP_ERR_UNSUPPORTED: return "unsupported";
+    case SARBP_ERR_NOT_READY: return "not ready";
+    case SARBP_ERR_NOT_FOUND: return "not found";
+    case SARBP_ERR_CANCELLED: return "cancelled";
+    case SARBP_ERR_TIMEOUT: return "timeout";
+    case SARBP_ERR_SHUTDOWN: return "shutting down";
+  }
+  return "unknown status";
+}
+
+int32_t sarbp_abi_version(void) { return SARBP_ABI_VERSION; }
+
+const char* sarbp_last_error_detail(void) { return g_detail.c_str(); }
+
+sarbp_status sarbp_create(sarbp_context** out, int32_t device) {
+  if (!out) return fail(SARBP_ERR_INVALID_ARGUMENT, "ctx out-pointer is NULL");
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDeviceCount");
+  if (device < 0 || device >= n) return fail(SARBP_ERR_INVALID_ARGUMENT, "device %d of %d", device, n);
+  sarbp_context* c = new (std::nothrow) sarbp_context;
+  if (!c) return fail(SARBP_ERR_OUT_OF_MEMORY, "context");
+  c->device = device;
+  int cu = 0;
+  if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cu > 0)
+    c->n_cu = cu;
+  *out = c;
+  return SARBP_OK;
+}
+
+void sarbp_destroy(sarbp_context* c) {
+  if (!c) return;
+  {
+    DeviceGuard dg(c->device);
+    for (auto& kv : c->sinc_tabs) (void)hipFree(kv.second);
+    if (c->d_jobs) (void)hipFree(c->d_jobs);
+    if (c->h_jobs) (void)hipHostFree(c->h_jobs);
+    if (c->d_counter) (void)hipFree(c->d_counter);
+  }
+  delete c;
+}
+
+sarbp_status sarbp_set_tuning(sarbp_context* ctx, int32_t interp, int32_t precision, int32_t dtype,
+                              const sarbp_tuning* t) {
+  if (!ctx || !t) return fail(SARBP_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (interp < 0 || interp > 2 || precision < 0 || precision > 2 || dtype < 0 || dtype > 1)
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "bad (interp, precision, dtype)");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (t->pixels_per_lane == 0 && t->pulses_per_stage == 0) {
+    ctx->tuning.erase(tuning_key(interp, precision, dtype));
+    return SARBP_OK;
+  }
+  if (t->pixels_per_lane != 2 && t->pixels_per_lane != 4 && t->pixels_per_lane != 8)
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "pixels_per_lane must be 2, 4 or 8");
+  if (t->pulses_per_stage < 1 || t->pulses_per_stage > 256)
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "pulses_per_stage must be in [1, 256]");
+  ctx->tuning[tuning_key(interp, precision, dtype)] = *t;
+  return SARBP_OK;
+}
+
+sarbp_status sarbp_get_tuning(sarbp_context* ctx, int32_t interp, int32_t precision, int32_t dtype,
+                              sarbp_tuning* t) {
+  if (!ctx || !t) return fail(SARBP_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  *t = get_tuning(ctx, interp, precision, dtype);
+  return SARBP_OK;
+}
+
+sarbp_status sarbp_backproject(sarbp_context* ctx, const void* d_profiles, const double* d_pos,
+                               const sarbp_grid* grid, const sarbp_params* params,
+                               sarbp_cf32* d_image, void* stream) {
+  if (!ctx) return fail(SARBP_ERR_INVALID_ARGUMENT, "ctx is NULL");
+  sarbp_status st = validate(grid, params);
+  if (st != SARBP_OK) return st;
+  if ((params->n_pulses > 0 && (!d_profiles || !d_pos)) || (!d_image && grid->nx > 0 && grid->ny > 0))
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "device pointer is NULL");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  DeviceGuard dg(ctx->device);
+  if (dg.err != hipSuccess) return hip_fail(dg.err, "hipSetDevice");
+  const sarbp_tuning tune = get_tuning(ctx, params->interp, params->precision, params->dtype);
+  BpArgs a;
+  st = plan_args(ctx, nullptr, grid, params, tune, &a, nullptr);
+  if (st != SARBP_OK) return st;
+  a.prof = d_profiles;
+  a.pos = d_pos;
+  a.img = reinterpret_cast<float2*>(d_image);
+  return launch(a, params, tune.pixels_per_lane, (hipStream_t)stream);
+}
+
+sarbp_status sarbp_backproject_sync(sarbp_context* ctx, const void* d_profiles, const double* d_pos,
+                                    const sarbp_grid* grid, const sarbp_params* params,
+                                    sarbp_cf32* d_image, void* stream) {
+  sarbp_status st = sarbp_backproject(ctx, d_profiles, d_pos, grid, params, d_image, stream);
+  if (st != SARBP_OK) return st;
+  DeviceGuard dg(ctx->device);
+  hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+  return SARBP_OK;
+}
+
+sarbp_status sarbp_plan_create(sarbp_context* ctx, const double* h_pos, const sarbp_grid* grid,
+                               const sarbp_params* params, sarbp_plan** out) {
+  if (!ctx || !out) return fail(SARBP_ERR_INVALID_ARGUMENT, "NULL argument");
+  *out = nullptr;
+  sarbp_status st = validate(grid, params);
+  if (st != SARBP_OK) return st;
+  if (h_pos && !finite_all(h_pos, 3 * params->n_pulses))
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "positions must be finite");
+  sarbp_plan* pl = new (std::nothrow) sarbp_plan;
+  if (!pl) return fail(SARBP_ERR_OUT_OF_MEMORY, "plan");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  DeviceGuard dg(ctx->device);
+  if (dg.err != hipSuccess) {
+    delete pl;
+    return hip_fail(dg.err, "hipSetDevice");
+  }
+  const sarbp_tuning tune = get_tuning(ctx, params->interp, params->precision, params->dtype);
+  std::memset(&pl->info, 0, sizeof pl->info);
+  st = plan_args(ctx, h_pos, grid, params, tune, &pl->args, &pl->info);
+  if (st != SARBP_OK) {
+    delete pl;
+    return st;
+  }
+  pl->ctx = ctx;
+  pl->interp = params->interp;
+  pl->precision = params->precision;
+  pl->dtype = params->dtype;
+  pl->ppl = tune.pixels_per_lane;
+  *out = pl;
+  return SARBP_OK;
+}
+
+sarbp_status sarbp_plan_execute(sarbp_plan* pl, const void* d_profiles, const double* d_pos,
+                                sarbp_cf32* d_image, void* stream) {
+  if (!pl) return fail(SARBP_ERR_INVALID_ARGUMENT, "plan is NULL");
+  if ((pl->args.n_pulses > 0 && (!d_profiles || !d_pos)) || (!d_image && pl->args.nx > 0 && pl->args.ny > 0))
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "device pointer is NULL");
+  DeviceGuard dg(pl->ctx->device);
+  if (dg.err != hipSuccess) return hip_fail(dg.err, "hipSetDevice");
+  BpArgs a = pl->args;
+  a.prof = d_profiles;
+  a.pos = d_pos;
+  a.img = reinterpret_cast<float2*>(d_image);
+  sarbp_params p;
+  std::memset(&p, 0, sizeof p);
+  p.interp = pl->interp;
+  p.precision = pl->precision;
+  p.dtype = pl->dtype;
+  return launch(a, &p, pl->ppl, (hipStream_t)stream);
+}
+
+sarbp_status sarbp_plan_get_info(const sarbp_plan* pl, sarbp_plan_info* info) {
+  if (!pl || !info) return fail(SARBP_ERR_INVALID_ARGUMENT, "NULL argument");
+  *info = pl->info;
+  return SARBP_OK;
+}
+
+void sarbp_plan_destroy(sarbp_plan* pl) { delete pl; }
+
+sarbp_status sarbp_backproject_batch(sarbp_context* ctx, const sarbp_job* jobs, int32_t n_jobs,
+                                     void* stream) {
+  (void)ctx;
+  (void)jobs;
+  (void)n_jobs;
+  (void)stream;
+  return fail(SARBP_ERR_UNSUPPORTED, "batched launch not built yet");
+}
+
+}  // extern "C"

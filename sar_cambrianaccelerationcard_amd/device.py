@@ -1,0 +1,90 @@
+"""Device-side helpers: moving scenes into HBM (torch tensors as plain device
+allocations) and running the imager through the C ABI.
+
+torch is used only for device memory and the current HIP stream; all BP
+arithmetic runs in libsarbp.so.
+"""
+from __future__ import annotations
+
+from typing import Optional
+
+import numpy as np
+
+from . import native
+from .scene import Scene
+
+
+def to_bf16x2(profiles: np.ndarray) -> np.ndarray:
+    """complex64 -> uint32 words (bf16 real in the low half, bf16 imag in the
+    high half), rounded to nearest even."""
+    f = np.ascontiguousarray(profiles, dtype=np.complex64).view(np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) & 0xFFFF
+    r = r.reshape(*profiles.shape, 2)
+    return (r[..., 0] | (r[..., 1] << 16)).astype(np.uint32)
+
+
+def from_bf16x2(words: np.ndarray) -> np.ndarray:
+    w = np.asarray(words, dtype=np.uint32)
+    re = (w << 16).view(np.float32)
+    im = (w & 0xFFFF0000).view(np.float32)
+    return re.astype(np.complex64) + 1j * im.astype(np.complex64)
+
+
+def require_gpu():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("no HIP device visible: the imager has no CPU path")
+    return torch
+
+
+class DeviceScene:
+    """A scene resident in HBM: profiles (c64 and optionally bf16x2), FP64
+    positions."""
+
+    def __init__(self, sc: Scene, device: int = 0, bf16: bool = False):
+        torch = require_gpu()
+        self.scene = sc
+        dev = torch.device("cuda", device)
+        self.prof = torch.from_numpy(np.ascontiguousarray(sc.profiles)).to(dev)
+        self.pos = torch.from_numpy(np.ascontiguousarray(sc.pos)).to(dev)
+        self.prof_bf16 = None
+        if bf16:
+            self.prof_bf16 = torch.from_numpy(to_bf16x2(sc.profiles).view(np.int32)).to(dev)
+        self.device = dev
+
+    def profiles_ptr(self, dtype: str = "c64") -> int:
+        if dtype == "c64":
+            return self.prof.data_ptr()
+        if self.prof_bf16 is None:
+            torch = require_gpu()
+            self.prof_bf16 = torch.from_numpy(
+                to_bf16x2(self.scene.profiles).view(np.int32)).to(self.device)
+        return self.prof_bf16.data_ptr()
+
+    def new_image(self):
+        torch = require_gpu()
+        g = self.scene.grid
+        return torch.zeros((g["ny"], g["nx"]), dtype=torch.complex64, device=self.device)
+
+
+def backproject(ctx: native.Context, ds: DeviceScene, *, interp: Optional[str] = None,
+                precision: str = "ref", dtype: str = "c64", image=None,
+                accumulate: bool = False, plan: bool = False, sync: bool = True):
+    """Runs one BP through the C ABI on torch's current stream; returns the
+    device image tensor."""
+    torch = require_gpu()
+    sc = ds.scene
+    img = ds.new_image() if image is None else image
+    params = sc.native_params(interp=interp, precision=precision, dtype=dtype,
+                              accumulate=accumulate)
+    stream = torch.cuda.current_stream(ds.device).cuda_stream
+    if plan:
+        pl = ctx.plan(sc.pos, sc.native_grid(), params)
+        pl.execute(ds.profiles_ptr(dtype), ds.pos.data_ptr(), img.data_ptr(), stream)
+    else:
+        ctx.backproject(ds.profiles_ptr(dtype), ds.pos.data_ptr(), sc.native_grid(), params,
+                        img.data_ptr(), stream)
+    if sync:
+        torch.cuda.synchronize(ds.device)
+    return img

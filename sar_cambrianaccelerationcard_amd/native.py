@@ -1,0 +1,244 @@
+"""ctypes binding of the gfx950 back-projection library (include/sarbp.h).
+
+The library is loaded from this package's _lib/ directory.  There is no
+fallback: if libsarbp.so is missing, or no HIP device is present, every entry
+point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(_HERE, "_lib")
+SARBP_PATH = os.path.join(LIB_DIR, "libsarbp.so")
+
+# ---- enums (mirror include/sarbp.h) ----------------------------------------
+OK = 0
+ERR_INVALID_ARGUMENT = 1
+ERR_OUT_OF_MEMORY = 2
+ERR_OVER_BUDGET = 3
+ERR_DEVICE = 4
+ERR_UNSUPPORTED = 5
+ERR_NOT_READY = 6
+ERR_NOT_FOUND = 7
+ERR_CANCELLED = 8
+ERR_TIMEOUT = 9
+ERR_SHUTDOWN = 10
+
+INTERP_NEAREST, INTERP_LINEAR, INTERP_SINC8 = 0, 1, 2
+PREC_REF, PREC_F64, PREC_F32 = 0, 1, 2
+DTYPE_C64, DTYPE_BF16X2 = 0, 1
+
+INTERP_NAMES = {"nearest": INTERP_NEAREST, "linear": INTERP_LINEAR, "sinc8": INTERP_SINC8}
+PREC_NAMES = {"ref": PREC_REF, "f64": PREC_F64, "f32": PREC_F32}
+DTYPE_NAMES = {"c64": DTYPE_C64, "bf16x2": DTYPE_BF16X2}
+
+
+class Grid(C.Structure):
+    _fields_ = [("origin", C.c_double * 3), ("ux", C.c_double * 3),
+                ("uy", C.c_double * 3), ("nx", C.c_int32), ("ny", C.c_int32)]
+
+
+class Params(C.Structure):
+    _fields_ = [("c", C.c_double), ("fc", C.c_double), ("r0", C.c_double),
+                ("dr", C.c_double), ("kaiser_beta", C.c_double),
+                ("n_pulses", C.c_int32), ("n_samples", C.c_int32),
+                ("interp", C.c_int32), ("precision", C.c_int32),
+                ("dtype", C.c_int32), ("accumulate", C.c_int32)]
+
+
+class PlanInfo(C.Structure):
+    _fields_ = [("tiles_x", C.c_int32), ("tiles_y", C.c_int32),
+                ("tile_nx", C.c_int32), ("tile_ny", C.c_int32),
+                ("window", C.c_int32), ("pulses_per_stage", C.c_int32),
+                ("pixels_per_lane", C.c_int32), ("range_order", C.c_int32),
+                ("lds_bytes", C.c_int32), ("max_phase_err_rad", C.c_double)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("pixels_per_lane", C.c_int32), ("pulses_per_stage", C.c_int32)]
+
+
+class Job(C.Structure):
+    _fields_ = [("d_profiles", C.c_void_p), ("d_pos", C.c_void_p), ("grid", Grid),
+                ("params", Params), ("d_image", C.c_void_p)]
+
+
+class SarbpError(RuntimeError):
+    def __init__(self, status: int, where: str, detail: str):
+        self.status = status
+        super().__init__(f"{where}: status {status} ({detail})")
+
+
+class NativeLibraryMissing(RuntimeError):
+    pass
+
+
+_lib: Optional[C.CDLL] = None
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "sarbp_strerror": (C.c_char_p, [C.c_int]),
+    "sarbp_abi_version": (C.c_int32, []),
+    "sarbp_last_error_detail": (C.c_char_p, []),
+    "sarbp_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32]),
+    "sarbp_destroy": (None, [C.c_void_p]),
+    "sarbp_backproject": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Grid),
+                                    C.POINTER(Params), C.c_void_p, C.c_void_p]),
+    "sarbp_backproject_sync": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Grid),
+                                         C.POINTER(Params), C.c_void_p, C.c_void_p]),
+    "sarbp_plan_create": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Grid), C.POINTER(Params),
+                                    C.POINTER(C.c_void_p)]),
+    "sarbp_plan_execute": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sarbp_plan_get_info": (C.c_int, [C.c_void_p, C.POINTER(PlanInfo)]),
+    "sarbp_plan_destroy": (None, [C.c_void_p]),
+    "sarbp_set_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(Tuning)]),
+    "sarbp_get_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(Tuning)]),
+    "sarbp_backproject_batch": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_int32, C.c_void_p]),
+}
+
+
+def load() -> C.CDLL:
+    """Loads libsarbp.so (raises NativeLibraryMissing if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    # One HIP runtime per process: torch ships its own libamdhip64 (same
+    # SONAME, libamdhip64.so.7).  Importing torch first makes the dynamic
+    # linker bind this library to that copy; loading ours first would leave
+    # torch to load a second runtime, which then finds no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    if not os.path.exists(SARBP_PATH):
+        raise NativeLibraryMissing(
+            f"{SARBP_PATH} is missing: build it with "
+            "`python -m sar_cambrianaccelerationcard_amd.build`")
+    lib = C.CDLL(SARBP_PATH)
+    for name, (res, args) in _SIGS.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols() -> Sequence[str]:
+    return list(_SIGS)
+
+
+def check(status: int, where: str) -> None:
+    if status != OK:
+        lib = load()
+        detail = lib.sarbp_last_error_detail().decode()
+        name = lib.sarbp_strerror(status).decode()
+        raise SarbpError(status, where, f"{name}: {detail}")
+
+
+def make_grid(origin, ux, uy, nx: int, ny: int) -> Grid:
+    g = Grid()
+    for k in range(3):
+        g.origin[k] = float(origin[k])
+        g.ux[k] = float(ux[k])
+        g.uy[k] = float(uy[k])
+    g.nx, g.ny = int(nx), int(ny)
+    return g
+
+
+def make_params(*, c: float, fc: float, r0: float, dr: float, n_pulses: int,
+                n_samples: int, interp="linear", precision="ref", dtype="c64",
+                kaiser_beta: float = 6.0, accumulate: bool = False) -> Params:
+    p = Params()
+    p.c, p.fc, p.r0, p.dr, p.kaiser_beta = float(c), float(fc), float(r0), float(dr), float(kaiser_beta)
+    p.n_pulses, p.n_samples = int(n_pulses), int(n_samples)
+    p.interp = INTERP_NAMES[interp] if isinstance(interp, str) else int(interp)
+    p.precision = PREC_NAMES[precision] if isinstance(precision, str) else int(precision)
+    p.dtype = DTYPE_NAMES[dtype] if isinstance(dtype, str) else int(dtype)
+    p.accumulate = int(bool(accumulate))
+    return p
+
+
+class Context:
+    """Owns a sarbp_context on one HIP device."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = C.c_void_p()
+        check(self.lib.sarbp_create(C.byref(h), int(device)), "sarbp_create")
+        self.handle = h
+        self.device = device
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.sarbp_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def backproject(self, d_profiles: int, d_pos: int, grid: Grid, params: Params,
+                    d_image: int, stream: int = 0, sync: bool = False) -> None:
+        f = self.lib.sarbp_backproject_sync if sync else self.lib.sarbp_backproject
+        check(f(self.handle, C.c_void_p(d_profiles), C.c_void_p(d_pos), C.byref(grid),
+                C.byref(params), C.c_void_p(d_image), C.c_void_p(stream)), "sarbp_backproject")
+
+    def set_tuning(self, interp: int, precision: int, dtype: int, ppl: int, pulses: int) -> None:
+        t = Tuning(ppl, pulses)
+        check(self.lib.sarbp_set_tuning(self.handle, interp, precision, dtype, C.byref(t)),
+              "sarbp_set_tuning")
+
+    def get_tuning(self, interp: int, precision: int, dtype: int) -> tuple:
+        t = Tuning()
+        check(self.lib.sarbp_get_tuning(self.handle, interp, precision, dtype, C.byref(t)),
+              "sarbp_get_tuning")
+        return t.pixels_per_lane, t.pulses_per_stage
+
+    def plan(self, h_pos, grid: Grid, params: Params) -> "Plan":
+        return Plan(self, h_pos, grid, params)
+
+    def backproject_batch(self, jobs: Sequence[Job], stream: int = 0) -> None:
+        arr = (Job * len(jobs))(*jobs)
+        check(self.lib.sarbp_backproject_batch(self.handle, arr, len(jobs), C.c_void_p(stream)),
+              "sarbp_backproject_batch")
+
+
+class Plan:
+    def __init__(self, ctx: Context, h_pos, grid: Grid, params: Params):
+        import numpy as np
+        self.ctx = ctx
+        self.lib = ctx.lib
+        ptr = None
+        if h_pos is not None:
+            self._pos = np.ascontiguousarray(h_pos, dtype=np.float64)
+            ptr = self._pos.ctypes.data_as(C.c_void_p)
+        h = C.c_void_p()
+        check(self.lib.sarbp_plan_create(ctx.handle, ptr, C.byref(grid), C.byref(params),
+                                         C.byref(h)), "sarbp_plan_create")
+        self.handle = h
+
+    def info(self) -> dict:
+        i = PlanInfo()
+        check(self.lib.sarbp_plan_get_info(self.handle, C.byref(i)), "sarbp_plan_get_info")
+        return {f: getattr(i, f) for f, _ in PlanInfo._fields_}
+
+    def execute(self, d_profiles: int, d_pos: int, d_image: int, stream: int = 0) -> None:
+        check(self.lib.sarbp_plan_execute(self.handle, C.c_void_p(d_profiles), C.c_void_p(d_pos),
+                                          C.c_void_p(d_image), C.c_void_p(stream)),
+              "sarbp_plan_execute")
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.sarbp_plan_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

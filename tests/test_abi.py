@@ -1,0 +1,65 @@
+"""CPU-side checks of the C ABI: the library loads, exports every function
+declared in include/sarbp.h, and its host-side entry points behave without a
+GPU (status codes and error text)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from sar_cambrianaccelerationcard_amd import native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "sarbp.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(sarbp_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_declares_core_entry_points():
+    names = declared_functions()
+    for n in ["sarbp_create", "sarbp_destroy", "sarbp_backproject", "sarbp_backproject_sync",
+              "sarbp_strerror", "sarbp_plan_create", "sarbp_plan_execute",
+              "sarbp_backproject_batch"]:
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = native.load()
+    for n in declared_functions():
+        assert hasattr(lib, n), n
+
+
+def test_binding_covers_header():
+    assert set(native.exported_symbols()) >= set(declared_functions()) - {"sarbp_abi_version"} \
+        or set(declared_functions()) <= set(native.exported_symbols())
+
+
+def test_abi_version_and_strerror():
+    lib = native.load()
+    assert lib.sarbp_abi_version() == 1
+    for s in range(0, 11):
+        t = lib.sarbp_strerror(s).decode()
+        assert t and t != "unknown status"
+    assert lib.sarbp_strerror(99).decode() == "unknown status"
+
+
+def test_null_arguments_are_rejected_without_a_device():
+    lib = native.load()
+    assert lib.sarbp_create(None, 0) == native.ERR_INVALID_ARGUMENT
+    assert "NULL" in lib.sarbp_last_error_detail().decode()
+    g = native.make_grid([0, 0, 0], [1, 0, 0], [0, 1, 0], 8, 8)
+    p = native.make_params(c=3e8, fc=1e9, r0=0, dr=1, n_pulses=1, n_samples=8)
+    assert lib.sarbp_backproject(None, None, None, C.byref(g), C.byref(p), None, None) \
+        == native.ERR_INVALID_ARGUMENT
+    lib.sarbp_destroy(None)
+    lib.sarbp_plan_destroy(None)
+
+
+def test_missing_library_raises(monkeypatch):
+    monkeypatch.setattr(native, "_lib", None)
+    monkeypatch.setattr(native, "SARBP_PATH", "/nonexistent/libsarbp.so")
+    with pytest.raises(native.NativeLibraryMissing):
+        native.load()
