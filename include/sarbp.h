@@ -194,6 +194,7 @@ sarbp_status sarbp_get_tuning(sarbp_context* ctx, int32_t interp,
 typedef struct sarbp_job {
   const void* d_profiles;
   const double* d_pos;
+  const double* h_pos; /* optional host copy of the positions (tighter plan)  */
   sarbp_grid grid;
   sarbp_params params;
   sarbp_cf32* d_image;
@@ -209,6 +210,13 @@ typedef struct sarbp_job {
  */
 sarbp_status sarbp_backproject_batch(sarbp_context* ctx, const sarbp_job* jobs,
                                      int32_t n_jobs, void* stream);
+
+/* ---- diagnostics -------------------------------------------------------------- */
+
+/* The 64 coefficients the SINC8 interpolator uses for Kaiser shape `beta`:
+ * out[k * 8 + t] multiplies e^k, e = f - 1/2, in the weight of tap t (sample
+ * floor(u) - 3 + t) at fractional bin f.  Host-only, no device needed. */
+sarbp_status sarbp_sinc8_coefficients(double beta, float* out);
 
 #ifdef __cplusplus
 }

@@ -71,19 +71,34 @@ double kaiser_sinc8(double x, double beta, double inv_i0b) {
   return s * w;
 }
 
-// Table row i, tap t: (K(i/Q + 3 - t), K((i+1)/Q + 3 - t) - K(i/Q + 3 - t)).
-std::vector<float> make_sinc_table(double beta) {
-  std::vector<float> tab((size_t)kSincQ * 16);
+// Coefficients C[k * 8 + t] of e^k, e = f - 1/2, in the weight of sinc8 tap t
+// (sample i0 - 3 + t) at fractional bin f: the degree-7 polynomial that
+// interpolates K(e + 1/2 + 3 - t) at the 8 Chebyshev nodes of [-1/2, 1/2]
+// (near-minimax; max error ~2e-6 for beta = 6).
+void sinc8_poly(double beta, float* out) {
+  const int N = kSincCoef;
   const double inv_i0b = 1.0 / bessel_i0(beta);
-  for (int i = 0; i < kSincQ; ++i) {
-    for (int t = 0; t < 8; ++t) {
-      const double k0 = kaiser_sinc8((double)i / kSincQ + 3.0 - t, beta, inv_i0b);
-      const double k1 = kaiser_sinc8((double)(i + 1) / kSincQ + 3.0 - t, beta, inv_i0b);
-      tab[(size_t)i * 16 + 2 * t] = (float)k0;
-      tab[(size_t)i * 16 + 2 * t + 1] = (float)(k1 - k0);
+  // monomial coefficients of the Chebyshev polynomials T_0..T_7 in x
+  double T[kSincCoef][kSincCoef] = {};
+  T[0][0] = 1.0;
+  T[1][1] = 1.0;
+  for (int k = 2; k < N; ++k)
+    for (int j = 0; j < N; ++j) T[k][j] = (j > 0 ? 2.0 * T[k - 1][j - 1] : 0.0) - T[k - 2][j];
+  for (int t = 0; t < kSincTaps; ++t) {
+    double g[kSincCoef];
+    for (int n = 0; n < N; ++n) {
+      const double x = std::cos(kPi * (n + 0.5) / N);  // node in [-1, 1], e = x / 2
+      g[n] = kaiser_sinc8(0.5 * x + 0.5 + 3.0 - t, beta, inv_i0b);
     }
+    double mono[kSincCoef] = {};
+    for (int k = 0; k < N; ++k) {
+      double ak = 0.0;
+      for (int n = 0; n < N; ++n) ak += g[n] * std::cos(kPi * k * (n + 0.5) / N);
+      ak *= (k == 0 ? 1.0 : 2.0) / N;
+      for (int j = 0; j < N; ++j) mono[j] += ak * T[k][j];
+    }
+    for (int j = 0; j < N; ++j) out[j * kSincTaps + t] = (float)(mono[j] * std::ldexp(1.0, j));  // x = 2e
   }
-  return tab;
 }
 
 int tuning_key(int interp, int prec, int dtype) { return interp * 100 + prec * 10 + dtype; }
@@ -102,13 +117,13 @@ struct sarbp_context {
   int device = 0;
   int n_cu = 256;
   std::mutex mu;
-  std::map<double, float4*> sinc_tabs;
   std::map<int, sarbp_tuning> tuning;
   // batched launches
   JobDesc* d_jobs = nullptr;
   int32_t jobs_cap = 0;
   JobDesc* h_jobs = nullptr;  // pinned staging copy of the descriptor table
   unsigned long long* d_counter = nullptr;
+  hipEvent_t batch_done = nullptr;  // recorded after the last batched launch
 };
 
 struct sarbp_plan {
@@ -217,30 +232,10 @@ Extent tile_extent(const sarbp_grid* g, int ppl, const double* h_pos, int n_puls
   return e;
 }
 
-sarbp_status upload_table(sarbp_context* ctx, double beta, float4** out) {
-  auto it = ctx->sinc_tabs.find(beta);
-  if (it != ctx->sinc_tabs.end()) {
-    *out = it->second;
-    return SARBP_OK;
-  }
-  std::vector<float> tab = make_sinc_table(beta);
-  float4* d = nullptr;
-  hipError_t e = hipMalloc(&d, tab.size() * sizeof(float));
-  if (e != hipSuccess) return hip_fail(e, "hipMalloc(sinc table)");
-  e = hipMemcpy(d, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    (void)hipFree(d);
-    return hip_fail(e, "hipMemcpy(sinc table)");
-  }
-  ctx->sinc_tabs[beta] = d;
-  *out = d;
-  return SARBP_OK;
-}
-
 constexpr size_t kLdsBudget = 64 * 1024;
 
 // Fills a launch descriptor.  Caller holds ctx->mu and the device guard.
-sarbp_status plan_args(sarbp_context* ctx, const double* h_pos, const sarbp_grid* g,
+sarbp_status plan_args(const double* h_pos, const sarbp_grid* g,
                        const sarbp_params* p, const sarbp_tuning& tune, BpArgs* a,
                        sarbp_plan_info* info) {
   const int ppl = tune.pixels_per_lane;
@@ -250,12 +245,10 @@ sarbp_status plan_args(sarbp_context* ctx, const double* h_pos, const sarbp_grid
   const double half_bins = std::ceil(ext.half_m / p->dr) + 6.0;
   if (half_bins > 4096.0)
     return fail(SARBP_ERR_UNSUPPORTED, "range window of a tile too long (%.0f samples): pixel spacing is too coarse for dr", 2 * half_bins);
-  int win = 16;
-  while (win < 2 * (int)half_bins + 2) win *= 2;
+  const int win = ((2 * (int)half_bins + 2) + 3) & ~3;  // multiple of 4 samples
   int P = std::max(1, tune.pulses_per_stage);
-  const size_t table = p->interp == SARBP_INTERP_SINC8 ? (size_t)kSincQ * 64 : 0;
-  while (P > 1 && lds_fixed_bytes(win, P) + table > kLdsBudget) P /= 2;
-  if (lds_fixed_bytes(win, P) + table > kLdsBudget)
+  while (P > 1 && lds_layout(p->interp, win, P).total > kLdsBudget) --P;
+  if (lds_layout(p->interp, win, P).total > kLdsBudget)
     return fail(SARBP_ERR_UNSUPPORTED, "range window of %d samples does not fit LDS", win);
 
   std::memset(a, 0, sizeof *a);
@@ -285,12 +278,7 @@ sarbp_status plan_args(sarbp_context* ctx, const double* h_pos, const sarbp_grid
   a->accumulate = p->accumulate;
   a->dmax = ext.dmax;
   a->order_tol = 1e-5;
-  if (p->interp == SARBP_INTERP_SINC8) {
-    float4* tab = nullptr;
-    sarbp_status st = upload_table(ctx, p->kaiser_beta, &tab);
-    if (st != SARBP_OK) return st;
-    a->sinc_tab = tab;
-  }
+  if (p->interp == SARBP_INTERP_SINC8) sinc8_poly(p->kaiser_beta, a->sinc_poly);
   if (info) {
     info->tiles_x = a->tiles_x;
     info->tiles_y = a->tiles_y;
@@ -299,7 +287,7 @@ sarbp_status plan_args(sarbp_context* ctx, const double* h_pos, const sarbp_grid
     info->window = win;
     info->pulses_per_stage = P;
     info->pixels_per_lane = ppl;
-    info->lds_bytes = (int32_t)lds_bytes(p->interp, win, P);
+    info->lds_bytes = (int32_t)lds_layout(p->interp, win, P).total;
     const double rm = ext.rmin > ext.dmax ? ext.rmin - ext.dmax : 0.0;
     const double d2 = ext.dmax * ext.dmax;
     info->range_order = rm > 0 && 2 * kPi * a->k_rev * 0.625 * d2 * d2 / (rm * rm * rm) <= a->order_tol ? 1 : 0;
@@ -361,9 +349,12 @@ void sarbp_destroy(sarbp_context* c) {
   if (!c) return;
   {
     DeviceGuard dg(c->device);
-    for (auto& kv : c->sinc_tabs) (void)hipFree(kv.second);
     if (c->d_jobs) (void)hipFree(c->d_jobs);
     if (c->h_jobs) (void)hipHostFree(c->h_jobs);
+    if (c->batch_done) {
+      (void)hipEventSynchronize(c->batch_done);
+      (void)hipEventDestroy(c->batch_done);
+    }
     if (c->d_counter) (void)hipFree(c->d_counter);
   }
   delete c;
@@ -408,7 +399,7 @@ sarbp_status sarbp_backproject(sarbp_context* ctx, const void* d_profiles, const
   if (dg.err != hipSuccess) return hip_fail(dg.err, "hipSetDevice");
   const sarbp_tuning tune = get_tuning(ctx, params->interp, params->precision, params->dtype);
   BpArgs a;
-  st = plan_args(ctx, nullptr, grid, params, tune, &a, nullptr);
+  st = plan_args(nullptr, grid, params, tune, &a, nullptr);
   if (st != SARBP_OK) return st;
   a.prof = d_profiles;
   a.pos = d_pos;
@@ -445,7 +436,7 @@ sarbp_status sarbp_plan_create(sarbp_context* ctx, const double* h_pos, const sa
   }
   const sarbp_tuning tune = get_tuning(ctx, params->interp, params->precision, params->dtype);
   std::memset(&pl->info, 0, sizeof pl->info);
-  st = plan_args(ctx, h_pos, grid, params, tune, &pl->args, &pl->info);
+  st = plan_args(h_pos, grid, params, tune, &pl->args, &pl->info);
   if (st != SARBP_OK) {
     delete pl;
     return st;
@@ -488,11 +479,91 @@ void sarbp_plan_destroy(sarbp_plan* pl) { delete pl; }
 
 sarbp_status sarbp_backproject_batch(sarbp_context* ctx, const sarbp_job* jobs, int32_t n_jobs,
                                      void* stream) {
-  (void)ctx;
-  (void)jobs;
-  (void)n_jobs;
-  (void)stream;
-  return fail(SARBP_ERR_UNSUPPORTED, "batched launch not built yet");
+  if (!ctx || (!jobs && n_jobs > 0) || n_jobs < 0)
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "NULL context or job list");
+  if (n_jobs == 0) return SARBP_OK;
+  const sarbp_params& p0 = jobs[0].params;
+  for (int32_t j = 0; j < n_jobs; ++j) {
+    const sarbp_job& jb = jobs[j];
+    sarbp_status st = validate(&jb.grid, &jb.params);
+    if (st != SARBP_OK) return fail(st, "job %d: %s", j, g_detail.c_str());
+    if (jb.params.interp != p0.interp || jb.params.precision != p0.precision || jb.params.dtype != p0.dtype)
+      return fail(SARBP_ERR_INVALID_ARGUMENT, "job %d: interp, precision and dtype must match job 0", j);
+    if ((jb.params.n_pulses > 0 && (!jb.d_profiles || !jb.d_pos)) ||
+        (!jb.d_image && jb.grid.nx > 0 && jb.grid.ny > 0))
+      return fail(SARBP_ERR_INVALID_ARGUMENT, "job %d: device pointer is NULL", j);
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  DeviceGuard dg(ctx->device);
+  if (dg.err != hipSuccess) return hip_fail(dg.err, "hipSetDevice");
+  const sarbp_tuning tune = get_tuning(ctx, p0.interp, p0.precision, p0.dtype);
+  // Largest per-tile cost first (pulses per tile), ties by submission order:
+  // the persistent grid drains items in this order.
+  std::vector<int32_t> order((size_t)n_jobs);
+  for (int32_t j = 0; j < n_jobs; ++j) order[j] = j;
+  std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+    return jobs[x].params.n_pulses > jobs[y].params.n_pulses;
+  });
+  std::vector<JobDesc> desc;
+  desc.reserve((size_t)n_jobs);
+  int64_t items = 0;
+  size_t shm = 0;
+  for (int32_t j : order) {
+    const sarbp_job& jb = jobs[j];
+    if (jb.grid.nx == 0 || jb.grid.ny == 0) continue;
+    JobDesc d;
+    sarbp_status st = plan_args(jb.h_pos, &jb.grid, &jb.params, tune, &d.a, nullptr);
+    if (st != SARBP_OK) return fail(st, "job %d: %s", j, g_detail.c_str());
+    d.a.prof = jb.d_profiles;
+    d.a.pos = jb.d_pos;
+    d.a.img = reinterpret_cast<float2*>(jb.d_image);
+    d.item_begin = items;
+    items += (int64_t)d.a.tiles_x * d.a.tiles_y;
+    shm = std::max(shm, lds_layout(p0.interp, d.a.win, d.a.pulses_per_stage).total);
+    desc.push_back(d);
+  }
+  if (desc.empty()) return SARBP_OK;
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e;
+  // The descriptor table and counter are reused by the next batch call on
+  // this context: wait until the previous batch has consumed them.
+  if (ctx->batch_done) {
+    if ((e = hipEventSynchronize(ctx->batch_done)) != hipSuccess) return hip_fail(e, "hipEventSynchronize");
+  } else if ((e = hipEventCreateWithFlags(&ctx->batch_done, hipEventDisableTiming)) != hipSuccess) {
+    return hip_fail(e, "hipEventCreate");
+  }
+  if ((int32_t)desc.size() > ctx->jobs_cap) {
+    if (ctx->d_jobs) (void)hipFree(ctx->d_jobs);
+    if (ctx->h_jobs) (void)hipHostFree(ctx->h_jobs);
+    ctx->d_jobs = nullptr;
+    ctx->h_jobs = nullptr;
+    ctx->jobs_cap = 0;
+    const int32_t cap = std::max<int32_t>(64, (int32_t)desc.size());
+    if ((e = hipMalloc(&ctx->d_jobs, sizeof(JobDesc) * cap)) != hipSuccess) return hip_fail(e, "hipMalloc(jobs)");
+    if ((e = hipHostMalloc(&ctx->h_jobs, sizeof(JobDesc) * cap, hipHostMallocDefault)) != hipSuccess)
+      return hip_fail(e, "hipHostMalloc(jobs)");
+    ctx->jobs_cap = cap;
+  }
+  if (!ctx->d_counter && (e = hipMalloc(&ctx->d_counter, sizeof(unsigned long long))) != hipSuccess)
+    return hip_fail(e, "hipMalloc(counter)");
+  std::memcpy(ctx->h_jobs, desc.data(), sizeof(JobDesc) * desc.size());
+  if ((e = hipMemcpyAsync(ctx->d_jobs, ctx->h_jobs, sizeof(JobDesc) * desc.size(), hipMemcpyHostToDevice, s)) != hipSuccess)
+    return hip_fail(e, "hipMemcpyAsync(jobs)");
+  if ((e = reset_counter(ctx->d_counter, s)) != hipSuccess) return hip_fail(e, "counter reset");
+  const int per_cu = batch_blocks_per_cu(p0.interp, p0.precision, p0.dtype, tune.pixels_per_lane, shm);
+  const int64_t grid = std::min<int64_t>(items, (int64_t)ctx->n_cu * per_cu);
+  if ((e = launch_batch(ctx->d_jobs, (int32_t)desc.size(), items, ctx->d_counter, p0.interp, p0.precision,
+                        p0.dtype, tune.pixels_per_lane, shm, (int)grid, s)) != hipSuccess)
+    return hip_fail(e, "batched BP kernel launch");
+  if ((e = hipEventRecord(ctx->batch_done, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+  return SARBP_OK;
+}
+
+sarbp_status sarbp_sinc8_coefficients(double beta, float* out) {
+  if (!out) return fail(SARBP_ERR_INVALID_ARGUMENT, "out is NULL");
+  if (!(std::isfinite(beta) && beta >= 0 && beta <= 50)) return fail(SARBP_ERR_INVALID_ARGUMENT, "beta");
+  sinc8_poly(beta, out);
+  return SARBP_OK;
 }
 
 }  // extern "C"

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 namespace sarbp {
@@ -12,59 +13,88 @@ namespace sarbp {
 constexpr int kTileX = 64;
 constexpr int kWaves = 4;
 constexpr int kThreads = kTileX * kWaves;
-// Sinc8 weight table: kSincQ intervals over the fractional bin [0, 1); row i
-// holds, for the 8 taps t = 0..7 (sample i0 - 3 + t), the pair
-// (K(i/Q + 3 - t), K((i+1)/Q + 3 - t) - K(i/Q + 3 - t)).
-constexpr int kSincQ = 512;
+// Sinc8: the weight of tap t (sample i0 - 3 + t) at fractional bin f is the
+// degree-7 polynomial sum_k C[k][t] e^k, e = f - 1/2.
+constexpr int kSincTaps = 8;
+constexpr int kSincCoef = 8;
 // Per-pulse parameter record in LDS (8 floats or 4 doubles).
 constexpr int kParBytes = 32;
 
+enum { kNearest = 0, kLinear = 1, kSinc8 = 2 };
+enum { kRef = 0, kF64 = 1, kF32 = 2 };
+
 // Everything one plain launch needs; passed by value as the kernel argument.
 struct BpArgs {
-  const void* prof;        // [n_pulses][n_samples] c64 or bf16x2
-  const double* pos;       // [n_pulses][3]
-  float2* img;             // [ny][nx]
-  const float4* sinc_tab;  // [kSincQ][4] float4 = 8 taps x (K, dK)
+  const void* prof;   // [n_pulses][n_samples] c64 or bf16x2
+  const double* pos;  // [n_pulses][3]
+  float2* img;        // [ny][nx]
   double origin[3], ux[3], uy[3];
   double r0, dr, inv_dr, k_rev;
+  double dmax;       // max |pixel - tile reference pixel| over a tile, m
+  double order_tol;  // truncation tolerance of the range expansion, rad
   float inv_dr_f, k_rev_f;
   int32_t n_pulses, n_samples, nx, ny;
   int32_t tiles_x, tiles_y;
-  int32_t win;        // LDS window samples per pulse
-  int32_t half;       // window start = floor(ref bin) - half
+  int32_t win;   // LDS window samples per pulse
+  int32_t half;  // window start = floor(ref bin) - half
   int32_t pulses_per_stage;
   int32_t pulse_begin, pulse_end;  // pulses [begin, end) are summed
   int32_t accumulate;
-  double dmax;       // max |pixel - tile reference pixel| over a tile, m
-  double order_tol;  // truncation tolerance of the range expansion, rad
+  float sinc_poly[kSincCoef * kSincTaps];  // C[k * 8 + t]
 };
 
 // Batched (persistent) launch: one work item = one tile of one job.
 struct JobDesc {
-  BpArgs a;              // per-job launch descriptor (tiles_x/tiles_y of the job)
-  int64_t item_begin;    // first global work-item index of this job
+  BpArgs a;            // per-job launch descriptor (tiles_x/tiles_y of the job)
+  int64_t item_begin;  // first global work-item index of this job
 };
 
-// LDS layout of one workgroup: [P][win] float2 windows | [P] 32-B pulse
-// records | [P] int window starts | [P] float 2Rc | int stage order | pad to
-// 16 B | sinc table (sinc8 only).
-__host__ __device__ inline size_t lds_fixed_bytes(int win, int pulses_per_stage) {
-  size_t b = (size_t)pulses_per_stage * win * 8 + (size_t)pulses_per_stage * kParBytes +
-             (size_t)pulses_per_stage * 8 + 4;
-  return (b + 15) & ~(size_t)15;
+// LDS layout of one workgroup (byte offsets):
+//   raw   [P][win] float2         staged range windows
+//   coef  [P][win-7][8] float2    sinc8 only: polynomial coefficients D_k of
+//                                 the interpolant at each window position
+//   par   [P] 32-B pulse records
+//   start [P] int window starts, rc2 [P] float 2Rc, order int
+struct LdsLayout {
+  size_t raw, coef, par, start, rc2, order, total;
+};
+
+__host__ __device__ inline LdsLayout lds_layout(int interp, int win, int P) {
+  LdsLayout L;
+  size_t o = 0;
+  L.raw = o;
+  o += (size_t)P * win * 8;
+  L.coef = o;
+  if (interp == kSinc8) o += (size_t)P * (win - 7) * kSincCoef * 8;
+  L.par = o;
+  o += (size_t)P * kParBytes;
+  L.start = o;
+  o += (size_t)P * 4;
+  L.rc2 = o;
+  o += (size_t)P * 4;
+  L.order = o;
+  o += 4;
+  L.total = (o + 15) & ~(size_t)15;
+  return L;
 }
-size_t lds_bytes(int interp, int win, int pulses_per_stage);
 
 // Launches the plain tile kernel.  Returns hipSuccess or the launch error.
 hipError_t launch_plain(const BpArgs& a, int interp, int precision, int dtype,
                         int ppl, hipStream_t stream);
 
 // Launches the persistent batched kernel over n_jobs descriptors in device
-// memory; n_items = total tiles; counter = device int64 work counter (zeroed
-// by the caller on the stream before the launch).
+// memory; n_items = total tiles; counter = device work counter (zeroed by
+// the caller on the stream before the launch); shm = max LDS over the jobs.
 hipError_t launch_batch(const JobDesc* d_jobs, int32_t n_jobs, int64_t n_items,
                         unsigned long long* d_counter, int interp,
-                        int precision, int dtype, int ppl, int pulses_per_stage,
-                        int win, int n_blocks, hipStream_t stream);
+                        int precision, int dtype, int ppl, size_t shm,
+                        int n_blocks, hipStream_t stream);
+
+// Zeroes the batch work counter with a one-thread kernel (a kernel node, not a
+// memset node, so the sequence can be captured into a graph).
+hipError_t reset_counter(unsigned long long* d_counter, hipStream_t stream);
+
+// Workgroups of the batched kernel that fit one CU at the given LDS size.
+int batch_blocks_per_cu(int interp, int precision, int dtype, int ppl, size_t shm);
 
 }  // namespace sarbp

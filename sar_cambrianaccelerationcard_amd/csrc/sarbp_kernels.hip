@@ -10,22 +10,31 @@
 // 64 x (4*PPL) pixels: lane = x offset, each lane accumulates PPL rows.  Pulses
 // are consumed in stages of P: for every pulse of a stage one lane computes, in
 // FP64, the tile-reference geometry (below) and the start of the pulse's range
-// window; the workgroup then copies each pulse's window of `win` samples from
-// HBM into LDS with coalesced loads (zero outside the profile), and every lane
+// window; the workgroup copies each pulse's window of `win` samples from HBM
+// into LDS with coalesced loads (zero outside the profile), and every lane
 // back-projects its pixels against the staged pulses.  The pulse index is
 // wavefront-uniform, so the per-pulse record is an LDS broadcast and the
 // sample gathers hit a short, contiguous window.
 //
+// Sinc8.  Each tap weight K(f + 3 - t) of the 8-tap Kaiser sinc is, over the
+// fractional bin f in [0, 1), a degree-7 polynomial in e = f - 1/2 (host-fitted
+// coefficients C[k][t], error ~2e-6).  The interpolant at window position i0 is
+// then v(e) = sum_k D_k e^k with D_k = sum_t C[k][t] s[i0 - 3 + t].  After
+// staging, the workgroup computes D_0..D_7 for every (pulse, position) of the
+// stage into LDS once; a pixel then reads its 8 coefficients (64 B) and runs a
+// complex Horner scheme (14 FMA) -- no weight table and no per-pixel tap loop.
+//
 // Precision (SARBP_PREC_REF).  With c the tile's reference pixel, e the unit
 // vector from antenna to c, Rc = |c - antenna| and d = p - c:
-//   s = e.d,  pp = |d|^2 - s^2 >= 0,
-//   R = Rc + s + q,  q = pp / (2 (Rc + s)) + O(pp^2 / Rc^3)
+//   s = e.d,  pp = |d|^2 - s^2 >= 0,  R = Rc + s + q,  q = pp / (R + Rc + s)
 // Rc, e and the phase of the reference, frac(2 fc Rc / c), are FP64 per
 // (tile, pulse).  s, pp and q are small (metres) and FP32 carries them to
 // ~1e-7 m.  The linear phase term k.s = (k e.ux) di + (k e.uy) dj (k = 2fc/c,
 // di, dj integer pixel offsets) is evaluated with the per-step phase
 // increments reduced modulo one revolution in FP64, which is exact for
-// integer di, dj; FP32 then only ever holds a few revolutions.
+// integer di, dj; FP32 then only ever holds a few revolutions.  q is expanded
+// to first or second order in s / Rc, or evaluated with one reciprocal, as
+// the FP64 setup's truncation bound for the stage requires.
 //
 // sin/cos: the hardware v_sin_f32 / v_cos_f32 take their argument in
 // revolutions, which is what the phase already is.
@@ -33,9 +42,6 @@
 
 namespace sarbp {
 namespace {
-
-enum { kNearest = 0, kLinear = 1, kSinc8 = 2 };
-enum { kRef = 0, kF64 = 1, kF32 = 2 };
 
 __device__ __forceinline__ float sin_rev(float x) { return __builtin_amdgcn_sinf(x); }
 __device__ __forceinline__ float cos_rev(float x) { return __builtin_amdgcn_cosf(x); }
@@ -58,51 +64,49 @@ __device__ __forceinline__ float2 load_sample(const void* prof, size_t idx) {
   }
 }
 
-// Interpolates the staged window w (win samples) at window-relative bin `bin`.
-// The tap index is clamped as an integer (v_cvt_i32_f32 maps NaN to 0), so no
-// input, however malformed, addresses LDS outside the window.
-template <int INTERP>
-__device__ __forceinline__ void interp(const float2* __restrict__ w, float bin,
-                                       int lim_hi, const float4* __restrict__ tab,
-                                       float& vr, float& vi) {
-  if constexpr (INTERP == kNearest) {
-    const int i = min(max((int)(bin + 0.5f), 0), lim_hi);
-    const float2 s = w[i];
-    vr = s.x;
-    vi = s.y;
-  } else if constexpr (INTERP == kLinear) {
-    const int i0 = min(max((int)bin, 0), lim_hi);
-    const float f = __builtin_amdgcn_fractf(bin);
-    const float2 s0 = w[i0], s1 = w[i0 + 1];
-    vr = fmaf(f, s1.x - s0.x, s0.x);
-    vi = fmaf(f, s1.y - s0.y, s0.y);
-  } else {
-    const int i0 = min(max((int)bin, 3), lim_hi);
-    const float f = __builtin_amdgcn_fractf(bin);
-    const float x = f * (float)kSincQ;
-    const int fi = min(max((int)x, 0), kSincQ - 1);
-    const float a = x - (float)fi;
-    const float4* tr = tab + 4 * fi;
-    const float2* s = w + (i0 - 3);
-    vr = 0.0f;
-    vi = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const float4 t = tr[k];
-      const float w0 = fmaf(t.y, a, t.x), w1 = fmaf(t.w, a, t.z);
-      const float2 s0 = s[2 * k], s1 = s[2 * k + 1];
-      vr = fmaf(w0, s0.x, vr);
-      vi = fmaf(w0, s0.y, vi);
-      vr = fmaf(w1, s1.x, vr);
-      vi = fmaf(w1, s1.y, vi);
-    }
-  }
-}
-
 // Largest first-tap index that keeps every tap inside a window of `win`.
 template <int INTERP>
 __host__ __device__ constexpr int bin_hi(int win) {
   return INTERP == kSinc8 ? win - 5 : (INTERP == kLinear ? win - 2 : win - 1);
+}
+
+// Interpolates pulse t of the stage at window-relative bin `bin`.  The tap
+// index is clamped as an integer (v_cvt_i32_f32 maps NaN to 0), so no input,
+// however malformed, addresses LDS outside the pulse's window.
+template <int INTERP>
+__device__ __forceinline__ void interp(const float2* __restrict__ raw,
+                                       const float4* __restrict__ coef, int t, int W,
+                                       float bin, int lim, float& vr, float& vi) {
+  if constexpr (INTERP == kNearest) {
+    const float2 s = raw[t * W + min(max((int)(bin + 0.5f), 0), lim)];
+    vr = s.x;
+    vi = s.y;
+  } else if constexpr (INTERP == kLinear) {
+    const float2* w = raw + t * W + min(max((int)bin, 0), lim);
+    const float f = __builtin_amdgcn_fractf(bin);
+    const float2 s0 = w[0], s1 = w[1];
+    vr = fmaf(f, s1.x - s0.x, s0.x);
+    vi = fmaf(f, s1.y - s0.y, s0.y);
+  } else {
+    const int j = min(max((int)bin, 3), lim) - 3;
+    const float e = __builtin_amdgcn_fractf(bin) - 0.5f;
+    const float4* d = coef + ((size_t)t * (W - 7) + j) * 4;
+    const float4 d67 = d[3], d45 = d[2], d23 = d[1], d01 = d[0];
+    vr = fmaf(d67.z, e, d67.x);
+    vi = fmaf(d67.w, e, d67.y);
+    vr = fmaf(vr, e, d45.z);
+    vi = fmaf(vi, e, d45.w);
+    vr = fmaf(vr, e, d45.x);
+    vi = fmaf(vi, e, d45.y);
+    vr = fmaf(vr, e, d23.z);
+    vi = fmaf(vi, e, d23.w);
+    vr = fmaf(vr, e, d23.x);
+    vi = fmaf(vi, e, d23.y);
+    vr = fmaf(vr, e, d01.z);
+    vi = fmaf(vi, e, d01.w);
+    vr = fmaf(vr, e, d01.x);
+    vi = fmaf(vi, e, d01.y);
+  }
 }
 
 // MAC of one interpolated sample rotated by exp(+j 2 pi ph).
@@ -120,16 +124,16 @@ __device__ __forceinline__ void rotate_acc(float vr, float vi, float ph, float& 
 //   2: q = pp (g - 2g^2 s + 4g^3 s^2 - g^3 pp)
 //   3: q = pp / (2 (Rc + s) + q1), q1 the order-1 value  (exact up to rounding:
 //      R - Rc - s = pp / (R + Rc + s) identically)
-// Per-pulse LDS record: {alpha, beta, pa, pb | c0, g, 2g^2, bin(ref)}; the
-// order-3 path also reads 2 Rc from the start-array neighbour (rc2).
+// Per-pulse LDS record: {alpha, beta, pa, pb | c0, g, 2g^2, bin(ref)}.
 template <int INTERP, int PPL, int ORD>
-__device__ __forceinline__ void stage_ref(const float4* __restrict__ par, const float* __restrict__ rc2,
-                                          const float2* __restrict__ win, int W, int cnt,
-                                          const float4* __restrict__ tab, float fdi,
-                                          const float* fdj, const float* dd, float inv_dr,
-                                          float k_rev, int lim, float* acc_r, float* acc_i) {
+__device__ __forceinline__ void stage_ref(const float4* __restrict__ par,
+                                          const float* __restrict__ rc2,
+                                          const float2* __restrict__ raw,
+                                          const float4* __restrict__ coef, int W, int cnt,
+                                          float fdi, const float* fdj, const float* dd,
+                                          float inv_dr, float k_rev, int lim, float* acc_r,
+                                          float* acc_i) {
   for (int t = 0; t < cnt; ++t) {
-    const float2* w = win + t * W;
     const float4 p0 = par[2 * t], p1 = par[2 * t + 1];
     const float s_x = p0.x * fdi;
     const float ph_x = fmaf(p0.z, fdi, p1.x);
@@ -153,7 +157,7 @@ __device__ __forceinline__ void stage_ref(const float4* __restrict__ par, const 
       }
       const float bin = fmaf(s + q, inv_dr, p1.w);
       float vr, vi;
-      interp<INTERP>(w, bin, lim, tab, vr, vi);
+      interp<INTERP>(raw, coef, t, W, bin, lim, vr, vi);
       float ph = fmaf(p0.w, fdj[n], ph_x);
       ph = fmaf(k_rev, q, ph);
       rotate_acc(vr, vi, ph, acc_r[n], acc_i[n]);
@@ -170,13 +174,15 @@ __device__ __forceinline__ void bp_tile(const BpArgs& a, int tile, char* lds) {
   const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
   const int x0 = tx * kTileX, y0 = ty * TY;
   const int P = a.pulses_per_stage, W = a.win;
+  const int NPOS = W - 7;
 
-  float2* lds_win = reinterpret_cast<float2*>(lds);
-  float* lds_par = reinterpret_cast<float*>(lds + (size_t)P * W * 8);
-  int* lds_start = reinterpret_cast<int*>(lds + (size_t)P * W * 8 + (size_t)P * kParBytes);
-  float* lds_rc2 = reinterpret_cast<float*>(lds_start + P);
-  int* lds_order = reinterpret_cast<int*>(lds_rc2 + P);
-  const float4* lds_tab = reinterpret_cast<const float4*>(lds + lds_fixed_bytes(W, P));
+  const LdsLayout L = lds_layout(INTERP, W, P);
+  float2* lds_raw = reinterpret_cast<float2*>(lds + L.raw);
+  float4* lds_coef = reinterpret_cast<float4*>(lds + L.coef);
+  float* lds_par = reinterpret_cast<float*>(lds + L.par);
+  int* lds_start = reinterpret_cast<int*>(lds + L.start);
+  float* lds_rc2 = reinterpret_cast<float*>(lds + L.rc2);
+  int* lds_order = reinterpret_cast<int*>(lds + L.order);
 
   // Tile reference pixel (integer pixel near the tile centre).
   const int ic = x0 + kTileX / 2, jc = y0 + TY / 2;
@@ -267,6 +273,9 @@ __device__ __forceinline__ void bp_tile(const BpArgs& a, int tile, char* lds) {
       }
     }
     __syncthreads();
+    // Range-expansion order of this stage: read by every thread here, reset by
+    // thread 0 after the next barrier, before any setup of the next stage.
+    const int order = __builtin_amdgcn_readfirstlane(*lds_order);
     // ---- stage the range windows ----
     for (int i = tid; i < cnt * W; i += kThreads) {
       const int t = i / W, k = i - t * W;
@@ -274,24 +283,52 @@ __device__ __forceinline__ void bp_tile(const BpArgs& a, int tile, char* lds) {
       float2 v = make_float2(0.0f, 0.0f);
       if ((unsigned)g < (unsigned)a.n_samples)
         v = load_sample<DT>(a.prof, (size_t)(m0 + t) * a.n_samples + g);
-      lds_win[i] = v;
+      lds_raw[i] = v;
     }
-    const int order = __builtin_amdgcn_readfirstlane(*lds_order);
     __syncthreads();
+    if (tid == 0) *lds_order = 1;
+    // ---- sinc8: interpolant coefficients for every (pulse, position) ----
+    if constexpr (INTERP == kSinc8) {
+      for (int i = tid; i < cnt * NPOS; i += kThreads) {
+        const int t = i / NPOS, j = i - t * NPOS;
+        const float2* r = lds_raw + t * W + j;
+        float sr[kSincTaps], si[kSincTaps];
+#pragma unroll
+        for (int k = 0; k < kSincTaps; ++k) {
+          const float2 v = r[k];
+          sr[k] = v.x;
+          si[k] = v.y;
+        }
+        float4* d = lds_coef + (size_t)i * 4;
+#pragma unroll
+        for (int k = 0; k < kSincCoef; k += 2) {
+          float a0 = 0.0f, b0 = 0.0f, a1 = 0.0f, b1 = 0.0f;
+#pragma unroll
+          for (int tp = 0; tp < kSincTaps; ++tp) {
+            const float c0 = a.sinc_poly[k * kSincTaps + tp];
+            const float c1 = a.sinc_poly[(k + 1) * kSincTaps + tp];
+            a0 = fmaf(c0, sr[tp], a0);
+            b0 = fmaf(c0, si[tp], b0);
+            a1 = fmaf(c1, sr[tp], a1);
+            b1 = fmaf(c1, si[tp], b1);
+          }
+          d[k / 2] = make_float4(a0, b0, a1, b1);
+        }
+      }
+      __syncthreads();
+    }
     // ---- back-project the staged pulses ----
     if constexpr (PREC == kRef) {
-      if (tid == 0) *lds_order = 1;  // all reads of this stage's order are done
       const float4* par = reinterpret_cast<const float4*>(lds_par);
       if (order == 1)
-        stage_ref<INTERP, PPL, 1>(par, lds_rc2, lds_win, W, cnt, lds_tab, fdi, fdj, dd, inv_dr, k_rev, lim, acc_r, acc_i);
+        stage_ref<INTERP, PPL, 1>(par, lds_rc2, lds_raw, lds_coef, W, cnt, fdi, fdj, dd, inv_dr, k_rev, lim, acc_r, acc_i);
       else if (order == 2)
-        stage_ref<INTERP, PPL, 2>(par, lds_rc2, lds_win, W, cnt, lds_tab, fdi, fdj, dd, inv_dr, k_rev, lim, acc_r, acc_i);
+        stage_ref<INTERP, PPL, 2>(par, lds_rc2, lds_raw, lds_coef, W, cnt, fdi, fdj, dd, inv_dr, k_rev, lim, acc_r, acc_i);
       else
-        stage_ref<INTERP, PPL, 3>(par, lds_rc2, lds_win, W, cnt, lds_tab, fdi, fdj, dd, inv_dr, k_rev, lim, acc_r, acc_i);
+        stage_ref<INTERP, PPL, 3>(par, lds_rc2, lds_raw, lds_coef, W, cnt, fdi, fdj, dd, inv_dr, k_rev, lim, acc_r, acc_i);
     } else {
       (void)order;
       for (int t = 0; t < cnt; ++t) {
-        const float2* w = lds_win + t * W;
         if constexpr (PREC == kF64) {
           const double* pd = reinterpret_cast<const double*>(lds_par + 8 * t);
           const double ax = pd[0], ay = pd[1], az = pd[2], rs = pd[3];
@@ -303,7 +340,7 @@ __device__ __forceinline__ void bp_tile(const BpArgs& a, int tile, char* lds) {
             double ph = a.k_rev * R;
             ph -= floor(ph);
             float vr, vi;
-            interp<INTERP>(w, bin, lim, lds_tab, vr, vi);
+            interp<INTERP>(lds_raw, lds_coef, t, W, bin, lim, vr, vi);
             rotate_acc(vr, vi, (float)ph, acc_r[n], acc_i[n]);
           }
         } else {
@@ -316,7 +353,7 @@ __device__ __forceinline__ void bp_tile(const BpArgs& a, int tile, char* lds) {
             const float bin = (R - p0.w) * inv_dr;
             const float ph = __builtin_amdgcn_fractf(R * k_rev);
             float vr, vi;
-            interp<INTERP>(w, bin, lim, lds_tab, vr, vi);
+            interp<INTERP>(lds_raw, lds_coef, t, W, bin, lim, vr, vi);
             rotate_acc(vr, vi, ph, acc_r[n], acc_i[n]);
           }
         }
@@ -344,147 +381,154 @@ __device__ __forceinline__ void bp_tile(const BpArgs& a, int tile, char* lds) {
   }
 }
 
-template <int INTERP>
-__device__ __forceinline__ void load_table(const BpArgs& a, char* lds) {
-  if constexpr (INTERP == kSinc8) {
-    float4* t = reinterpret_cast<float4*>(lds + lds_fixed_bytes(a.win, a.pulses_per_stage));
-    for (int i = threadIdx.x; i < 4 * kSincQ; i += kThreads) t[i] = a.sinc_tab[i];
-  }
-}
-
 template <int INTERP, int PREC, int DT, int PPL>
 __global__ __launch_bounds__(kThreads) void bp_plain_kernel(BpArgs a) {
   extern __shared__ __align__(16) char lds[];
-  load_table<INTERP>(a, lds);
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   bp_tile<INTERP, PREC, DT, PPL>(a, tile, lds);
 }
 
+// Persistent batched kernel: workgroups claim (job, tile) items from a device
+// counter until the list is exhausted; every wave of every workgroup reaches
+// the exit once the counter passes n_items.
 template <int INTERP, int PREC, int DT, int PPL>
 __global__ __launch_bounds__(kThreads) void bp_batch_kernel(const JobDesc* __restrict__ jobs,
                                                             int32_t n_jobs, int64_t n_items,
                                                             unsigned long long* counter) {
   extern __shared__ __align__(16) char lds[];
   __shared__ long long item_sh;
-  bool table_loaded = false;
   for (;;) {
     if (threadIdx.x == 0) item_sh = (long long)atomicAdd(counter, 1ull);
     __syncthreads();
     const long long item = item_sh;
     __syncthreads();
-    if (item >= n_items) break;  // every wave of every block reaches this exit
+    if (item >= n_items) break;
     // job = last j with jobs[j].item_begin <= item (binary search, uniform)
     int lo = 0, hi = n_jobs - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
       if (jobs[mid].item_begin <= item) lo = mid; else hi = mid - 1;
     }
-    const BpArgs a = jobs[lo].a;
-    if (!table_loaded) {
-      load_table<INTERP>(a, lds);
-      table_loaded = true;
-    }
-    bp_tile<INTERP, PREC, DT, PPL>(a, (int)(item - jobs[lo].item_begin), lds);
+    lo = __builtin_amdgcn_readfirstlane(lo);  // uniform: descriptor reads become scalar loads
+    bp_tile<INTERP, PREC, DT, PPL>(jobs[lo].a, (int)(item - jobs[lo].item_begin), lds);
   }
 }
 
+__global__ void reset_counter_kernel(unsigned long long* counter) { *counter = 0ull; }
+
+// ---- dispatch over the template parameters ----
 template <int INTERP, int PREC, int DT, int PPL>
 hipError_t launch_plain_t(const BpArgs& a, hipStream_t s) {
   const int nb = a.tiles_x * a.tiles_y;
-  const size_t shm = lds_bytes(INTERP, a.win, a.pulses_per_stage);
+  const size_t shm = lds_layout(INTERP, a.win, a.pulses_per_stage).total;
   hipLaunchKernelGGL((bp_plain_kernel<INTERP, PREC, DT, PPL>), dim3(nb), dim3(kThreads), shm, s, a);
   return hipGetLastError();
 }
 
 template <int INTERP, int PREC, int DT, int PPL>
 hipError_t launch_batch_t(const JobDesc* jobs, int32_t n_jobs, int64_t n_items,
-                          unsigned long long* counter, size_t shm, int n_blocks,
-                          hipStream_t s) {
+                          unsigned long long* counter, size_t shm, int n_blocks, hipStream_t s) {
   hipLaunchKernelGGL((bp_batch_kernel<INTERP, PREC, DT, PPL>), dim3(n_blocks), dim3(kThreads),
                      shm, s, jobs, n_jobs, n_items, counter);
   return hipGetLastError();
 }
 
-// Dispatch tables over the template parameters.
-template <int INTERP, int PREC, int DT>
-hipError_t plain_ppl(const BpArgs& a, int ppl, hipStream_t s) {
-  switch (ppl) {
-    case 2: return launch_plain_t<INTERP, PREC, DT, 2>(a, s);
-    case 4: return launch_plain_t<INTERP, PREC, DT, 4>(a, s);
-    case 8: return launch_plain_t<INTERP, PREC, DT, 8>(a, s);
-    default: return hipErrorInvalidValue;
-  }
-}
-template <int INTERP, int PREC>
-hipError_t plain_dt(const BpArgs& a, int dt, int ppl, hipStream_t s) {
-  return dt == 0 ? plain_ppl<INTERP, PREC, 0>(a, ppl, s) : plain_ppl<INTERP, PREC, 1>(a, ppl, s);
-}
-template <int INTERP>
-hipError_t plain_prec(const BpArgs& a, int prec, int dt, int ppl, hipStream_t s) {
-  switch (prec) {
-    case kRef: return plain_dt<INTERP, kRef>(a, dt, ppl, s);
-    case kF64: return plain_dt<INTERP, kF64>(a, dt, ppl, s);
-    case kF32: return plain_dt<INTERP, kF32>(a, dt, ppl, s);
-    default: return hipErrorInvalidValue;
-  }
+template <int INTERP, int PREC, int DT, int PPL>
+int occupancy_t(size_t shm) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &nb, reinterpret_cast<const void*>(&bp_batch_kernel<INTERP, PREC, DT, PPL>), kThreads,
+          shm) != hipSuccess)
+    return 1;
+  return nb > 0 ? nb : 1;
 }
 
-template <int INTERP, int PREC, int DT>
-hipError_t batch_ppl(const JobDesc* j, int32_t n, int64_t ni, unsigned long long* c,
-                     size_t shm, int nb, int ppl, hipStream_t s) {
-  switch (ppl) {
-    case 2: return launch_batch_t<INTERP, PREC, DT, 2>(j, n, ni, c, shm, nb, s);
-    case 4: return launch_batch_t<INTERP, PREC, DT, 4>(j, n, ni, c, shm, nb, s);
-    case 8: return launch_batch_t<INTERP, PREC, DT, 8>(j, n, ni, c, shm, nb, s);
+// F is invoked with the compile-time parameters matching the runtime
+// (interp, precision, dtype, ppl); combinations that are not instantiated
+// return hipErrorInvalidValue.
+template <typename F>
+hipError_t dispatch(int interp, int prec, int dt, int ppl, const F& f) {
+#define SARBP_CASE_PPL(I, P, D)                                  \
+  switch (ppl) {                                                 \
+    case 2: return f.template run<I, P, D, 2>();                 \
+    case 4: return f.template run<I, P, D, 4>();                 \
+    case 8: return f.template run<I, P, D, 8>();                 \
+    default: return hipErrorInvalidValue;                        \
+  }
+#define SARBP_CASE_DT(I, P) \
+  if (dt == 0) { SARBP_CASE_PPL(I, P, 0) } else { SARBP_CASE_PPL(I, P, 1) }
+#define SARBP_CASE_PREC(I)                \
+  switch (prec) {                         \
+    case kRef: { SARBP_CASE_DT(I, kRef) } \
+    case kF64: { SARBP_CASE_DT(I, kF64) } \
+    case kF32: { SARBP_CASE_DT(I, kF32) } \
+    default: return hipErrorInvalidValue; \
+  }
+  switch (interp) {
+    case kNearest: { SARBP_CASE_PREC(kNearest) }
+    case kLinear: { SARBP_CASE_PREC(kLinear) }
+    case kSinc8: { SARBP_CASE_PREC(kSinc8) }
     default: return hipErrorInvalidValue;
   }
+#undef SARBP_CASE_PREC
+#undef SARBP_CASE_DT
+#undef SARBP_CASE_PPL
 }
-template <int INTERP, int PREC>
-hipError_t batch_dt(const JobDesc* j, int32_t n, int64_t ni, unsigned long long* c, size_t shm,
-                    int nb, int dt, int ppl, hipStream_t s) {
-  return dt == 0 ? batch_ppl<INTERP, PREC, 0>(j, n, ni, c, shm, nb, ppl, s)
-                 : batch_ppl<INTERP, PREC, 1>(j, n, ni, c, shm, nb, ppl, s);
-}
-template <int INTERP>
-hipError_t batch_prec(const JobDesc* j, int32_t n, int64_t ni, unsigned long long* c, size_t shm,
-                      int nb, int prec, int dt, int ppl, hipStream_t s) {
-  switch (prec) {
-    case kRef: return batch_dt<INTERP, kRef>(j, n, ni, c, shm, nb, dt, ppl, s);
-    case kF64: return batch_dt<INTERP, kF64>(j, n, ni, c, shm, nb, dt, ppl, s);
-    case kF32: return batch_dt<INTERP, kF32>(j, n, ni, c, shm, nb, dt, ppl, s);
-    default: return hipErrorInvalidValue;
+
+struct PlainLauncher {
+  const BpArgs& a;
+  hipStream_t s;
+  template <int I, int P, int D, int L>
+  hipError_t run() const { return launch_plain_t<I, P, D, L>(a, s); }
+};
+
+struct BatchLauncher {
+  const JobDesc* jobs;
+  int32_t n_jobs;
+  int64_t n_items;
+  unsigned long long* counter;
+  size_t shm;
+  int n_blocks;
+  hipStream_t s;
+  template <int I, int P, int D, int L>
+  hipError_t run() const {
+    return launch_batch_t<I, P, D, L>(jobs, n_jobs, n_items, counter, shm, n_blocks, s);
   }
-}
+};
+
+struct OccupancyQuery {
+  size_t shm;
+  int* out;
+  template <int I, int P, int D, int L>
+  hipError_t run() const {
+    *out = occupancy_t<I, P, D, L>(shm);
+    return hipSuccess;
+  }
+};
 
 }  // namespace
 
-size_t lds_bytes(int interp, int win, int pulses_per_stage) {
-  size_t b = lds_fixed_bytes(win, pulses_per_stage);
-  if (interp == kSinc8) b += (size_t)kSincQ * 64;
-  return b;
-}
-
 hipError_t launch_plain(const BpArgs& a, int interp, int precision, int dtype, int ppl,
                         hipStream_t stream) {
-  switch (interp) {
-    case kNearest: return plain_prec<kNearest>(a, precision, dtype, ppl, stream);
-    case kLinear: return plain_prec<kLinear>(a, precision, dtype, ppl, stream);
-    case kSinc8: return plain_prec<kSinc8>(a, precision, dtype, ppl, stream);
-    default: return hipErrorInvalidValue;
-  }
+  return dispatch(interp, precision, dtype, ppl, PlainLauncher{a, stream});
 }
 
 hipError_t launch_batch(const JobDesc* d_jobs, int32_t n_jobs, int64_t n_items,
                         unsigned long long* d_counter, int interp, int precision, int dtype,
-                        int ppl, int pulses_per_stage, int win, int n_blocks,
-                        hipStream_t stream) {
-  const size_t shm = lds_bytes(interp, win, pulses_per_stage);
-  switch (interp) {
-    case kNearest: return batch_prec<kNearest>(d_jobs, n_jobs, n_items, d_counter, shm, n_blocks, precision, dtype, ppl, stream);
-    case kLinear: return batch_prec<kLinear>(d_jobs, n_jobs, n_items, d_counter, shm, n_blocks, precision, dtype, ppl, stream);
-    case kSinc8: return batch_prec<kSinc8>(d_jobs, n_jobs, n_items, d_counter, shm, n_blocks, precision, dtype, ppl, stream);
-    default: return hipErrorInvalidValue;
-  }
+                        int ppl, size_t shm, int n_blocks, hipStream_t stream) {
+  return dispatch(interp, precision, dtype, ppl,
+                  BatchLauncher{d_jobs, n_jobs, n_items, d_counter, shm, n_blocks, stream});
+}
+
+hipError_t reset_counter(unsigned long long* d_counter, hipStream_t stream) {
+  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, d_counter);
+  return hipGetLastError();
+}
+
+int batch_blocks_per_cu(int interp, int precision, int dtype, int ppl, size_t shm) {
+  int nb = 1;
+  (void)dispatch(interp, precision, dtype, ppl, OccupancyQuery{shm, &nb});
+  return nb;
 }
 
 }  // namespace sarbp

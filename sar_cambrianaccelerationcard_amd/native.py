@@ -62,8 +62,8 @@ class Tuning(C.Structure):
 
 
 class Job(C.Structure):
-    _fields_ = [("d_profiles", C.c_void_p), ("d_pos", C.c_void_p), ("grid", Grid),
-                ("params", Params), ("d_image", C.c_void_p)]
+    _fields_ = [("d_profiles", C.c_void_p), ("d_pos", C.c_void_p), ("h_pos", C.c_void_p),
+                ("grid", Grid), ("params", Params), ("d_image", C.c_void_p)]
 
 
 class SarbpError(RuntimeError):
@@ -97,6 +97,7 @@ _SIGS = {
     "sarbp_set_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(Tuning)]),
     "sarbp_get_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(Tuning)]),
     "sarbp_backproject_batch": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_int32, C.c_void_p]),
+    "sarbp_sinc8_coefficients": (C.c_int, [C.c_double, C.POINTER(C.c_float)]),
 }
 
 
@@ -206,6 +207,14 @@ class Context:
         arr = (Job * len(jobs))(*jobs)
         check(self.lib.sarbp_backproject_batch(self.handle, arr, len(jobs), C.c_void_p(stream)),
               "sarbp_backproject_batch")
+
+
+def sinc8_coefficients(beta: float):
+    """The library's sinc8 polynomial coefficients as a numpy [8 powers][8 taps]."""
+    import numpy as np
+    out = (C.c_float * 64)()
+    check(load().sarbp_sinc8_coefficients(float(beta), out), "sarbp_sinc8_coefficients")
+    return np.frombuffer(out, dtype=np.float32).reshape(8, 8).copy()
 
 
 class Plan:
