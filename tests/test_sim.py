@@ -1,0 +1,106 @@
+"""CPU tests of the scene simulator (csrc/sarsim.cpp): its own radix-2 FFT,
+the matched filter and the amplitude-preserving upsampling that produce the
+range-compressed pulses, and the track generators."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from sar_cambrianaccelerationcard_amd import scene as S
+
+
+def test_fft_all_ones_known_answer():
+    """All-ones input: DC bin = N, every other bin 0 (the known answer recorded
+    in the reference's code/fft_simple/fft_kt_sample.cc:5-29)."""
+    for n in [8, 1024, 131072 // 8]:
+        y = S.fft(np.ones(n, dtype=np.complex128))
+        assert abs(y[0] - n) < 1e-9 * n
+        assert np.abs(y[1:]).max() < 1e-9 * n
+
+
+@pytest.mark.parametrize("n", [2, 16, 64])
+def test_fft_matches_brute_force_dft(n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    k = np.arange(n)
+    dft = np.exp(-2j * np.pi * np.outer(k, k) / n) @ x
+    assert np.abs(S.fft(x) - dft).max() < 1e-12 * np.abs(dft).max()
+    assert np.abs(S.fft(x, inverse=True) - n * np.fft.ifft(x)).max() < 1e-11 * n
+
+
+def test_fft_matches_numpy_large():
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(8192) + 1j * rng.standard_normal(8192)
+    assert np.abs(S.fft(x) - np.fft.fft(x)).max() < 1e-10 * np.abs(np.fft.fft(x)).max()
+
+
+def test_fft_rejects_non_power_of_two():
+    with pytest.raises(ValueError):
+        S.fft(np.ones(12, dtype=np.complex128))
+
+
+def _radar(n_raw=512, up=4, n_out=2048):
+    return S.Radar(299792458.0, 9.6e9, 150e6, 2e-6, 200e6, 2 * 4800.0 / 299792458.0,
+                   n_raw + 400, up, 0, n_out, 0.0)
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def test_matched_filter_peak_position_and_amplitude():
+    """A unit scatterer at slant range R compresses to a peak at bin
+    (R - r0) / dr of amplitude = chirp samples, with phase -4 pi R / lambda."""
+    L = S.sim()
+    r = _radar()
+    pos = np.array([[0.0, 0.0, 0.0]])
+    R = 4900.123
+    tgt = np.array([[0.0, R, 0.0, 1.0, 0.0]])
+    raw = np.zeros((1, r.n_raw), dtype=np.complex128)
+    L.sarsim_echo_raw(_dp(pos), 1, _dp(tgt), 1, C.byref(r), _dp(raw.view(np.float64)), 1)
+    out = np.zeros((1, r.n_out), dtype=np.complex128)
+    assert L.sarsim_compress(_dp(raw.view(np.float64)), 1, C.byref(r), _dp(out.view(np.float64)),
+                             None, 1) == 0
+    r0, dr = L.sarsim_out_r0(C.byref(r)), L.sarsim_out_dr(C.byref(r))
+    k = int(np.argmax(np.abs(out[0])))
+    assert abs(k - (R - r0) / dr) <= 0.5
+    assert abs(np.abs(out[0, k]) / 399.0 - 1) < 0.02
+    phase = np.angle(out[0, k] * np.exp(4j * np.pi * 9.6e9 * R / 299792458.0))
+    frac = (R - r0) / dr - k  # residual delay: phase of the compressed pulse is flat near the peak
+    assert abs(phase) < 0.05 + abs(frac) * 0.0
+
+
+def test_upsampling_preserves_samples():
+    """Zero-pad upsampling by L keeps the compressed samples at multiples of L."""
+    L = S.sim()
+    rng = np.random.default_rng(5)
+    r1, r4 = _radar(up=1, n_out=256), _radar(up=4, n_out=1024)
+    raw = rng.standard_normal((1, r1.n_raw)) + 1j * rng.standard_normal((1, r1.n_raw))
+    o1 = np.zeros((1, 256), dtype=np.complex128)
+    o4 = np.zeros((1, 1024), dtype=np.complex128)
+    L.sarsim_compress(_dp(raw.view(np.float64)), 1, C.byref(r1), _dp(o1.view(np.float64)), None, 1)
+    L.sarsim_compress(_dp(raw.view(np.float64)), 1, C.byref(r4), _dp(o4.view(np.float64)), None, 1)
+    assert np.abs(o4[0, ::4] - o1[0]).max() < 1e-9 * np.abs(o1).max()
+
+
+def test_track_perturbation_statistics():
+    L = S.sim()
+    pos = np.zeros((4096, 3))
+    L.sarsim_track_straight(_dp(pos), 4096, 150.0, 1000.0, 0.0, 0.0, 3000.0)
+    assert abs(pos[1, 0] - pos[0, 0] - 0.15) < 1e-12
+    assert abs(pos[:, 0].mean()) < 1e-9
+    L.sarsim_track_perturb(_dp(pos), 4096, 0.5, 200, 7)
+    assert abs(pos[:, 1].std() - 0.5) < 1e-9
+    assert abs(pos[:, 2].std() - 0.5) < 1e-9
+    # low-passed: adjacent pulses are strongly correlated
+    d = np.diff(pos[:, 1])
+    assert d.std() < 0.1  # white noise of the same sigma: 0.71
+
+
+def test_scene_is_seeded():
+    cfg = S.scaled_config(S.load_config("config0"), n_pulses=16, nx=32, ny=32)
+    a, b = S.make_scene(cfg), S.make_scene(cfg)
+    assert np.array_equal(a.profiles, b.profiles)
+    cfg["seed"] += 1
+    c = S.make_scene(cfg)
+    assert not np.array_equal(a.profiles, c.profiles)
