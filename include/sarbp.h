@@ -189,6 +189,31 @@ sarbp_status sarbp_get_tuning(sarbp_context* ctx, int32_t interp,
                               int32_t precision, int32_t dtype,
                               sarbp_tuning* tuning);
 
+/* Result of sarbp_autotune. */
+typedef struct sarbp_autotune_result {
+  sarbp_tuning best;      /* stored as this context's tuning               */
+  double best_ms;         /* per launch on the timing crop                 */
+  double default_ms;      /* the built-in default on the same crop         */
+  int32_t n_candidates;   /* configurations timed                          */
+  int32_t crop_nx, crop_ny;
+} sarbp_autotune_result;
+
+/*
+ * Project autotuner.  Times every (pixels_per_lane, pulses_per_stage)
+ * configuration whose LDS footprint fits, on a centred crop of at most
+ * max_crop_px x max_crop_px pixels of the given job (all pulses, the plan the
+ * optional host positions allow), with HIP events on `stream`, and stores the
+ * fastest as this context's tuning for the job's (interp, precision, dtype).
+ * d_scratch receives the crop images and must hold scratch_pixels >=
+ * crop_nx * crop_ny complex64 values.  Synchronous.
+ */
+sarbp_status sarbp_autotune(sarbp_context* ctx, const void* d_profiles,
+                            const double* d_pos, const double* h_pos,
+                            const sarbp_grid* grid, const sarbp_params* params,
+                            sarbp_cf32* d_scratch, size_t scratch_pixels,
+                            int32_t max_crop_px, void* stream,
+                            sarbp_autotune_result* result);
+
 /* ---- batched launch --------------------------------------------------------- */
 
 typedef struct sarbp_job {
@@ -210,6 +235,90 @@ typedef struct sarbp_job {
  */
 sarbp_status sarbp_backproject_batch(sarbp_context* ctx, const sarbp_job* jobs,
                                      int32_t n_jobs, void* stream);
+
+/* ---- serving --------------------------------------------------------------------
+ *
+ * A server owns one device.  Requests are queued on submit; an admission
+ * thread moves them, in arrival order, into a device arena of
+ * hbm_budget_bytes (profiles, positions and image resident in HBM) as memory
+ * and job slots free up; a scheduler thread runs epochs: one batched
+ * persistent launch (replayed from a captured hipGraph) in which every
+ * resident job back-projects its next chunk of pulses_per_epoch pulses.  Jobs
+ * admitted while others are mid-flight join the next epoch (continuous
+ * batching); a completion thread copies each finished image to the caller's
+ * host buffer and frees its arena space.  A job's pixel sums are formed
+ * chunk by chunk in a fixed order, so its image is bitwise the same whatever
+ * else is in flight.  No counterpart in the reference, which images one
+ * simulated scene per process (code/SAR_BP/CUDA/main.cpp:3-19).
+ */
+typedef struct sarbp_server sarbp_server;
+
+typedef struct sarbp_server_config {
+  int32_t device;
+  int32_t interp, precision, dtype; /* shared by every job of the server      */
+  size_t hbm_budget_bytes;          /* device arena for resident jobs         */
+  int32_t max_active_jobs;          /* resident jobs at once (0: no limit)    */
+  int32_t pulses_per_epoch;         /* pulse chunk per job per epoch (0: all) */
+  int32_t pixels_per_lane;          /* launch tuning (0: built-in default)    */
+  int32_t pulses_per_stage;
+  double epoch_watchdog_s;          /* epochs slower than this fail their jobs
+                                       with SARBP_ERR_TIMEOUT (0: off)        */
+} sarbp_server_config;
+
+/* Host buffers must stay valid until the job is done, failed or cancelled. */
+typedef struct sarbp_request {
+  const void* h_profiles; /* [n_pulses][n_samples], the server's dtype     */
+  const double* h_pos;    /* [n_pulses][3]                                  */
+  sarbp_grid grid;
+  sarbp_params params;    /* interp / precision / dtype = the server's      */
+  sarbp_cf32* h_image;    /* [ny][nx] output                                */
+} sarbp_request;
+
+typedef enum sarbp_job_state {
+  SARBP_JOB_QUEUED = 0,
+  SARBP_JOB_RESIDENT = 1, /* data in HBM, no epoch started yet */
+  SARBP_JOB_RUNNING = 2,
+  SARBP_JOB_DONE = 3,
+  SARBP_JOB_FAILED = 4,
+  SARBP_JOB_CANCELLED = 5
+} sarbp_job_state;
+
+typedef struct sarbp_job_info {
+  int32_t state;        /* sarbp_job_state                                   */
+  int32_t status;       /* sarbp_status of a failed job, else SARBP_OK       */
+  int32_t epochs;       /* epochs the job took part in                       */
+  int32_t reserved;
+  double submit_s, admit_s, first_epoch_s, done_s; /* server clock, seconds  */
+  double work;          /* pixel-pulses                                      */
+} sarbp_job_info;
+
+typedef struct sarbp_server_stats {
+  int64_t submitted, completed, failed, cancelled, epochs;
+  int32_t queued, active;
+  double work_done;     /* pixel-pulses of completed jobs                    */
+  double gpu_busy_s;    /* summed device time of the epochs                  */
+  double uptime_s;
+  size_t hbm_in_use, hbm_peak;
+} sarbp_server_stats;
+
+sarbp_status sarbp_server_create(const sarbp_server_config* cfg, sarbp_server** server);
+/* Rejects requests that can never fit the arena with SARBP_ERR_OVER_BUDGET. */
+sarbp_status sarbp_server_submit(sarbp_server* server, const sarbp_request* req,
+                                 uint64_t* job_id);
+sarbp_status sarbp_server_poll(sarbp_server* server, uint64_t job_id,
+                               sarbp_job_info* info);
+/* Waits until the job is done, failed or cancelled (SARBP_ERR_TIMEOUT after
+ * timeout_s; timeout_s < 0 waits forever). */
+sarbp_status sarbp_server_wait(sarbp_server* server, uint64_t job_id,
+                               double timeout_s, sarbp_job_info* info);
+sarbp_status sarbp_server_cancel(sarbp_server* server, uint64_t job_id);
+/* Forgets a finished job's record (SARBP_ERR_NOT_READY while it runs). */
+sarbp_status sarbp_server_release(sarbp_server* server, uint64_t job_id);
+sarbp_status sarbp_server_get_stats(sarbp_server* server, sarbp_server_stats* stats);
+/* Waits until no job is queued or active (SARBP_ERR_TIMEOUT after timeout_s). */
+sarbp_status sarbp_server_drain(sarbp_server* server, double timeout_s);
+/* Stops accepting work, cancels queued jobs, waits for in-flight epochs. */
+void sarbp_server_destroy(sarbp_server* server);
 
 /* ---- diagnostics -------------------------------------------------------------- */
 

@@ -34,6 +34,9 @@ HIP_FLAGS = [
 ]
 
 
+SARBP_SOURCES = ("sarbp_kernels.hip", "sarbp_api.hip", "sarbp_batch.hip", "sarbp_server.hip")
+
+
 def _run(cmd: list[str]) -> None:
     r = subprocess.run(cmd, capture_output=True, text=True)
     # hipcc's host pass does not know the packed-fp32 feature and says so.
@@ -53,15 +56,16 @@ def _newer(target: str, sources: list[str]) -> bool:
 def build_sarbp(force: bool = True) -> str:
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libsarbp.so")
-    srcs = [os.path.join(CSRC, f) for f in ("sarbp_kernels.hip", "sarbp_api.hip")]
-    deps = srcs + [os.path.join(CSRC, "sarbp_internal.h"), os.path.join(INCLUDE, "sarbp.h")]
+    srcs = [os.path.join(CSRC, f) for f in SARBP_SOURCES]
+    deps = srcs + [os.path.join(CSRC, h) for h in ("sarbp_internal.h", "sarbp_host.h")] + \
+        [os.path.join(INCLUDE, "sarbp.h")]
     if not force and _newer(out, deps):
         return out
-    objs = []
-    for s in srcs:
-        o = os.path.join(LIB, os.path.basename(s) + ".o")
-        _run([HIPCC, *HIP_FLAGS, "-c", s, "-o", o])
-        objs.append(o)
+    from concurrent.futures import ThreadPoolExecutor
+    objs = [os.path.join(LIB, os.path.basename(s) + ".o") for s in srcs]
+    with ThreadPoolExecutor(max_workers=len(srcs)) as ex:
+        list(ex.map(lambda so: _run([HIPCC, *HIP_FLAGS, "-c", so[0], "-o", so[1]]),
+                    zip(srcs, objs)))
     tmp = out + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, out)

@@ -3,7 +3,7 @@
 // depth), per-context Kaiser-sinc tables, and the plain / planned / batched
 // launches.  Nothing here exits or throws across the ABI.
 #include "sarbp.h"
-#include "sarbp_internal.h"
+#include "sarbp_host.h"
 
 #include <algorithm>
 #include <cmath>
@@ -16,13 +16,12 @@
 #include <string>
 #include <vector>
 
-using namespace sarbp;
-
-namespace {
+namespace sarbp {
 
 thread_local std::string g_detail;
 
-sarbp_status fail(sarbp_status st, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+const std::string& last_detail() { return g_detail; }
+
 sarbp_status fail(sarbp_status st, const char* fmt, ...) {
   char buf[512];
   va_list ap;
@@ -37,18 +36,6 @@ sarbp_status hip_fail(hipError_t e, const char* what) {
   return fail(e == hipErrorOutOfMemory ? SARBP_ERR_OUT_OF_MEMORY : SARBP_ERR_DEVICE,
               "%s: %s", what, hipGetErrorString(e));
 }
-
-// Restores the caller's current device on scope exit.
-struct DeviceGuard {
-  int prev = -1;
-  hipError_t err = hipSuccess;
-  explicit DeviceGuard(int dev) {
-    if ((err = hipGetDevice(&prev)) == hipSuccess && prev != dev) err = hipSetDevice(dev);
-  }
-  ~DeviceGuard() {
-    if (prev >= 0) (void)hipSetDevice(prev);
-  }
-};
 
 constexpr double kPi = 3.14159265358979323846;
 
@@ -111,19 +98,16 @@ bool finite_all(const double* v, int n) {
 
 double norm3(const double* v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
 
-}  // namespace
+}  // namespace sarbp
+
+using namespace sarbp;
 
 struct sarbp_context {
   int device = 0;
   int n_cu = 256;
   std::mutex mu;
   std::map<int, sarbp_tuning> tuning;
-  // batched launches
-  JobDesc* d_jobs = nullptr;
-  int32_t jobs_cap = 0;
-  JobDesc* h_jobs = nullptr;  // pinned staging copy of the descriptor table
-  unsigned long long* d_counter = nullptr;
-  hipEvent_t batch_done = nullptr;  // recorded after the last batched launch
+  BatchExec* batch = nullptr;  // graph-replayed batched launches
 };
 
 struct sarbp_plan {
@@ -133,7 +117,7 @@ struct sarbp_plan {
   sarbp_plan_info info;
 };
 
-namespace {
+namespace sarbp {
 
 sarbp_status validate(const sarbp_grid* g, const sarbp_params* p) {
   if (!g || !p) return fail(SARBP_ERR_INVALID_ARGUMENT, "grid or params is NULL");
@@ -159,9 +143,17 @@ sarbp_status validate(const sarbp_grid* g, const sarbp_params* p) {
 sarbp_tuning get_tuning(sarbp_context* ctx, int interp, int prec, int dtype) {
   auto it = ctx->tuning.find(tuning_key(interp, prec, dtype));
   if (it != ctx->tuning.end()) return it->second;
+  return default_tuning(interp);
+}
+
+sarbp_tuning default_tuning(int interp) {
+  // Built-in defaults: the best points of a (pixels/lane x pulses/stage) sweep
+  // on MI355X at 0.25 m pixels over a 4096-pulse aperture.  Sinc8's LDS
+  // footprint per pulse is ~9x linear's, so it stages fewer pulses to keep
+  // several workgroups resident per CU.
   sarbp_tuning t;
-  t.pixels_per_lane = 4;
-  t.pulses_per_stage = 32;
+  t.pixels_per_lane = 8;
+  t.pulses_per_stage = interp == SARBP_INTERP_SINC8 ? 8 : 32;
   return t;
 }
 
@@ -232,8 +224,6 @@ Extent tile_extent(const sarbp_grid* g, int ppl, const double* h_pos, int n_puls
   return e;
 }
 
-constexpr size_t kLdsBudget = 64 * 1024;
-
 // Fills a launch descriptor.  Caller holds ctx->mu and the device guard.
 sarbp_status plan_args(const double* h_pos, const sarbp_grid* g,
                        const sarbp_params* p, const sarbp_tuning& tune, BpArgs* a,
@@ -303,7 +293,7 @@ sarbp_status launch(const BpArgs& a, const sarbp_params* p, int ppl, hipStream_t
   return SARBP_OK;
 }
 
-}  // namespace
+}  // namespace sarbp
 
 extern "C" {
 
@@ -349,13 +339,7 @@ void sarbp_destroy(sarbp_context* c) {
   if (!c) return;
   {
     DeviceGuard dg(c->device);
-    if (c->d_jobs) (void)hipFree(c->d_jobs);
-    if (c->h_jobs) (void)hipHostFree(c->h_jobs);
-    if (c->batch_done) {
-      (void)hipEventSynchronize(c->batch_done);
-      (void)hipEventDestroy(c->batch_done);
-    }
-    if (c->d_counter) (void)hipFree(c->d_counter);
+    delete c->batch;
   }
   delete c;
 }
@@ -506,8 +490,6 @@ sarbp_status sarbp_backproject_batch(sarbp_context* ctx, const sarbp_job* jobs, 
   });
   std::vector<JobDesc> desc;
   desc.reserve((size_t)n_jobs);
-  int64_t items = 0;
-  size_t shm = 0;
   for (int32_t j : order) {
     const sarbp_job& jb = jobs[j];
     if (jb.grid.nx == 0 || jb.grid.ny == 0) continue;
@@ -517,45 +499,113 @@ sarbp_status sarbp_backproject_batch(sarbp_context* ctx, const sarbp_job* jobs, 
     d.a.prof = jb.d_profiles;
     d.a.pos = jb.d_pos;
     d.a.img = reinterpret_cast<float2*>(jb.d_image);
-    d.item_begin = items;
-    items += (int64_t)d.a.tiles_x * d.a.tiles_y;
-    shm = std::max(shm, lds_layout(p0.interp, d.a.win, d.a.pulses_per_stage).total);
+    d.item_begin = 0;
     desc.push_back(d);
   }
   if (desc.empty()) return SARBP_OK;
+  if (!ctx->batch && !(ctx->batch = new (std::nothrow) BatchExec))
+    return fail(SARBP_ERR_OUT_OF_MEMORY, "batch executor");
+  return ctx->batch->run(desc, p0.interp, p0.precision, p0.dtype, tune.pixels_per_lane, ctx->n_cu,
+                         (hipStream_t)stream);
+}
+
+sarbp_status sarbp_autotune(sarbp_context* ctx, const void* d_profiles, const double* d_pos,
+                            const double* h_pos, const sarbp_grid* grid, const sarbp_params* params,
+                            sarbp_cf32* d_scratch, size_t scratch_pixels, int32_t max_crop_px,
+                            void* stream, sarbp_autotune_result* out) {
+  if (!ctx || !out) return fail(SARBP_ERR_INVALID_ARGUMENT, "NULL argument");
+  sarbp_status st = validate(grid, params);
+  if (st != SARBP_OK) return st;
+  if (params->n_pulses < 1 || !d_profiles || !d_pos || !d_scratch)
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "autotune needs pulses and device buffers");
+  if (max_crop_px < kTileX) return fail(SARBP_ERR_INVALID_ARGUMENT, "max_crop_px < %d", kTileX);
+  sarbp_grid crop = *grid;
+  crop.nx = std::min(grid->nx, max_crop_px);
+  crop.ny = std::min(grid->ny, max_crop_px);
+  if (crop.nx < 1 || crop.ny < 1) return fail(SARBP_ERR_INVALID_ARGUMENT, "empty image");
+  if ((size_t)crop.nx * crop.ny > scratch_pixels)
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "scratch holds %zu pixels, crop needs %d", scratch_pixels,
+                crop.nx * crop.ny);
+  const int ox = (grid->nx - crop.nx) / 2, oy = (grid->ny - crop.ny) / 2;
+  for (int k = 0; k < 3; ++k) crop.origin[k] = grid->origin[k] + ox * grid->ux[k] + oy * grid->uy[k];
+  sarbp_params p = *params;
+  p.accumulate = 0;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  DeviceGuard dg(ctx->device);
+  if (dg.err != hipSuccess) return hip_fail(dg.err, "hipSetDevice");
   hipStream_t s = (hipStream_t)stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   hipError_t e;
-  // The descriptor table and counter are reused by the next batch call on
-  // this context: wait until the previous batch has consumed them.
-  if (ctx->batch_done) {
-    if ((e = hipEventSynchronize(ctx->batch_done)) != hipSuccess) return hip_fail(e, "hipEventSynchronize");
-  } else if ((e = hipEventCreateWithFlags(&ctx->batch_done, hipEventDisableTiming)) != hipSuccess) {
+  if ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess) {
+    if (e0) (void)hipEventDestroy(e0);
     return hip_fail(e, "hipEventCreate");
   }
-  if ((int32_t)desc.size() > ctx->jobs_cap) {
-    if (ctx->d_jobs) (void)hipFree(ctx->d_jobs);
-    if (ctx->h_jobs) (void)hipHostFree(ctx->h_jobs);
-    ctx->d_jobs = nullptr;
-    ctx->h_jobs = nullptr;
-    ctx->jobs_cap = 0;
-    const int32_t cap = std::max<int32_t>(64, (int32_t)desc.size());
-    if ((e = hipMalloc(&ctx->d_jobs, sizeof(JobDesc) * cap)) != hipSuccess) return hip_fail(e, "hipMalloc(jobs)");
-    if ((e = hipHostMalloc(&ctx->h_jobs, sizeof(JobDesc) * cap, hipHostMallocDefault)) != hipSuccess)
-      return hip_fail(e, "hipHostMalloc(jobs)");
-    ctx->jobs_cap = cap;
+  auto time_one = [&](const sarbp_tuning& t, double* ms, int* eff_p) -> sarbp_status {
+    BpArgs a;
+    sarbp_status r = plan_args(h_pos, &crop, &p, t, &a, nullptr);
+    if (r != SARBP_OK) return r;
+    *eff_p = a.pulses_per_stage;
+    a.prof = d_profiles;
+    a.pos = d_pos;
+    a.img = reinterpret_cast<float2*>(d_scratch);
+    if ((r = launch(a, &p, t.pixels_per_lane, s)) != SARBP_OK) return r;  // warm-up
+    hipError_t he;
+    if ((he = hipEventRecord(e0, s)) != hipSuccess) return hip_fail(he, "hipEventRecord");
+    const int reps = 3;
+    for (int i = 0; i < reps; ++i)
+      if ((r = launch(a, &p, t.pixels_per_lane, s)) != SARBP_OK) return r;
+    if ((he = hipEventRecord(e1, s)) != hipSuccess || (he = hipEventSynchronize(e1)) != hipSuccess)
+      return hip_fail(he, "hipEventSynchronize");
+    float f = 0.0f;
+    if ((he = hipEventElapsedTime(&f, e0, e1)) != hipSuccess) return hip_fail(he, "hipEventElapsedTime");
+    *ms = f / reps;
+    return SARBP_OK;
+  };
+  // default first (built-in, ignoring any stored tuning)
+  sarbp_tuning def;
+  def.pixels_per_lane = 8;
+  def.pulses_per_stage = p.interp == SARBP_INTERP_SINC8 ? 8 : 32;
+  int eff = 0;
+  double def_ms = 0.0;
+  st = time_one(def, &def_ms, &eff);
+  sarbp_tuning best = def;
+  double best_ms = def_ms;
+  int n = st == SARBP_OK ? 1 : 0;
+  std::vector<std::pair<int, int>> seen;
+  seen.emplace_back(def.pixels_per_lane, eff);
+  const int ppls[] = {2, 4, 8};
+  const int stages[] = {4, 6, 8, 12, 16, 24, 32, 48, 64};
+  for (int ppl : ppls) {
+    if (st != SARBP_OK) break;
+    for (int P : stages) {
+      sarbp_tuning t;
+      t.pixels_per_lane = ppl;
+      t.pulses_per_stage = P;
+      BpArgs probe;
+      if (plan_args(h_pos, &crop, &p, t, &probe, nullptr) != SARBP_OK) continue;
+      const std::pair<int, int> key(ppl, probe.pulses_per_stage);
+      if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
+      seen.push_back(key);
+      double ms = 0.0;
+      if ((st = time_one(t, &ms, &eff)) != SARBP_OK) break;
+      ++n;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = t;
+        best.pulses_per_stage = eff;
+      }
+    }
   }
-  if (!ctx->d_counter && (e = hipMalloc(&ctx->d_counter, sizeof(unsigned long long))) != hipSuccess)
-    return hip_fail(e, "hipMalloc(counter)");
-  std::memcpy(ctx->h_jobs, desc.data(), sizeof(JobDesc) * desc.size());
-  if ((e = hipMemcpyAsync(ctx->d_jobs, ctx->h_jobs, sizeof(JobDesc) * desc.size(), hipMemcpyHostToDevice, s)) != hipSuccess)
-    return hip_fail(e, "hipMemcpyAsync(jobs)");
-  if ((e = reset_counter(ctx->d_counter, s)) != hipSuccess) return hip_fail(e, "counter reset");
-  const int per_cu = batch_blocks_per_cu(p0.interp, p0.precision, p0.dtype, tune.pixels_per_lane, shm);
-  const int64_t grid = std::min<int64_t>(items, (int64_t)ctx->n_cu * per_cu);
-  if ((e = launch_batch(ctx->d_jobs, (int32_t)desc.size(), items, ctx->d_counter, p0.interp, p0.precision,
-                        p0.dtype, tune.pixels_per_lane, shm, (int)grid, s)) != hipSuccess)
-    return hip_fail(e, "batched BP kernel launch");
-  if ((e = hipEventRecord(ctx->batch_done, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st != SARBP_OK) return st;
+  ctx->tuning[tuning_key(p.interp, p.precision, p.dtype)] = best;
+  out->best = best;
+  out->best_ms = best_ms;
+  out->default_ms = def_ms;
+  out->n_candidates = n;
+  out->crop_nx = crop.nx;
+  out->crop_ny = crop.ny;
   return SARBP_OK;
 }
 

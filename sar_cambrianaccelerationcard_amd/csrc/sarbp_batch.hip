@@ -1,0 +1,122 @@
+// sarbp_batch.hip -- BatchExec: the batched persistent BP launch, captured once
+// per configuration into a hipGraph and replayed over a device job table.
+#include <algorithm>
+#include <cstring>
+
+#include "sarbp_host.h"
+
+namespace sarbp {
+
+BatchExec::~BatchExec() {
+  if (done_) {
+    (void)hipEventSynchronize(done_);
+    (void)hipEventDestroy(done_);
+  }
+  drop_graphs();
+  if (capture_stream_) (void)hipStreamDestroy(capture_stream_);
+  if (d_jobs_) (void)hipFree(d_jobs_);
+  if (h_jobs_) (void)hipHostFree(h_jobs_);
+  if (d_hdr_) (void)hipFree(d_hdr_);
+  if (h_hdr_) (void)hipHostFree(h_hdr_);
+  if (d_counter_) (void)hipFree(d_counter_);
+}
+
+void BatchExec::drop_graphs() {
+  for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+  graphs_.clear();
+}
+
+sarbp_status BatchExec::wait() {
+  if (!done_) return SARBP_OK;
+  hipError_t e = hipEventSynchronize(done_);
+  return e == hipSuccess ? SARBP_OK : hip_fail(e, "hipEventSynchronize(batch)");
+}
+
+sarbp_status BatchExec::reserve(size_t n) {
+  hipError_t e;
+  if (!d_hdr_) {
+    if ((e = hipMalloc(&d_hdr_, sizeof(BatchHeader))) != hipSuccess) return hip_fail(e, "hipMalloc(header)");
+    if ((e = hipHostMalloc(&h_hdr_, sizeof(BatchHeader), hipHostMallocDefault)) != hipSuccess)
+      return hip_fail(e, "hipHostMalloc(header)");
+    if ((e = hipMalloc(&d_counter_, sizeof(unsigned long long))) != hipSuccess)
+      return hip_fail(e, "hipMalloc(counter)");
+  }
+  if (n <= cap_) return SARBP_OK;
+  // The graphs hold the old table pointer.
+  drop_graphs();
+  if (d_jobs_) (void)hipFree(d_jobs_);
+  if (h_jobs_) (void)hipHostFree(h_jobs_);
+  d_jobs_ = nullptr;
+  h_jobs_ = nullptr;
+  cap_ = 0;
+  const size_t cap = std::max<size_t>(64, n + n / 2);
+  if ((e = hipMalloc(&d_jobs_, sizeof(JobDesc) * cap)) != hipSuccess) return hip_fail(e, "hipMalloc(jobs)");
+  if ((e = hipHostMalloc(&h_jobs_, sizeof(JobDesc) * cap, hipHostMallocDefault)) != hipSuccess)
+    return hip_fail(e, "hipHostMalloc(jobs)");
+  cap_ = cap;
+  return SARBP_OK;
+}
+
+sarbp_status BatchExec::run(std::vector<JobDesc>& jobs, int interp, int precision, int dtype, int ppl,
+                            int n_cu, hipStream_t s) {
+  if (jobs.empty()) return SARBP_OK;
+  hipError_t e;
+  // The pinned staging copies of the previous run must have been consumed.
+  sarbp_status st = wait();
+  if (st != SARBP_OK) return st;
+  if (!done_ && (e = hipEventCreateWithFlags(&done_, hipEventDisableTiming)) != hipSuccess)
+    return hip_fail(e, "hipEventCreate(batch)");
+  if ((st = reserve(jobs.size())) != SARBP_OK) return st;
+  int64_t items = 0;
+  size_t shm = 0;
+  for (JobDesc& d : jobs) {
+    d.item_begin = items;
+    items += (int64_t)d.a.tiles_x * d.a.tiles_y;
+    shm = std::max(shm, lds_layout(interp, d.a.win, d.a.pulses_per_stage).total);
+  }
+  if (items == 0) return SARBP_OK;
+  // LDS size classes of 8 KiB keep the number of captured graphs small.
+  const size_t shm_class = std::min<size_t>(kLdsBudget, (shm + 8191) & ~(size_t)8191);
+  std::memcpy(h_jobs_, jobs.data(), sizeof(JobDesc) * jobs.size());
+  h_hdr_->n_items = items;
+  h_hdr_->n_jobs = (int32_t)jobs.size();
+  h_hdr_->reserved = 0;
+  if ((e = hipMemcpyAsync(d_jobs_, h_jobs_, sizeof(JobDesc) * jobs.size(), hipMemcpyHostToDevice, s)) !=
+      hipSuccess)
+    return hip_fail(e, "hipMemcpyAsync(jobs)");
+  if ((e = hipMemcpyAsync(d_hdr_, h_hdr_, sizeof(BatchHeader), hipMemcpyHostToDevice, s)) != hipSuccess)
+    return hip_fail(e, "hipMemcpyAsync(header)");
+
+  const auto key = std::make_tuple(interp, precision, dtype, ppl, shm_class);
+  auto it = graphs_.find(key);
+  if (it == graphs_.end()) {
+    if (!capture_stream_ && (e = hipStreamCreateWithFlags(&capture_stream_, hipStreamNonBlocking)) != hipSuccess)
+      return hip_fail(e, "hipStreamCreate(capture)");
+    const int per_cu = batch_blocks_per_cu(interp, precision, dtype, ppl, shm_class);
+    const int n_blocks = std::max(1, n_cu * per_cu);
+    if ((e = hipStreamBeginCapture(capture_stream_, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+      return hip_fail(e, "hipStreamBeginCapture");
+    hipError_t e1 = reset_counter(d_counter_, capture_stream_);
+    hipError_t e2 = e1 == hipSuccess ? launch_batch(d_jobs_, d_hdr_, d_counter_, interp, precision, dtype, ppl,
+                                                    shm_class, n_blocks, capture_stream_)
+                                     : e1;
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(capture_stream_, &g);
+    if (e2 != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return hip_fail(e2, "batched launch capture");
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    hipGraphExec_t ge = nullptr;
+    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    it = graphs_.emplace(key, ge).first;
+    ++captured_;
+  }
+  if ((e = hipGraphLaunch(it->second, s)) != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+  if ((e = hipEventRecord(done_, s)) != hipSuccess) return hip_fail(e, "hipEventRecord(batch)");
+  return SARBP_OK;
+}
+
+}  // namespace sarbp

@@ -1,0 +1,87 @@
+// sarbp_host.h -- host-side planning helpers and the graph-replayed batched
+// launcher shared by the C ABI (sarbp_api.hip) and the server
+// (sarbp_server.hip).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "sarbp.h"
+#include "sarbp_internal.h"
+
+namespace sarbp {
+
+constexpr size_t kLdsBudget = 64 * 1024;  // dynamic LDS per workgroup
+
+// Records a detail message for sarbp_last_error_detail and returns st.
+sarbp_status fail(sarbp_status st, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+sarbp_status hip_fail(hipError_t e, const char* what);
+const std::string& last_detail();
+
+// Makes `dev` current for the scope and restores the caller's device after.
+struct DeviceGuard {
+  int prev = -1;
+  hipError_t err = hipSuccess;
+  explicit DeviceGuard(int dev) {
+    if ((err = hipGetDevice(&prev)) == hipSuccess && prev != dev) err = hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+  DeviceGuard(const DeviceGuard&) = delete;
+  DeviceGuard& operator=(const DeviceGuard&) = delete;
+};
+
+sarbp_status validate(const sarbp_grid* g, const sarbp_params* p);
+sarbp_tuning default_tuning(int interp);
+// Fills a launch descriptor (device pointers left NULL) for one image.
+sarbp_status plan_args(const double* h_pos, const sarbp_grid* g, const sarbp_params* p,
+                       const sarbp_tuning& tune, BpArgs* a, sarbp_plan_info* info);
+sarbp_status launch(const BpArgs& a, const sarbp_params* p, int ppl, hipStream_t s);
+
+// Batched persistent launch replayed from a captured hipGraph.  The graph is
+// [counter reset kernel -> persistent BP kernel] and reads the job table and
+// its header from fixed device buffers, which each run rewrites (async H2D
+// from pinned staging) on the caller's stream before replaying the graph.
+// One graph is captured per (interp, precision, dtype, pixels/lane, LDS size
+// class); buffers grow on demand (which drops the cached graphs).  Not
+// thread-safe: callers serialise.
+class BatchExec {
+ public:
+  BatchExec() = default;
+  ~BatchExec();
+  BatchExec(const BatchExec&) = delete;
+  BatchExec& operator=(const BatchExec&) = delete;
+
+  // Enqueues one batched launch over `jobs` (planned descriptors with device
+  // pointers set; item_begin filled in here) on stream s.
+  sarbp_status run(std::vector<JobDesc>& jobs, int interp, int precision, int dtype, int ppl,
+                   int n_cu, hipStream_t s);
+  // Blocks until the last run has finished on the device.
+  sarbp_status wait();
+  // Event recorded after the last run (NULL before the first run).
+  hipEvent_t done_event() const { return done_; }
+  // Number of graphs captured so far (for diagnostics and tests).
+  int graphs_captured() const { return captured_; }
+
+ private:
+  sarbp_status reserve(size_t n_jobs);
+  void drop_graphs();
+
+  JobDesc* d_jobs_ = nullptr;
+  JobDesc* h_jobs_ = nullptr;
+  BatchHeader* d_hdr_ = nullptr;
+  BatchHeader* h_hdr_ = nullptr;
+  unsigned long long* d_counter_ = nullptr;
+  size_t cap_ = 0;
+  hipEvent_t done_ = nullptr;
+  hipStream_t capture_stream_ = nullptr;
+  std::map<std::tuple<int, int, int, int, size_t>, hipGraphExec_t> graphs_;
+  int captured_ = 0;
+};
+
+}  // namespace sarbp

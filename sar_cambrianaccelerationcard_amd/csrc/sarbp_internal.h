@@ -49,6 +49,14 @@ struct JobDesc {
   int64_t item_begin;  // first global work-item index of this job
 };
 
+// Size of the job table a batched launch drains (device memory, so one
+// captured graph serves tables of any length).
+struct BatchHeader {
+  int64_t n_items;
+  int32_t n_jobs;
+  int32_t reserved;
+};
+
 // LDS layout of one workgroup (byte offsets):
 //   raw   [P][win] float2         staged range windows
 //   coef  [P][win-7][8] float2    sinc8 only: polynomial coefficients D_k of
@@ -82,10 +90,10 @@ __host__ __device__ inline LdsLayout lds_layout(int interp, int win, int P) {
 hipError_t launch_plain(const BpArgs& a, int interp, int precision, int dtype,
                         int ppl, hipStream_t stream);
 
-// Launches the persistent batched kernel over n_jobs descriptors in device
-// memory; n_items = total tiles; counter = device work counter (zeroed by
-// the caller on the stream before the launch); shm = max LDS over the jobs.
-hipError_t launch_batch(const JobDesc* d_jobs, int32_t n_jobs, int64_t n_items,
+// Launches the persistent batched kernel over the job table described by
+// *d_hdr (device memory); counter = device work counter (zeroed on the
+// stream before the launch); shm = LDS bytes >= every job's layout.
+hipError_t launch_batch(const JobDesc* d_jobs, const BatchHeader* d_hdr,
                         unsigned long long* d_counter, int interp,
                         int precision, int dtype, int ppl, size_t shm,
                         int n_blocks, hipStream_t stream);

@@ -393,10 +393,12 @@ __global__ __launch_bounds__(kThreads) void bp_plain_kernel(BpArgs a) {
 // the exit once the counter passes n_items.
 template <int INTERP, int PREC, int DT, int PPL>
 __global__ __launch_bounds__(kThreads) void bp_batch_kernel(const JobDesc* __restrict__ jobs,
-                                                            int32_t n_jobs, int64_t n_items,
+                                                            const BatchHeader* __restrict__ hdr,
                                                             unsigned long long* counter) {
   extern __shared__ __align__(16) char lds[];
   __shared__ long long item_sh;
+  const int64_t n_items = hdr->n_items;
+  const int32_t n_jobs = hdr->n_jobs;
   for (;;) {
     if (threadIdx.x == 0) item_sh = (long long)atomicAdd(counter, 1ull);
     __syncthreads();
@@ -426,10 +428,10 @@ hipError_t launch_plain_t(const BpArgs& a, hipStream_t s) {
 }
 
 template <int INTERP, int PREC, int DT, int PPL>
-hipError_t launch_batch_t(const JobDesc* jobs, int32_t n_jobs, int64_t n_items,
-                          unsigned long long* counter, size_t shm, int n_blocks, hipStream_t s) {
+hipError_t launch_batch_t(const JobDesc* jobs, const BatchHeader* hdr, unsigned long long* counter,
+                          size_t shm, int n_blocks, hipStream_t s) {
   hipLaunchKernelGGL((bp_batch_kernel<INTERP, PREC, DT, PPL>), dim3(n_blocks), dim3(kThreads),
-                     shm, s, jobs, n_jobs, n_items, counter);
+                     shm, s, jobs, hdr, counter);
   return hipGetLastError();
 }
 
@@ -484,15 +486,14 @@ struct PlainLauncher {
 
 struct BatchLauncher {
   const JobDesc* jobs;
-  int32_t n_jobs;
-  int64_t n_items;
+  const BatchHeader* hdr;
   unsigned long long* counter;
   size_t shm;
   int n_blocks;
   hipStream_t s;
   template <int I, int P, int D, int L>
   hipError_t run() const {
-    return launch_batch_t<I, P, D, L>(jobs, n_jobs, n_items, counter, shm, n_blocks, s);
+    return launch_batch_t<I, P, D, L>(jobs, hdr, counter, shm, n_blocks, s);
   }
 };
 
@@ -513,11 +514,11 @@ hipError_t launch_plain(const BpArgs& a, int interp, int precision, int dtype, i
   return dispatch(interp, precision, dtype, ppl, PlainLauncher{a, stream});
 }
 
-hipError_t launch_batch(const JobDesc* d_jobs, int32_t n_jobs, int64_t n_items,
+hipError_t launch_batch(const JobDesc* d_jobs, const BatchHeader* d_hdr,
                         unsigned long long* d_counter, int interp, int precision, int dtype,
                         int ppl, size_t shm, int n_blocks, hipStream_t stream) {
   return dispatch(interp, precision, dtype, ppl,
-                  BatchLauncher{d_jobs, n_jobs, n_items, d_counter, shm, n_blocks, stream});
+                  BatchLauncher{d_jobs, d_hdr, d_counter, shm, n_blocks, stream});
 }
 
 hipError_t reset_counter(unsigned long long* d_counter, hipStream_t stream) {

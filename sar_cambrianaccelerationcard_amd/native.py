@@ -61,9 +61,43 @@ class Tuning(C.Structure):
     _fields_ = [("pixels_per_lane", C.c_int32), ("pulses_per_stage", C.c_int32)]
 
 
+class AutotuneResult(C.Structure):
+    _fields_ = [("best", Tuning), ("best_ms", C.c_double), ("default_ms", C.c_double),
+                ("n_candidates", C.c_int32), ("crop_nx", C.c_int32), ("crop_ny", C.c_int32)]
+
+
 class Job(C.Structure):
     _fields_ = [("d_profiles", C.c_void_p), ("d_pos", C.c_void_p), ("h_pos", C.c_void_p),
                 ("grid", Grid), ("params", Params), ("d_image", C.c_void_p)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int32), ("interp", C.c_int32), ("precision", C.c_int32),
+                ("dtype", C.c_int32), ("hbm_budget_bytes", C.c_size_t),
+                ("max_active_jobs", C.c_int32), ("pulses_per_epoch", C.c_int32),
+                ("pixels_per_lane", C.c_int32), ("pulses_per_stage", C.c_int32),
+                ("epoch_watchdog_s", C.c_double)]
+
+
+class Request(C.Structure):
+    _fields_ = [("h_profiles", C.c_void_p), ("h_pos", C.c_void_p), ("grid", Grid),
+                ("params", Params), ("h_image", C.c_void_p)]
+
+
+class JobInfo(C.Structure):
+    _fields_ = [("state", C.c_int32), ("status", C.c_int32), ("epochs", C.c_int32),
+                ("reserved", C.c_int32), ("submit_s", C.c_double), ("admit_s", C.c_double),
+                ("first_epoch_s", C.c_double), ("done_s", C.c_double), ("work", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
+                ("cancelled", C.c_int64), ("epochs", C.c_int64), ("queued", C.c_int32),
+                ("active", C.c_int32), ("work_done", C.c_double), ("gpu_busy_s", C.c_double),
+                ("uptime_s", C.c_double), ("hbm_in_use", C.c_size_t), ("hbm_peak", C.c_size_t)]
+
+
+JOB_QUEUED, JOB_RESIDENT, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = range(6)
 
 
 class SarbpError(RuntimeError):
@@ -98,6 +132,18 @@ _SIGS = {
     "sarbp_get_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(Tuning)]),
     "sarbp_backproject_batch": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_int32, C.c_void_p]),
     "sarbp_sinc8_coefficients": (C.c_int, [C.c_double, C.POINTER(C.c_float)]),
+    "sarbp_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
+    "sarbp_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+    "sarbp_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(JobInfo)]),
+    "sarbp_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.c_double, C.POINTER(JobInfo)]),
+    "sarbp_server_cancel": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "sarbp_server_release": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "sarbp_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
+    "sarbp_server_drain": (C.c_int, [C.c_void_p, C.c_double]),
+    "sarbp_server_destroy": (None, [C.c_void_p]),
+    "sarbp_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Grid),
+                                 C.POINTER(Params), C.c_void_p, C.c_size_t, C.c_int32, C.c_void_p,
+                                 C.POINTER(AutotuneResult)]),
 }
 
 
@@ -200,6 +246,25 @@ class Context:
               "sarbp_get_tuning")
         return t.pixels_per_lane, t.pulses_per_stage
 
+    def autotune(self, d_profiles: int, d_pos: int, h_pos, grid: Grid, params: Params,
+                 d_scratch: int, scratch_pixels: int, max_crop_px: int = 1024,
+                 stream: int = 0) -> dict:
+        """Runs the library autotuner; the winner becomes this context's tuning."""
+        import numpy as np
+        hp = None
+        if h_pos is not None:
+            self._tune_pos = np.ascontiguousarray(h_pos, dtype=np.float64)
+            hp = self._tune_pos.ctypes.data_as(C.c_void_p)
+        r = AutotuneResult()
+        check(self.lib.sarbp_autotune(self.handle, C.c_void_p(d_profiles), C.c_void_p(d_pos), hp,
+                                      C.byref(grid), C.byref(params), C.c_void_p(d_scratch),
+                                      int(scratch_pixels), int(max_crop_px), C.c_void_p(stream),
+                                      C.byref(r)), "sarbp_autotune")
+        return {"pixels_per_lane": r.best.pixels_per_lane,
+                "pulses_per_stage": r.best.pulses_per_stage, "best_ms": r.best_ms,
+                "default_ms": r.default_ms, "n_candidates": r.n_candidates,
+                "crop": [r.crop_nx, r.crop_ny]}
+
     def plan(self, h_pos, grid: Grid, params: Params) -> "Plan":
         return Plan(self, h_pos, grid, params)
 
@@ -245,6 +310,94 @@ class Plan:
         if self.handle:
             self.lib.sarbp_plan_destroy(self.handle)
             self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _info_dict(i: JobInfo) -> dict:
+    return {f: getattr(i, f) for f, _ in JobInfo._fields_}
+
+
+class Server:
+    """Continuous-batching BP server (sarbp_server_*).  Host arrays passed to
+    submit() are kept referenced until the job is released."""
+
+    def __init__(self, *, device=0, interp="sinc8", precision="ref", dtype="c64",
+                 hbm_budget_bytes=8 << 30, max_active_jobs=0, pulses_per_epoch=1024,
+                 pixels_per_lane=0, pulses_per_stage=0, epoch_watchdog_s=0.0):
+        self.lib = load()
+        c = ServerConfig()
+        c.device = device
+        c.interp = INTERP_NAMES[interp] if isinstance(interp, str) else int(interp)
+        c.precision = PREC_NAMES[precision] if isinstance(precision, str) else int(precision)
+        c.dtype = DTYPE_NAMES[dtype] if isinstance(dtype, str) else int(dtype)
+        c.hbm_budget_bytes = int(hbm_budget_bytes)
+        c.max_active_jobs = int(max_active_jobs)
+        c.pulses_per_epoch = int(pulses_per_epoch)
+        c.pixels_per_lane = int(pixels_per_lane)
+        c.pulses_per_stage = int(pulses_per_stage)
+        c.epoch_watchdog_s = float(epoch_watchdog_s)
+        h = C.c_void_p()
+        check(self.lib.sarbp_server_create(C.byref(c), C.byref(h)), "sarbp_server_create")
+        self.handle = h
+        self.config = c
+        self._keep = {}
+
+    def submit(self, profiles, pos, grid: Grid, params: Params, image) -> int:
+        """profiles: host array (complex64, or uint32 bf16x2 words); pos: float64
+        [n][3]; image: complex64 [ny][nx] host array written when done."""
+        r = Request()
+        r.h_profiles = profiles.ctypes.data if profiles.size else None
+        r.h_pos = pos.ctypes.data if pos.size else None
+        r.grid = grid
+        r.params = params
+        r.h_image = image.ctypes.data if image.size else None
+        jid = C.c_uint64()
+        check(self.lib.sarbp_server_submit(self.handle, C.byref(r), C.byref(jid)),
+              "sarbp_server_submit")
+        self._keep[jid.value] = (profiles, pos, image)
+        return jid.value
+
+    def poll(self, jid: int) -> dict:
+        i = JobInfo()
+        st = self.lib.sarbp_server_poll(self.handle, jid, C.byref(i))
+        if st not in (OK, ERR_NOT_READY):
+            check(st, "sarbp_server_poll")
+        return _info_dict(i)
+
+    def wait(self, jid: int, timeout_s: float = -1.0, raise_on_error: bool = True) -> dict:
+        i = JobInfo()
+        st = self.lib.sarbp_server_wait(self.handle, jid, float(timeout_s), C.byref(i))
+        if raise_on_error:
+            check(st, "sarbp_server_wait")
+        d = _info_dict(i)
+        d["wait_status"] = st
+        return d
+
+    def cancel(self, jid: int) -> None:
+        check(self.lib.sarbp_server_cancel(self.handle, jid), "sarbp_server_cancel")
+
+    def release(self, jid: int) -> None:
+        check(self.lib.sarbp_server_release(self.handle, jid), "sarbp_server_release")
+        self._keep.pop(jid, None)
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        check(self.lib.sarbp_server_get_stats(self.handle, C.byref(s)), "sarbp_server_get_stats")
+        return {f: getattr(s, f) for f, _ in ServerStats._fields_}
+
+    def drain(self, timeout_s: float = -1.0) -> None:
+        check(self.lib.sarbp_server_drain(self.handle, float(timeout_s)), "sarbp_server_drain")
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.sarbp_server_destroy(self.handle)
+            self.handle = C.c_void_p()
+            self._keep.clear()
 
     def __del__(self):
         try:

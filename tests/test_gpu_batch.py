@@ -96,3 +96,21 @@ def test_mismatched_interp_rejected(ctx, jobs):
     b.params.interp = native.INTERP_LINEAR
     with pytest.raises(native.SarbpError):
         ctx.backproject_batch([a, b], 0)
+
+
+def test_graph_replays_with_eager_work_between(ctx, jobs):
+    """The batched launch is a captured graph replayed over a rewritten job
+    table: several replays, with eager launches and host syncs between them,
+    each reproduce the eager result bit for bit."""
+    import torch
+    scs, dss = jobs
+    stream = torch.cuda.current_stream().cuda_stream
+    eager = [D.backproject(ctx, ds, interp="sinc8").cpu().numpy() for ds in dss]
+    for rep in range(3):
+        imgs = [ds.new_image() for ds in dss]
+        order = list(range(len(dss)))[rep:] + list(range(len(dss)))[:rep]
+        ctx.backproject_batch([_job(dss[i], imgs[i]) for i in order], stream)
+        _ = D.backproject(ctx, dss[rep], interp="linear")  # eager work in between
+        torch.cuda.synchronize()
+        for i in range(len(dss)):
+            assert np.array_equal(imgs[i].cpu().numpy().view(np.uint64), eager[i].view(np.uint64)), (rep, i)

@@ -1,0 +1,139 @@
+"""The continuous-batching server (sarbp_server_*): every served image matches
+the FP64 oracle (rel-L2 <= 1e-4) and is bitwise the same alone and in a mix;
+requests arriving while others run join mid-flight; admission respects the
+HBM budget; cancellation and the epoch watchdog (forced from the host with a
+tiny limit -- the kernels themselves always finish) report through status."""
+import time
+
+import numpy as np
+import pytest
+
+from sar_cambrianaccelerationcard_amd import metrics as M
+from sar_cambrianaccelerationcard_amd import native
+from sar_cambrianaccelerationcard_amd import scene as S
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene(seed, n_pulses, nx, ny):
+    cfg = S.scaled_config(S.load_config("config0"), n_pulses=n_pulses, nx=nx, ny=ny)
+    cfg["seed"] = seed
+    cfg["image"]["interp"] = "sinc8"
+    return S.make_scene(cfg)
+
+
+@pytest.fixture(scope="module")
+def scenes():
+    sizes = [(300, 160, 96), (64, 64, 64), (200, 96, 200), (128, 130, 70), (96, 64, 32)]
+    return [_scene(2000 + i, *s) for i, s in enumerate(sizes)]
+
+
+def _submit(srv, sc):
+    img = np.zeros((sc.grid["ny"], sc.grid["nx"]), dtype=np.complex64)
+    jid = srv.submit(sc.profiles, sc.pos, sc.native_grid(), sc.native_params(interp="sinc8"), img)
+    return jid, img
+
+
+def test_served_images_match_oracle_and_alone(scenes):
+    print(f"[server] {len(scenes)} jobs, 100-pulse epochs", flush=True)
+    srv = native.Server(interp="sinc8", hbm_budget_bytes=1 << 30, pulses_per_epoch=100)
+    subs = [_submit(srv, sc) for sc in scenes]
+    infos = [srv.wait(j, timeout_s=120) for j, _ in subs]
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == len(scenes) and st["failed"] == 0
+    for sc, (jid, img), info in zip(scenes, subs, infos):
+        assert info["state"] == native.JOB_DONE
+        assert info["epochs"] == max(1, -(-sc.n_pulses // 100))
+        p = O.params(sc.c, sc.fc, sc.r0, sc.dr, "sinc8", sc.kaiser_beta)
+        ref = O.backproject(sc.profiles, sc.pos, sc.grid, p)
+        err = M.rel_l2(img, ref)
+        print(f"  job {jid}: {sc.n_pulses}x{sc.grid['nx']}x{sc.grid['ny']} rel_l2={err:.2e} "
+              f"latency={info['done_s'] - info['submit_s']:.3f}s", flush=True)
+        assert err <= 1e-4
+    # each job alone on a fresh server: bitwise identical
+    for sc, (_, img) in zip(scenes[:3], subs[:3]):
+        alone = native.Server(interp="sinc8", hbm_budget_bytes=1 << 30, pulses_per_epoch=100)
+        jid, a = _submit(alone, sc)
+        alone.wait(jid, timeout_s=120)
+        alone.close()
+        assert np.array_equal(a.view(np.uint64), img.view(np.uint64))
+
+
+def test_arrivals_join_while_running(scenes):
+    big = _scene(3001, 2048, 512, 512)
+    srv = native.Server(interp="sinc8", hbm_budget_bytes=1 << 30, pulses_per_epoch=64)
+    jb, _ = _submit(srv, big)
+    while srv.poll(jb)["state"] != native.JOB_RUNNING:
+        time.sleep(0.001)
+    late = [_submit(srv, sc) for sc in scenes[1:3]]
+    li = [srv.wait(j, timeout_s=120) for j, _ in late]
+    bi = srv.wait(jb, timeout_s=120)
+    srv.close()
+    for i in li:
+        assert i["admit_s"] < bi["done_s"]          # admitted while the big job ran
+        assert i["first_epoch_s"] < bi["done_s"]    # and shared its epochs
+        assert i["done_s"] < bi["done_s"]           # small jobs overtook it
+
+
+def test_budget_admission_and_rejection(scenes):
+    sc = scenes[0]
+    need = sc.profiles.nbytes + sc.pos.nbytes + sc.grid["nx"] * sc.grid["ny"] * 8
+    with pytest.raises(native.SarbpError) as ei:
+        native.Server(interp="sinc8", hbm_budget_bytes=need // 2).submit(
+            sc.profiles, sc.pos, sc.native_grid(), sc.native_params(interp="sinc8"),
+            np.zeros((sc.grid["ny"], sc.grid["nx"]), np.complex64))
+    assert ei.value.status == native.ERR_OVER_BUDGET
+    # room for one job at a time: the second waits for the first to finish
+    srv = native.Server(interp="sinc8", hbm_budget_bytes=int(need * 1.5) + 4096,
+                        pulses_per_epoch=50)
+    a, _ = _submit(srv, sc)
+    b, _ = _submit(srv, sc)
+    ia, ib = srv.wait(a, 120), srv.wait(b, 120)
+    st = srv.stats()
+    srv.close()
+    assert ib["admit_s"] >= ia["done_s"] - 1e-3
+    assert st["hbm_peak"] <= int(need * 1.5) + 4096
+
+
+def test_cancel_and_release(scenes):
+    srv = native.Server(interp="sinc8", hbm_budget_bytes=1 << 30, max_active_jobs=1,
+                        pulses_per_epoch=10)
+    a, _ = _submit(srv, scenes[0])
+    b, _ = _submit(srv, scenes[1])   # waits for the single slot
+    srv.cancel(b)
+    ib = srv.wait(b, 60, raise_on_error=False)
+    assert ib["state"] == native.JOB_CANCELLED and ib["wait_status"] == native.ERR_CANCELLED
+    srv.wait(a, 120)
+    srv.release(a)
+    srv.release(b)
+    with pytest.raises(native.SarbpError) as ei:
+        srv.poll(a)
+    assert ei.value.status == native.ERR_NOT_FOUND
+    srv.close()
+
+
+def test_watchdog_reports_timeout(scenes):
+    srv = native.Server(interp="sinc8", hbm_budget_bytes=1 << 30, pulses_per_epoch=1000,
+                        epoch_watchdog_s=1e-9)
+    jid, _ = _submit(srv, scenes[0])
+    info = srv.wait(jid, 60, raise_on_error=False)
+    srv.drain(60)
+    st = srv.stats()
+    srv.close()
+    assert info["state"] == native.JOB_FAILED and info["wait_status"] == native.ERR_TIMEOUT
+    assert st["hbm_in_use"] == 0
+
+
+def test_request_validation(scenes):
+    sc = scenes[1]
+    srv = native.Server(interp="sinc8", hbm_budget_bytes=1 << 28)
+    img = np.zeros((sc.grid["ny"], sc.grid["nx"]), np.complex64)
+    with pytest.raises(native.SarbpError):
+        srv.submit(sc.profiles, sc.pos, sc.native_grid(), sc.native_params(interp="linear"), img)
+    with pytest.raises(native.SarbpError):
+        srv.submit(sc.profiles, sc.pos, sc.native_grid(),
+                   sc.native_params(interp="sinc8", accumulate=True), img)
+    srv.close()

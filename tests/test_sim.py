@@ -66,8 +66,7 @@ def test_matched_filter_peak_position_and_amplitude():
     assert abs(k - (R - r0) / dr) <= 0.5
     assert abs(np.abs(out[0, k]) / 399.0 - 1) < 0.02
     phase = np.angle(out[0, k] * np.exp(4j * np.pi * 9.6e9 * R / 299792458.0))
-    frac = (R - r0) / dr - k  # residual delay: phase of the compressed pulse is flat near the peak
-    assert abs(phase) < 0.05 + abs(frac) * 0.0
+    assert abs(phase) < 0.05  # the centred chirp compresses to a real, positive peak
 
 
 def test_upsampling_preserves_samples():
