@@ -152,8 +152,9 @@ typedef struct sarbp_plan_info {
   int32_t tile_nx, tile_ny;     /* pixels per tile                           */
   int32_t window;               /* LDS range window per pulse, samples       */
   int32_t pulses_per_stage;     /* pulses staged per barrier interval        */
-  int32_t pixels_per_lane;
-  int32_t range_order;          /* 1 or 2: order of the per-pixel expansion  */
+  int32_t rows_per_lane, cols_per_lane;
+  int32_t range_order;          /* 1 if the first-order expansion holds over
+                                   the whole scene, else 0 (chosen per stage) */
   int32_t lds_bytes;            /* dynamic LDS per workgroup                 */
   double max_phase_err_rad;     /* bound on the expansion truncation error   */
 } sarbp_plan_info;
@@ -172,16 +173,20 @@ const char* sarbp_last_error_detail(void);
 
 /* ---- launch tuning -------------------------------------------------------- */
 
-/* Kernel configuration knobs chosen by the autotuner.  pixels_per_lane is the
- * number of image rows each lane accumulates (tile = 64 x (4*pixels_per_lane)
- * pixels), pulses_per_stage the pulses staged in LDS per barrier interval. */
+/* Kernel configuration knobs chosen by the autotuner.  A workgroup's tile is
+ * (64 cols_per_lane) x (4 rows_per_lane) pixels: each lane accumulates
+ * cols_per_lane columns (64 apart) of rows_per_lane rows.  pulses_per_stage
+ * is the number of pulses staged in LDS per barrier interval (reduced by the
+ * planner until the stage fits 64 KiB of LDS).  Shapes: rows 2, 4 or 8 with
+ * cols 1 or 2; rows 2 or 4 with cols 4 (SARBP_PREC_REF); cols 1 otherwise. */
 typedef struct sarbp_tuning {
-  int32_t pixels_per_lane;   /* 1, 2, 4 or 8                                 */
-  int32_t pulses_per_stage;  /* 8, 16, 32 or 64                              */
+  int32_t rows_per_lane;
+  int32_t cols_per_lane;
+  int32_t pulses_per_stage;
 } sarbp_tuning;
 
 /* Sets the configuration used by later calls with this (interp, precision,
- * dtype) triple on this context; {0,0} restores the built-in default. */
+ * dtype) triple on this context; all-zero restores the built-in default. */
 sarbp_status sarbp_set_tuning(sarbp_context* ctx, int32_t interp,
                               int32_t precision, int32_t dtype,
                               const sarbp_tuning* tuning);
@@ -199,10 +204,11 @@ typedef struct sarbp_autotune_result {
 } sarbp_autotune_result;
 
 /*
- * Project autotuner.  Times every (pixels_per_lane, pulses_per_stage)
- * configuration whose LDS footprint fits, on a centred crop of at most
- * max_crop_px x max_crop_px pixels of the given job (all pulses, the plan the
- * optional host positions allow), with HIP events on `stream`, and stores the
+ * Project autotuner.  Times every (tile shape, pulses_per_stage)
+ * configuration whose LDS footprint fits, on a centred crop of the given job
+ * (all pulses, the plan the optional host positions allow) that keeps up to
+ * max(max_crop_px, 4096) columns and holds about max_crop_px^2 pixels -- enough
+ * tiles to fill the device -- with HIP events on `stream`, and stores the
  * fastest as this context's tuning for the job's (interp, precision, dtype).
  * d_scratch receives the crop images and must hold scratch_pixels >=
  * crop_nx * crop_ny complex64 values.  Synchronous.
@@ -259,7 +265,8 @@ typedef struct sarbp_server_config {
   size_t hbm_budget_bytes;          /* device arena for resident jobs         */
   int32_t max_active_jobs;          /* resident jobs at once (0: no limit)    */
   int32_t pulses_per_epoch;         /* pulse chunk per job per epoch (0: all) */
-  int32_t pixels_per_lane;          /* launch tuning (0: built-in default)    */
+  int32_t rows_per_lane;            /* launch tuning (0: built-in default)    */
+  int32_t cols_per_lane;
   int32_t pulses_per_stage;
   double epoch_watchdog_s;          /* epochs slower than this fail their jobs
                                        with SARBP_ERR_TIMEOUT (0: off)        */
@@ -323,8 +330,8 @@ void sarbp_server_destroy(sarbp_server* server);
 /* ---- diagnostics -------------------------------------------------------------- */
 
 /* The 64 coefficients the SINC8 interpolator uses for Kaiser shape `beta`:
- * out[k * 8 + t] multiplies e^k, e = f - 1/2, in the weight of tap t (sample
- * floor(u) - 3 + t) at fractional bin f.  Host-only, no device needed. */
+ * out[k * 8 + t] multiplies f^k in the weight of tap t (sample floor(u) - 3 + t)
+ * at fractional bin f = u - floor(u).  Host-only, no device needed. */
 sarbp_status sarbp_sinc8_coefficients(double beta, float* out);
 
 #ifdef __cplusplus

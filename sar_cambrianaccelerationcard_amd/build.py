@@ -34,7 +34,8 @@ HIP_FLAGS = [
 ]
 
 
-SARBP_SOURCES = ("sarbp_kernels.hip", "sarbp_api.hip", "sarbp_batch.hip", "sarbp_server.hip")
+SARBP_SOURCES = ("sarbp_kernels_nearest.hip", "sarbp_kernels_linear.hip", "sarbp_kernels_sinc8.hip",
+                 "sarbp_api.hip", "sarbp_batch.hip", "sarbp_server.hip")
 
 
 def _run(cmd: list[str]) -> None:
@@ -57,7 +58,8 @@ def build_sarbp(force: bool = True) -> str:
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libsarbp.so")
     srcs = [os.path.join(CSRC, f) for f in SARBP_SOURCES]
-    deps = srcs + [os.path.join(CSRC, h) for h in ("sarbp_internal.h", "sarbp_host.h")] + \
+    deps = srcs + [os.path.join(CSRC, h) for h in ("sarbp_internal.h", "sarbp_host.h",
+                                                   "sarbp_kernels.inc")] + \
         [os.path.join(INCLUDE, "sarbp.h")]
     if not force and _newer(out, deps):
         return out

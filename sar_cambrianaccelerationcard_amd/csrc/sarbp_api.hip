@@ -58,33 +58,35 @@ double kaiser_sinc8(double x, double beta, double inv_i0b) {
   return s * w;
 }
 
-// Coefficients C[k * 8 + t] of e^k, e = f - 1/2, in the weight of sinc8 tap t
-// (sample i0 - 3 + t) at fractional bin f: the degree-7 polynomial that
-// interpolates K(e + 1/2 + 3 - t) at the 8 Chebyshev nodes of [-1/2, 1/2]
-// (near-minimax; max error ~2e-6 for beta = 6).
+// Coefficients C[k * 8 + t] of f^k in the weight of sinc8 tap t (sample
+// i0 - 3 + t) at fractional bin f in [0, 1): the degree-7 polynomial that
+// interpolates K(f + 3 - t) at the 8 Chebyshev nodes of [0, 1]
+// (near-minimax; max error ~2e-6 for beta = 6).  Solved as an 8x8 Vandermonde
+// system in FP64 with partial pivoting.
 void sinc8_poly(double beta, float* out) {
-  const int N = kSincCoef;
+  constexpr int N = kSincCoef;
   const double inv_i0b = 1.0 / bessel_i0(beta);
-  // monomial coefficients of the Chebyshev polynomials T_0..T_7 in x
-  double T[kSincCoef][kSincCoef] = {};
-  T[0][0] = 1.0;
-  T[1][1] = 1.0;
-  for (int k = 2; k < N; ++k)
-    for (int j = 0; j < N; ++j) T[k][j] = (j > 0 ? 2.0 * T[k - 1][j - 1] : 0.0) - T[k - 2][j];
+  double nodes[N];
+  for (int n = 0; n < N; ++n) nodes[n] = 0.5 + 0.5 * std::cos(kPi * (n + 0.5) / N);
   for (int t = 0; t < kSincTaps; ++t) {
-    double g[kSincCoef];
+    double A[N][N + 1];
     for (int n = 0; n < N; ++n) {
-      const double x = std::cos(kPi * (n + 0.5) / N);  // node in [-1, 1], e = x / 2
-      g[n] = kaiser_sinc8(0.5 * x + 0.5 + 3.0 - t, beta, inv_i0b);
+      double v = 1.0;
+      for (int k = 0; k < N; ++k, v *= nodes[n]) A[n][k] = v;
+      A[n][N] = kaiser_sinc8(nodes[n] + 3.0 - t, beta, inv_i0b);
     }
-    double mono[kSincCoef] = {};
-    for (int k = 0; k < N; ++k) {
-      double ak = 0.0;
-      for (int n = 0; n < N; ++n) ak += g[n] * std::cos(kPi * k * (n + 0.5) / N);
-      ak *= (k == 0 ? 1.0 : 2.0) / N;
-      for (int j = 0; j < N; ++j) mono[j] += ak * T[k][j];
+    for (int c = 0; c < N; ++c) {
+      int piv = c;
+      for (int r = c + 1; r < N; ++r)
+        if (std::fabs(A[r][c]) > std::fabs(A[piv][c])) piv = r;
+      for (int k = 0; k <= N; ++k) std::swap(A[c][k], A[piv][k]);
+      for (int r = 0; r < N; ++r) {
+        if (r == c) continue;
+        const double m = A[r][c] / A[c][c];
+        for (int k = c; k <= N; ++k) A[r][k] -= m * A[c][k];
+      }
     }
-    for (int j = 0; j < N; ++j) out[j * kSincTaps + t] = (float)(mono[j] * std::ldexp(1.0, j));  // x = 2e
+    for (int k = 0; k < N; ++k) out[k * kSincTaps + t] = (float)(A[k][N] / A[k][k]);
   }
 }
 
@@ -113,7 +115,8 @@ struct sarbp_context {
 struct sarbp_plan {
   sarbp_context* ctx;
   BpArgs args;
-  int interp, precision, dtype, ppl;
+  int interp, precision, dtype;
+  sarbp_tuning tune;
   sarbp_plan_info info;
 };
 
@@ -147,15 +150,21 @@ sarbp_tuning get_tuning(sarbp_context* ctx, int interp, int prec, int dtype) {
 }
 
 sarbp_tuning default_tuning(int interp) {
-  // Built-in defaults: the best points of a (pixels/lane x pulses/stage) sweep
+  // Built-in defaults: the best points of a (tile shape x pulses/stage) sweep
   // on MI355X at 0.25 m pixels over a 4096-pulse aperture.  Sinc8's LDS
   // footprint per pulse is ~9x linear's, so it stages fewer pulses to keep
   // several workgroups resident per CU.
   sarbp_tuning t;
-  t.pixels_per_lane = 8;
-  t.pulses_per_stage = interp == SARBP_INTERP_SINC8 ? 8 : 32;
+  t.cols_per_lane = 1;
+  t.rows_per_lane = interp == SARBP_INTERP_SINC8 ? 8 : 4;
+  t.pulses_per_stage = interp == SARBP_INTERP_SINC8 ? 6 : 32;
   return t;
 }
+
+// Largest |pixel - tile reference| the FP32 differential supports: pp = |d|^2
+// - s^2 carries ~ulp(dmax^2) of rounding into q = pp / (2R), which must stay
+// far below a wavelength (at 48 m, ~1e-7 m for R >= 3 km).
+constexpr double kMaxTileRadius = 48.0;
 
 // Range extent (metres, half-width) of one tile seen from any pulse, and the
 // minimum slant range over the scene.  Without positions: triangle inequality
@@ -169,10 +178,10 @@ struct Extent {
   double rmin;     // lower bound on slant range (0 if unknown)
 };
 
-Extent tile_extent(const sarbp_grid* g, int ppl, const double* h_pos, int n_pulses) {
-  const int ty = kWaves * ppl;
-  // Offsets of tile pixels from the reference pixel: di in [-32, 31], dj in [-ty/2, ty/2-1].
-  const int di_lo = -kTileX / 2, di_hi = kTileX / 2 - 1, dj_lo = -ty / 2, dj_hi = ty / 2 - 1;
+Extent tile_extent(const sarbp_grid* g, int px, int py, const double* h_pos, int n_pulses) {
+  const int tx = kTileX * px, ty = kWaves * py;
+  // Offsets of tile pixels from the reference pixel: di in [-tx/2, tx/2-1], dj in [-ty/2, ty/2-1].
+  const int di_lo = -tx / 2, di_hi = tx / 2 - 1, dj_lo = -ty / 2, dj_hi = ty / 2 - 1;
   double dmax = 0.0;
   for (int ci = 0; ci < 4; ++ci) {
     const int di = (ci & 1) ? di_hi : di_lo, dj = (ci & 2) ? dj_hi : dj_lo;
@@ -190,7 +199,7 @@ Extent tile_extent(const sarbp_grid* g, int ppl, const double* h_pos, int n_puls
   for (double& v : n) v /= nn;
   double corners[4][3];
   for (int ci = 0; ci < 4; ++ci) {
-    const double ix = (ci & 1) ? std::max(g->nx - 1, 0) + kTileX : -kTileX;
+    const double ix = (ci & 1) ? std::max(g->nx - 1, 0) + tx : -tx;
     const double iy = (ci & 2) ? std::max(g->ny - 1, 0) + ty : -ty;
     for (int k = 0; k < 3; ++k) corners[ci][k] = g->origin[k] + ix * g->ux[k] + iy * g->uy[k];
   }
@@ -217,7 +226,7 @@ Extent tile_extent(const sarbp_grid* g, int ppl, const double* h_pos, int n_puls
     bmax = std::max(bmax, std::min(luy, ny_ / std::fabs(sep)));
     rmin = std::min(rmin, std::fabs(sep));
   }
-  const double hx = kTileX / 2, hy = ty / 2;
+  const double hx = tx / 2, hy = ty / 2;
   const double q = dmax * dmax / rmin;
   e.half_m = std::min(dmax, amax * hx + bmax * hy + q);
   e.rmin = rmin;
@@ -228,9 +237,14 @@ Extent tile_extent(const sarbp_grid* g, int ppl, const double* h_pos, int n_puls
 sarbp_status plan_args(const double* h_pos, const sarbp_grid* g,
                        const sarbp_params* p, const sarbp_tuning& tune, BpArgs* a,
                        sarbp_plan_info* info) {
-  const int ppl = tune.pixels_per_lane;
-  if (ppl != 2 && ppl != 4 && ppl != 8) return fail(SARBP_ERR_INVALID_ARGUMENT, "pixels_per_lane %d", ppl);
-  const Extent ext = tile_extent(g, ppl, h_pos, p->n_pulses);
+  const int px = tune.cols_per_lane, py = tune.rows_per_lane;
+  if (!tile_shape_supported(p->precision, px, py))
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "unsupported tile shape: %d cols x %d rows per lane", px, py);
+  const Extent ext = tile_extent(g, px, py, h_pos, p->n_pulses);
+  if (p->precision == SARBP_PREC_REF && ext.dmax > kMaxTileRadius)
+    return fail(SARBP_ERR_UNSUPPORTED,
+                "tile of %d x %d pixels spans %.1f m from its centre (limit %.0f m): use fewer pixels per lane",
+                kTileX * px, kWaves * py, ext.dmax, kMaxTileRadius);
   // half window in samples: extent + sinc taps (3 left / 4 right) + 2 margin
   const double half_bins = std::ceil(ext.half_m / p->dr) + 6.0;
   if (half_bins > 4096.0)
@@ -257,8 +271,8 @@ sarbp_status plan_args(const double* h_pos, const sarbp_grid* g,
   a->n_samples = p->n_samples;
   a->nx = g->nx;
   a->ny = g->ny;
-  const int ty = kWaves * ppl;
-  a->tiles_x = (g->nx + kTileX - 1) / kTileX;
+  const int tx = kTileX * px, ty = kWaves * py;
+  a->tiles_x = (g->nx + tx - 1) / tx;
   a->tiles_y = (g->ny + ty - 1) / ty;
   a->win = win;
   a->half = win / 2;
@@ -272,11 +286,12 @@ sarbp_status plan_args(const double* h_pos, const sarbp_grid* g,
   if (info) {
     info->tiles_x = a->tiles_x;
     info->tiles_y = a->tiles_y;
-    info->tile_nx = kTileX;
+    info->tile_nx = tx;
     info->tile_ny = ty;
     info->window = win;
     info->pulses_per_stage = P;
-    info->pixels_per_lane = ppl;
+    info->rows_per_lane = py;
+    info->cols_per_lane = px;
     info->lds_bytes = (int32_t)lds_layout(p->interp, win, P).total;
     const double rm = ext.rmin > ext.dmax ? ext.rmin - ext.dmax : 0.0;
     const double d2 = ext.dmax * ext.dmax;
@@ -286,9 +301,9 @@ sarbp_status plan_args(const double* h_pos, const sarbp_grid* g,
   return SARBP_OK;
 }
 
-sarbp_status launch(const BpArgs& a, const sarbp_params* p, int ppl, hipStream_t s) {
+sarbp_status launch(const BpArgs& a, const sarbp_params* p, const sarbp_tuning& t, hipStream_t s) {
   if (a.nx == 0 || a.ny == 0) return SARBP_OK;
-  hipError_t e = launch_plain(a, p->interp, p->precision, p->dtype, ppl, s);
+  hipError_t e = launch_plain(a, p->interp, p->precision, p->dtype, t.cols_per_lane, t.rows_per_lane, s);
   if (e != hipSuccess) return hip_fail(e, "BP kernel launch");
   return SARBP_OK;
 }
@@ -350,12 +365,13 @@ sarbp_status sarbp_set_tuning(sarbp_context* ctx, int32_t interp, int32_t precis
   if (interp < 0 || interp > 2 || precision < 0 || precision > 2 || dtype < 0 || dtype > 1)
     return fail(SARBP_ERR_INVALID_ARGUMENT, "bad (interp, precision, dtype)");
   std::lock_guard<std::mutex> lk(ctx->mu);
-  if (t->pixels_per_lane == 0 && t->pulses_per_stage == 0) {
+  if (t->rows_per_lane == 0 && t->cols_per_lane == 0 && t->pulses_per_stage == 0) {
     ctx->tuning.erase(tuning_key(interp, precision, dtype));
     return SARBP_OK;
   }
-  if (t->pixels_per_lane != 2 && t->pixels_per_lane != 4 && t->pixels_per_lane != 8)
-    return fail(SARBP_ERR_INVALID_ARGUMENT, "pixels_per_lane must be 2, 4 or 8");
+  if (!tile_shape_supported(precision, t->cols_per_lane, t->rows_per_lane))
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "unsupported tile shape: %d cols x %d rows per lane",
+                t->cols_per_lane, t->rows_per_lane);
   if (t->pulses_per_stage < 1 || t->pulses_per_stage > 256)
     return fail(SARBP_ERR_INVALID_ARGUMENT, "pulses_per_stage must be in [1, 256]");
   ctx->tuning[tuning_key(interp, precision, dtype)] = *t;
@@ -388,7 +404,7 @@ sarbp_status sarbp_backproject(sarbp_context* ctx, const void* d_profiles, const
   a.prof = d_profiles;
   a.pos = d_pos;
   a.img = reinterpret_cast<float2*>(d_image);
-  return launch(a, params, tune.pixels_per_lane, (hipStream_t)stream);
+  return launch(a, params, tune, (hipStream_t)stream);
 }
 
 sarbp_status sarbp_backproject_sync(sarbp_context* ctx, const void* d_profiles, const double* d_pos,
@@ -429,7 +445,7 @@ sarbp_status sarbp_plan_create(sarbp_context* ctx, const double* h_pos, const sa
   pl->interp = params->interp;
   pl->precision = params->precision;
   pl->dtype = params->dtype;
-  pl->ppl = tune.pixels_per_lane;
+  pl->tune = tune;
   *out = pl;
   return SARBP_OK;
 }
@@ -450,7 +466,7 @@ sarbp_status sarbp_plan_execute(sarbp_plan* pl, const void* d_profiles, const do
   p.interp = pl->interp;
   p.precision = pl->precision;
   p.dtype = pl->dtype;
-  return launch(a, &p, pl->ppl, (hipStream_t)stream);
+  return launch(a, &p, pl->tune, (hipStream_t)stream);
 }
 
 sarbp_status sarbp_plan_get_info(const sarbp_plan* pl, sarbp_plan_info* info) {
@@ -505,8 +521,7 @@ sarbp_status sarbp_backproject_batch(sarbp_context* ctx, const sarbp_job* jobs, 
   if (desc.empty()) return SARBP_OK;
   if (!ctx->batch && !(ctx->batch = new (std::nothrow) BatchExec))
     return fail(SARBP_ERR_OUT_OF_MEMORY, "batch executor");
-  return ctx->batch->run(desc, p0.interp, p0.precision, p0.dtype, tune.pixels_per_lane, ctx->n_cu,
-                         (hipStream_t)stream);
+  return ctx->batch->run(desc, p0.interp, p0.precision, p0.dtype, tune, ctx->n_cu, (hipStream_t)stream);
 }
 
 sarbp_status sarbp_autotune(sarbp_context* ctx, const void* d_profiles, const double* d_pos,
@@ -519,9 +534,13 @@ sarbp_status sarbp_autotune(sarbp_context* ctx, const void* d_profiles, const do
   if (params->n_pulses < 1 || !d_profiles || !d_pos || !d_scratch)
     return fail(SARBP_ERR_INVALID_ARGUMENT, "autotune needs pulses and device buffers");
   if (max_crop_px < kTileX) return fail(SARBP_ERR_INVALID_ARGUMENT, "max_crop_px < %d", kTileX);
+  // The crop must hold enough tiles to fill every CU at full occupancy, or
+  // small-tile configurations win on tail effects alone: keep up to 4096
+  // columns and as many rows as a max_crop_px^2 pixel budget allows.
   sarbp_grid crop = *grid;
-  crop.nx = std::min(grid->nx, max_crop_px);
-  crop.ny = std::min(grid->ny, max_crop_px);
+  const int64_t budget = (int64_t)max_crop_px * max_crop_px;
+  crop.nx = std::min(grid->nx, std::max(max_crop_px, 4096));
+  crop.ny = (int32_t)std::min<int64_t>(grid->ny, std::max<int64_t>(1, budget / std::max(crop.nx, 1)));
   if (crop.nx < 1 || crop.ny < 1) return fail(SARBP_ERR_INVALID_ARGUMENT, "empty image");
   if ((size_t)crop.nx * crop.ny > scratch_pixels)
     return fail(SARBP_ERR_INVALID_ARGUMENT, "scratch holds %zu pixels, crop needs %d", scratch_pixels,
@@ -548,12 +567,12 @@ sarbp_status sarbp_autotune(sarbp_context* ctx, const void* d_profiles, const do
     a.prof = d_profiles;
     a.pos = d_pos;
     a.img = reinterpret_cast<float2*>(d_scratch);
-    if ((r = launch(a, &p, t.pixels_per_lane, s)) != SARBP_OK) return r;  // warm-up
+    if ((r = launch(a, &p, t, s)) != SARBP_OK) return r;  // warm-up
     hipError_t he;
     if ((he = hipEventRecord(e0, s)) != hipSuccess) return hip_fail(he, "hipEventRecord");
     const int reps = 3;
     for (int i = 0; i < reps; ++i)
-      if ((r = launch(a, &p, t.pixels_per_lane, s)) != SARBP_OK) return r;
+      if ((r = launch(a, &p, t, s)) != SARBP_OK) return r;
     if ((he = hipEventRecord(e1, s)) != hipSuccess || (he = hipEventSynchronize(e1)) != hipSuccess)
       return hip_fail(he, "hipEventSynchronize");
     float f = 0.0f;
@@ -562,28 +581,30 @@ sarbp_status sarbp_autotune(sarbp_context* ctx, const void* d_profiles, const do
     return SARBP_OK;
   };
   // default first (built-in, ignoring any stored tuning)
-  sarbp_tuning def;
-  def.pixels_per_lane = 8;
-  def.pulses_per_stage = p.interp == SARBP_INTERP_SINC8 ? 8 : 32;
+  const sarbp_tuning def = default_tuning(p.interp);
   int eff = 0;
-  double def_ms = 0.0;
+  double def_ms = INFINITY;
   st = time_one(def, &def_ms, &eff);
+  if (st == SARBP_ERR_UNSUPPORTED) st = SARBP_OK;  // default shape too large for this grid
   sarbp_tuning best = def;
+  best.pulses_per_stage = eff;
   double best_ms = def_ms;
-  int n = st == SARBP_OK ? 1 : 0;
-  std::vector<std::pair<int, int>> seen;
-  seen.emplace_back(def.pixels_per_lane, eff);
-  const int ppls[] = {2, 4, 8};
+  int n = std::isfinite(def_ms) ? 1 : 0;
+  std::vector<std::tuple<int, int, int>> seen;
+  seen.emplace_back(def.cols_per_lane, def.rows_per_lane, eff);
+  const int shapes[][2] = {{1, 2}, {1, 4}, {1, 8}, {2, 2}, {2, 4}, {2, 8}, {4, 2}, {4, 4}};
   const int stages[] = {4, 6, 8, 12, 16, 24, 32, 48, 64};
-  for (int ppl : ppls) {
+  for (const auto& sh : shapes) {
     if (st != SARBP_OK) break;
+    if (!tile_shape_supported(p.precision, sh[0], sh[1])) continue;
     for (int P : stages) {
       sarbp_tuning t;
-      t.pixels_per_lane = ppl;
+      t.cols_per_lane = sh[0];
+      t.rows_per_lane = sh[1];
       t.pulses_per_stage = P;
       BpArgs probe;
       if (plan_args(h_pos, &crop, &p, t, &probe, nullptr) != SARBP_OK) continue;
-      const std::pair<int, int> key(ppl, probe.pulses_per_stage);
+      const auto key = std::make_tuple(sh[0], sh[1], probe.pulses_per_stage);
       if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
       seen.push_back(key);
       double ms = 0.0;

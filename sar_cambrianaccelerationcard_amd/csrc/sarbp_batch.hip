@@ -7,6 +7,48 @@
 
 namespace sarbp {
 
+namespace {
+__global__ void reset_counter_kernel(unsigned long long* counter) { *counter = 0ull; }
+}  // namespace
+
+hipError_t reset_counter(unsigned long long* d_counter, hipStream_t stream) {
+  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, d_counter);
+  return hipGetLastError();
+}
+
+hipError_t launch_plain(const BpArgs& a, int interp, int precision, int dtype, int px, int py,
+                        hipStream_t stream) {
+  switch (interp) {
+    case kNearest: return launch_plain_nearest(a, precision, dtype, px, py, stream);
+    case kLinear: return launch_plain_linear(a, precision, dtype, px, py, stream);
+    case kSinc8: return launch_plain_sinc8(a, precision, dtype, px, py, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_batch(const JobDesc* d_jobs, const BatchHeader* d_hdr, unsigned long long* d_counter,
+                        int interp, int precision, int dtype, int px, int py, size_t shm, int n_blocks,
+                        hipStream_t stream) {
+  switch (interp) {
+    case kNearest:
+      return launch_batch_nearest(d_jobs, d_hdr, d_counter, precision, dtype, px, py, shm, n_blocks, stream);
+    case kLinear:
+      return launch_batch_linear(d_jobs, d_hdr, d_counter, precision, dtype, px, py, shm, n_blocks, stream);
+    case kSinc8:
+      return launch_batch_sinc8(d_jobs, d_hdr, d_counter, precision, dtype, px, py, shm, n_blocks, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+int batch_blocks_per_cu(int interp, int precision, int dtype, int px, int py, size_t shm) {
+  switch (interp) {
+    case kNearest: return blocks_per_cu_nearest(precision, dtype, px, py, shm);
+    case kLinear: return blocks_per_cu_linear(precision, dtype, px, py, shm);
+    case kSinc8: return blocks_per_cu_sinc8(precision, dtype, px, py, shm);
+    default: return 1;
+  }
+}
+
 BatchExec::~BatchExec() {
   if (done_) {
     (void)hipEventSynchronize(done_);
@@ -57,8 +99,9 @@ sarbp_status BatchExec::reserve(size_t n) {
   return SARBP_OK;
 }
 
-sarbp_status BatchExec::run(std::vector<JobDesc>& jobs, int interp, int precision, int dtype, int ppl,
-                            int n_cu, hipStream_t s) {
+sarbp_status BatchExec::run(std::vector<JobDesc>& jobs, int interp, int precision, int dtype,
+                            const sarbp_tuning& tune, int n_cu, hipStream_t s) {
+  const int px = tune.cols_per_lane, py = tune.rows_per_lane;
   if (jobs.empty()) return SARBP_OK;
   hipError_t e;
   // The pinned staging copies of the previous run must have been consumed.
@@ -87,18 +130,18 @@ sarbp_status BatchExec::run(std::vector<JobDesc>& jobs, int interp, int precisio
   if ((e = hipMemcpyAsync(d_hdr_, h_hdr_, sizeof(BatchHeader), hipMemcpyHostToDevice, s)) != hipSuccess)
     return hip_fail(e, "hipMemcpyAsync(header)");
 
-  const auto key = std::make_tuple(interp, precision, dtype, ppl, shm_class);
+  const auto key = std::make_tuple(interp, precision, dtype, px, py, shm_class);
   auto it = graphs_.find(key);
   if (it == graphs_.end()) {
     if (!capture_stream_ && (e = hipStreamCreateWithFlags(&capture_stream_, hipStreamNonBlocking)) != hipSuccess)
       return hip_fail(e, "hipStreamCreate(capture)");
-    const int per_cu = batch_blocks_per_cu(interp, precision, dtype, ppl, shm_class);
+    const int per_cu = batch_blocks_per_cu(interp, precision, dtype, px, py, shm_class);
     const int n_blocks = std::max(1, n_cu * per_cu);
     if ((e = hipStreamBeginCapture(capture_stream_, hipStreamCaptureModeThreadLocal)) != hipSuccess)
       return hip_fail(e, "hipStreamBeginCapture");
     hipError_t e1 = reset_counter(d_counter_, capture_stream_);
-    hipError_t e2 = e1 == hipSuccess ? launch_batch(d_jobs_, d_hdr_, d_counter_, interp, precision, dtype, ppl,
-                                                    shm_class, n_blocks, capture_stream_)
+    hipError_t e2 = e1 == hipSuccess ? launch_batch(d_jobs_, d_hdr_, d_counter_, interp, precision, dtype, px,
+                                                    py, shm_class, n_blocks, capture_stream_)
                                      : e1;
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(capture_stream_, &g);

@@ -41,13 +41,13 @@ sarbp_tuning default_tuning(int interp);
 // Fills a launch descriptor (device pointers left NULL) for one image.
 sarbp_status plan_args(const double* h_pos, const sarbp_grid* g, const sarbp_params* p,
                        const sarbp_tuning& tune, BpArgs* a, sarbp_plan_info* info);
-sarbp_status launch(const BpArgs& a, const sarbp_params* p, int ppl, hipStream_t s);
+sarbp_status launch(const BpArgs& a, const sarbp_params* p, const sarbp_tuning& t, hipStream_t s);
 
 // Batched persistent launch replayed from a captured hipGraph.  The graph is
 // [counter reset kernel -> persistent BP kernel] and reads the job table and
 // its header from fixed device buffers, which each run rewrites (async H2D
 // from pinned staging) on the caller's stream before replaying the graph.
-// One graph is captured per (interp, precision, dtype, pixels/lane, LDS size
+// One graph is captured per (interp, precision, dtype, tile shape, LDS size
 // class); buffers grow on demand (which drops the cached graphs).  Not
 // thread-safe: callers serialise.
 class BatchExec {
@@ -59,8 +59,8 @@ class BatchExec {
 
   // Enqueues one batched launch over `jobs` (planned descriptors with device
   // pointers set; item_begin filled in here) on stream s.
-  sarbp_status run(std::vector<JobDesc>& jobs, int interp, int precision, int dtype, int ppl,
-                   int n_cu, hipStream_t s);
+  sarbp_status run(std::vector<JobDesc>& jobs, int interp, int precision, int dtype,
+                   const sarbp_tuning& tune, int n_cu, hipStream_t s);
   // Blocks until the last run has finished on the device.
   sarbp_status wait();
   // Event recorded after the last run (NULL before the first run).
@@ -80,7 +80,7 @@ class BatchExec {
   size_t cap_ = 0;
   hipEvent_t done_ = nullptr;
   hipStream_t capture_stream_ = nullptr;
-  std::map<std::tuple<int, int, int, int, size_t>, hipGraphExec_t> graphs_;
+  std::map<std::tuple<int, int, int, int, int, size_t>, hipGraphExec_t> graphs_;
   int captured_ = 0;
 };
 

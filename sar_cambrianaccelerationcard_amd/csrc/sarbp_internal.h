@@ -8,17 +8,18 @@
 
 namespace sarbp {
 
-// One image tile is 64 pixels along the grid's x axis (one wave64, lane = x)
-// by 4 * PPL pixels along y (4 waves, PPL rows per lane).
+// One image tile is 64 PX pixels along the grid's x axis by 4 PY along y: a
+// workgroup of 4 wave64s, each lane accumulating PX columns (stride 64) x PY
+// rows.
 constexpr int kTileX = 64;
 constexpr int kWaves = 4;
 constexpr int kThreads = kTileX * kWaves;
 // Sinc8: the weight of tap t (sample i0 - 3 + t) at fractional bin f is the
-// degree-7 polynomial sum_k C[k][t] e^k, e = f - 1/2.
+// degree-7 polynomial sum_k C[k][t] f^k.
 constexpr int kSincTaps = 8;
 constexpr int kSincCoef = 8;
-// Per-pulse parameter record in LDS (8 floats or 4 doubles).
-constexpr int kParBytes = 32;
+// Per-pulse parameter record in LDS (12 floats or 4 doubles).
+constexpr int kParBytes = 48;
 
 enum { kNearest = 0, kLinear = 1, kSinc8 = 2 };
 enum { kRef = 0, kF64 = 1, kF32 = 2 };
@@ -61,10 +62,10 @@ struct BatchHeader {
 //   raw   [P][win] float2         staged range windows
 //   coef  [P][win-7][8] float2    sinc8 only: polynomial coefficients D_k of
 //                                 the interpolant at each window position
-//   par   [P] 32-B pulse records
-//   start [P] int window starts, rc2 [P] float 2Rc, order int
+//   par   [P] 48-B pulse records
+//   start [P] int window starts, order int
 struct LdsLayout {
-  size_t raw, coef, par, start, rc2, order, total;
+  size_t raw, coef, par, start, order, total;
 };
 
 __host__ __device__ inline LdsLayout lds_layout(int interp, int win, int P) {
@@ -78,24 +79,43 @@ __host__ __device__ inline LdsLayout lds_layout(int interp, int win, int P) {
   o += (size_t)P * kParBytes;
   L.start = o;
   o += (size_t)P * 4;
-  L.rc2 = o;
-  o += (size_t)P * 4;
   L.order = o;
   o += 4;
   L.total = (o + 15) & ~(size_t)15;
   return L;
 }
 
+// Tile shapes (PX columns x PY rows per lane) with compiled kernels: every
+// shape for SARBP_PREC_REF, the PX = 1 shapes for the F64 and F32 modes.
+inline bool tile_shape_supported(int precision, int px, int py) {
+  const bool x1 = px == 1 && (py == 2 || py == 4 || py == 8);
+  if (precision != kRef) return x1;
+  return x1 || (px == 2 && (py == 2 || py == 4 || py == 8)) || (px == 4 && (py == 2 || py == 4));
+}
+
+// Per-interpolator entry points (sarbp_kernels_<interp>.hip).
+#define SARBP_DECLARE_TU(I)                                                                   \
+  hipError_t launch_plain_##I(const BpArgs& a, int precision, int dtype, int px, int py,      \
+                              hipStream_t stream);                                            \
+  hipError_t launch_batch_##I(const JobDesc* d_jobs, const BatchHeader* d_hdr,               \
+                              unsigned long long* d_counter, int precision, int dtype,        \
+                              int px, int py, size_t shm, int n_blocks, hipStream_t stream);  \
+  int blocks_per_cu_##I(int precision, int dtype, int px, int py, size_t shm);
+SARBP_DECLARE_TU(nearest)
+SARBP_DECLARE_TU(linear)
+SARBP_DECLARE_TU(sinc8)
+#undef SARBP_DECLARE_TU
+
 // Launches the plain tile kernel.  Returns hipSuccess or the launch error.
 hipError_t launch_plain(const BpArgs& a, int interp, int precision, int dtype,
-                        int ppl, hipStream_t stream);
+                        int px, int py, hipStream_t stream);
 
 // Launches the persistent batched kernel over the job table described by
 // *d_hdr (device memory); counter = device work counter (zeroed on the
 // stream before the launch); shm = LDS bytes >= every job's layout.
 hipError_t launch_batch(const JobDesc* d_jobs, const BatchHeader* d_hdr,
                         unsigned long long* d_counter, int interp,
-                        int precision, int dtype, int ppl, size_t shm,
+                        int precision, int dtype, int px, int py, size_t shm,
                         int n_blocks, hipStream_t stream);
 
 // Zeroes the batch work counter with a one-thread kernel (a kernel node, not a
@@ -103,6 +123,6 @@ hipError_t launch_batch(const JobDesc* d_jobs, const BatchHeader* d_hdr,
 hipError_t reset_counter(unsigned long long* d_counter, hipStream_t stream);
 
 // Workgroups of the batched kernel that fit one CU at the given LDS size.
-int batch_blocks_per_cu(int interp, int precision, int dtype, int ppl, size_t shm);
+int batch_blocks_per_cu(int interp, int precision, int dtype, int px, int py, size_t shm);
 
 }  // namespace sarbp

@@ -329,8 +329,7 @@ void sarbp_server::sched_loop() {
     if (e == hipSuccess) e = hipEventCreate(&ep.ev1);
     if (e == hipSuccess) e = hipEventRecord(ep.ev0, s_compute);
     sarbp_status st = e == hipSuccess
-                          ? exec[k & 1].run(desc, cfg.interp, cfg.precision, cfg.dtype, tune.pixels_per_lane,
-                                            n_cu, s_compute)
+                          ? exec[k & 1].run(desc, cfg.interp, cfg.precision, cfg.dtype, tune, n_cu, s_compute)
                           : hip_fail(e, "epoch events");
     if (st == SARBP_OK && (e = hipEventRecord(ep.ev1, s_compute)) != hipSuccess) st = hip_fail(e, "hipEventRecord");
     if (st != SARBP_OK) {
@@ -429,10 +428,12 @@ sarbp_status sarbp_server_create(const sarbp_server_config* cfg, sarbp_server** 
   if (!s) return fail(SARBP_ERR_OUT_OF_MEMORY, "server");
   s->cfg = *cfg;
   s->tune = default_tuning(cfg->interp);
-  if (cfg->pixels_per_lane) s->tune.pixels_per_lane = cfg->pixels_per_lane;
+  if (cfg->rows_per_lane) s->tune.rows_per_lane = cfg->rows_per_lane;
+  if (cfg->cols_per_lane) s->tune.cols_per_lane = cfg->cols_per_lane;
   if (cfg->pulses_per_stage) s->tune.pulses_per_stage = cfg->pulses_per_stage;
-  if (s->tune.pixels_per_lane != 2 && s->tune.pixels_per_lane != 4 && s->tune.pixels_per_lane != 8)
-    return fail(SARBP_ERR_INVALID_ARGUMENT, "pixels_per_lane must be 2, 4 or 8");
+  if (!tile_shape_supported(cfg->precision, s->tune.cols_per_lane, s->tune.rows_per_lane))
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "unsupported tile shape %d x %d per lane", s->tune.cols_per_lane,
+                s->tune.rows_per_lane);
   if (s->tune.pulses_per_stage < 1 || s->tune.pulses_per_stage > 256)
     return fail(SARBP_ERR_INVALID_ARGUMENT, "pulses_per_stage must be in [1, 256]");
   DeviceGuard dg(cfg->device);

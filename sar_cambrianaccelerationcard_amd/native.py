@@ -53,12 +53,14 @@ class PlanInfo(C.Structure):
     _fields_ = [("tiles_x", C.c_int32), ("tiles_y", C.c_int32),
                 ("tile_nx", C.c_int32), ("tile_ny", C.c_int32),
                 ("window", C.c_int32), ("pulses_per_stage", C.c_int32),
-                ("pixels_per_lane", C.c_int32), ("range_order", C.c_int32),
+                ("rows_per_lane", C.c_int32), ("cols_per_lane", C.c_int32),
+                ("range_order", C.c_int32),
                 ("lds_bytes", C.c_int32), ("max_phase_err_rad", C.c_double)]
 
 
 class Tuning(C.Structure):
-    _fields_ = [("pixels_per_lane", C.c_int32), ("pulses_per_stage", C.c_int32)]
+    _fields_ = [("rows_per_lane", C.c_int32), ("cols_per_lane", C.c_int32),
+                ("pulses_per_stage", C.c_int32)]
 
 
 class AutotuneResult(C.Structure):
@@ -75,7 +77,8 @@ class ServerConfig(C.Structure):
     _fields_ = [("device", C.c_int32), ("interp", C.c_int32), ("precision", C.c_int32),
                 ("dtype", C.c_int32), ("hbm_budget_bytes", C.c_size_t),
                 ("max_active_jobs", C.c_int32), ("pulses_per_epoch", C.c_int32),
-                ("pixels_per_lane", C.c_int32), ("pulses_per_stage", C.c_int32),
+                ("rows_per_lane", C.c_int32), ("cols_per_lane", C.c_int32),
+                ("pulses_per_stage", C.c_int32),
                 ("epoch_watchdog_s", C.c_double)]
 
 
@@ -235,8 +238,9 @@ class Context:
         check(f(self.handle, C.c_void_p(d_profiles), C.c_void_p(d_pos), C.byref(grid),
                 C.byref(params), C.c_void_p(d_image), C.c_void_p(stream)), "sarbp_backproject")
 
-    def set_tuning(self, interp: int, precision: int, dtype: int, ppl: int, pulses: int) -> None:
-        t = Tuning(ppl, pulses)
+    def set_tuning(self, interp: int, precision: int, dtype: int, rows: int, cols: int,
+                   pulses: int) -> None:
+        t = Tuning(rows, cols, pulses)
         check(self.lib.sarbp_set_tuning(self.handle, interp, precision, dtype, C.byref(t)),
               "sarbp_set_tuning")
 
@@ -244,7 +248,7 @@ class Context:
         t = Tuning()
         check(self.lib.sarbp_get_tuning(self.handle, interp, precision, dtype, C.byref(t)),
               "sarbp_get_tuning")
-        return t.pixels_per_lane, t.pulses_per_stage
+        return t.rows_per_lane, t.cols_per_lane, t.pulses_per_stage
 
     def autotune(self, d_profiles: int, d_pos: int, h_pos, grid: Grid, params: Params,
                  d_scratch: int, scratch_pixels: int, max_crop_px: int = 1024,
@@ -260,7 +264,7 @@ class Context:
                                       C.byref(grid), C.byref(params), C.c_void_p(d_scratch),
                                       int(scratch_pixels), int(max_crop_px), C.c_void_p(stream),
                                       C.byref(r)), "sarbp_autotune")
-        return {"pixels_per_lane": r.best.pixels_per_lane,
+        return {"rows_per_lane": r.best.rows_per_lane, "cols_per_lane": r.best.cols_per_lane,
                 "pulses_per_stage": r.best.pulses_per_stage, "best_ms": r.best_ms,
                 "default_ms": r.default_ms, "n_candidates": r.n_candidates,
                 "crop": [r.crop_nx, r.crop_ny]}
@@ -328,7 +332,7 @@ class Server:
 
     def __init__(self, *, device=0, interp="sinc8", precision="ref", dtype="c64",
                  hbm_budget_bytes=8 << 30, max_active_jobs=0, pulses_per_epoch=1024,
-                 pixels_per_lane=0, pulses_per_stage=0, epoch_watchdog_s=0.0):
+                 rows_per_lane=0, cols_per_lane=0, pulses_per_stage=0, epoch_watchdog_s=0.0):
         self.lib = load()
         c = ServerConfig()
         c.device = device
@@ -338,7 +342,8 @@ class Server:
         c.hbm_budget_bytes = int(hbm_budget_bytes)
         c.max_active_jobs = int(max_active_jobs)
         c.pulses_per_epoch = int(pulses_per_epoch)
-        c.pixels_per_lane = int(pixels_per_lane)
+        c.rows_per_lane = int(rows_per_lane)
+        c.cols_per_lane = int(cols_per_lane)
         c.pulses_per_stage = int(pulses_per_stage)
         c.epoch_watchdog_s = float(epoch_watchdog_s)
         h = C.c_void_p()
