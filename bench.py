@@ -176,7 +176,19 @@ def main() -> int:
     ap.add_argument("--served-rate", type=float, default=6.0, help="Poisson arrivals per second")
     ap.add_argument("--hbm-budget", type=int, default=200 * 10**9)
     ap.add_argument("--pulses-per-epoch", type=int, default=1024)
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run a device-to-device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
+
+    if args.copy_bytes:
+        import torch
+        src = torch.ones(args.copy_bytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copied {args.copy_bytes} bytes 3 times")
+        return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the BP imager is a single-GPU build "
@@ -215,7 +227,7 @@ def main() -> int:
     elif not args.no_autotune:
         t1 = time.time()
         tune = ctx.autotune(prof_ptr, ds.pos.data_ptr(), sc.pos, sc.native_grid(), params,
-                            img.data_ptr(), sc.grid["nx"] * sc.grid["ny"], 2048, sp)
+                            img.data_ptr(), sc.grid["nx"] * sc.grid["ny"], 4096, sp)
         log(f"autotune ({tune['n_candidates']} candidates on a {tune['crop']} crop, "
             f"{time.time() - t1:.1f}s): {tune['cols_per_lane']}x{tune['rows_per_lane']} px/lane, pulses/stage "
             f"{tune['pulses_per_stage']}: {tune['best_ms']:.2f} ms vs default {tune['default_ms']:.2f} ms")
@@ -284,6 +296,13 @@ def main() -> int:
 
     fw = F_W[interp]
     achieved = wps * fw / 1e12
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "bp_sinc8_4096x4096_4096pulses_traffic.json")
+    if os.path.exists(tpath):
+        t = json.load(open(tpath))
+        if (t["workload"], t["interp"], t["precision"], t["dtype"]) == \
+                (cfg["name"], interp, args.precision, args.dtype):
+            traffic = t["traffic_bytes"]
     line = {
         "metric": "pixel·pulse back-projections/s at rel-L2≤1e-4 vs FP64; p50/p99 job latency",
         "value": wps,
@@ -307,7 +326,9 @@ def main() -> int:
                    "profile_dtype": args.dtype, "plan": info, "autotune": tune},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_OPS / 1e12,
                      "unit": "Top/s", "frac": achieved / (PEAK_OPS / 1e12),
-                     "ops_per_W": fw, "traffic": None,
+                     "ops_per_W": fw, "traffic": traffic,
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE, "
+                                     "copy-calibrated; profiles/)",
                      "flop_convention": {"achieved_tflops": wps * FLOP_W[interp] / 1e12,
                                          "peak_tflops": PEAK_TFLOPS,
                                          "frac": wps * FLOP_W[interp] / 1e12 / PEAK_TFLOPS}},

@@ -534,9 +534,10 @@ sarbp_status sarbp_autotune(sarbp_context* ctx, const void* d_profiles, const do
   if (params->n_pulses < 1 || !d_profiles || !d_pos || !d_scratch)
     return fail(SARBP_ERR_INVALID_ARGUMENT, "autotune needs pulses and device buffers");
   if (max_crop_px < kTileX) return fail(SARBP_ERR_INVALID_ARGUMENT, "max_crop_px < %d", kTileX);
-  // The crop must hold enough tiles to fill every CU at full occupancy, or
-  // small-tile configurations win on tail effects alone: keep up to 4096
-  // columns and as many rows as a max_crop_px^2 pixel budget allows.
+  // The crop must hold many waves of workgroups, or small-tile configurations
+  // win on the last wave's tail alone (a 4096 x 1024 crop, ~3 waves, already
+  // mis-ranks 64x32 against 64x16 tiles): keep up to 4096 columns and as many
+  // rows as a max_crop_px^2 pixel budget allows.
   sarbp_grid crop = *grid;
   const int64_t budget = (int64_t)max_crop_px * max_crop_px;
   crop.nx = std::min(grid->nx, std::max(max_crop_px, 4096));
@@ -570,7 +571,7 @@ sarbp_status sarbp_autotune(sarbp_context* ctx, const void* d_profiles, const do
     if ((r = launch(a, &p, t, s)) != SARBP_OK) return r;  // warm-up
     hipError_t he;
     if ((he = hipEventRecord(e0, s)) != hipSuccess) return hip_fail(he, "hipEventRecord");
-    const int reps = 3;
+    const int reps = 2;
     for (int i = 0; i < reps; ++i)
       if ((r = launch(a, &p, t, s)) != SARBP_OK) return r;
     if ((he = hipEventRecord(e1, s)) != hipSuccess || (he = hipEventSynchronize(e1)) != hipSuccess)
