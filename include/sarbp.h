@@ -133,6 +133,33 @@ sarbp_status sarbp_backproject_sync(sarbp_context* ctx, const void* d_profiles,
                                     const sarbp_params* params,
                                     sarbp_cf32* d_image, void* stream);
 
+/* ---- reference-compatible entry point ------------------------------------------ */
+
+/* The reference's SARParams (code/SAR_BP/CUDA/BPA_CUDA.h:28-50), field for
+ * field, with its defaults documented there; lambda = c / fc. */
+typedef struct sarbp_sar_params {
+  float c, fc, lambda, Kr, Tr, Br, Fr, vr, H, La, Rg0, Az0;
+  int32_t Nfast, Mslow, Ntarget;
+} sarbp_sar_params;
+
+/*
+ * Drop-in for launchBackProjection(Srnm, Image, params, Na, Nr)
+ * (BPA_CUDA_kernels.cu:91-99): same arguments (device Srnm [Mslow][Nfast] and
+ * Image [Na][Nr], params by value), synchronous like the original
+ * (cudaDeviceSynchronize at :97-98), but a status code instead of exit().
+ * Geometry as the reference's host driver builds it (BPA_CUDA.cpp:35-48): pulse
+ * m at (vr ta_m, 0, H), ta = linspace(-La/(2 vr), La/(2 vr), Mslow); range
+ * sample k at c k / (2 Fr).  Image row i / column j is the ground point
+ * (Az0 + i - Na/2, Rg0 + j - Nr/2, 0), 1 m pixels as the reference kernel's
+ * x = j - Nr/2, y = i - Na/2 (BPA_CUDA_kernels.cu:56-57).  The sum follows
+ * BPA_SAR.m:133-153 (nearest bin, exp(+j 4 pi R / lambda)); the reference
+ * kernel's pulse-invariant range is not reproduced.  Uses a per-device
+ * default context; the current HIP device.
+ */
+sarbp_status sarbp_launch_backprojection(sarbp_cf32* Srnm, sarbp_cf32* Image,
+                                         sarbp_sar_params params, int32_t Na,
+                                         int32_t Nr);
+
 /* ---- plans ------------------------------------------------------------------ */
 
 /*

@@ -631,6 +631,76 @@ sarbp_status sarbp_autotune(sarbp_context* ctx, const void* d_profiles, const do
   return SARBP_OK;
 }
 
+sarbp_status sarbp_launch_backprojection(sarbp_cf32* Srnm, sarbp_cf32* Image, sarbp_sar_params sp,
+                                         int32_t Na, int32_t Nr) {
+  if (!Srnm || !Image) return fail(SARBP_ERR_INVALID_ARGUMENT, "Srnm or Image is NULL");
+  if (Na < 1 || Nr < 1 || sp.Mslow < 1 || sp.Nfast < 1)
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "Na, Nr, Mslow and Nfast must be >= 1");
+  if (!(sp.c > 0 && sp.fc > 0 && sp.Fr > 0 && sp.vr > 0 && std::isfinite(sp.La)))
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "SARParams c, fc, Fr, vr must be > 0");
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  // One default context per device, created on first use and kept for the
+  // process (the reference keeps no state between calls; this caches tables).
+  static std::mutex mu;
+  static std::map<int, sarbp_context*> ctxs;
+  sarbp_context* ctx = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = ctxs.find(dev);
+    if (it == ctxs.end()) {
+      sarbp_status st = sarbp_create(&ctx, dev);
+      if (st != SARBP_OK) return st;
+      ctxs[dev] = ctx;
+    } else {
+      ctx = it->second;
+    }
+  }
+  const int M = sp.Mslow;
+  std::vector<double> pos((size_t)3 * M);
+  const double ta0 = -(double)sp.La / (2.0 * sp.vr), ta1 = (double)sp.La / (2.0 * sp.vr);
+  for (int m = 0; m < M; ++m) {
+    const double ta = M > 1 ? ta0 + (ta1 - ta0) * m / (M - 1) : 0.0;
+    pos[3 * (size_t)m] = (double)sp.vr * ta;
+    pos[3 * (size_t)m + 1] = 0.0;
+    pos[3 * (size_t)m + 2] = sp.H;
+  }
+  double* d_pos = nullptr;
+  if ((e = hipMalloc(&d_pos, pos.size() * sizeof(double))) != hipSuccess) return hip_fail(e, "hipMalloc(pos)");
+  if ((e = hipMemcpy(d_pos, pos.data(), pos.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess) {
+    (void)hipFree(d_pos);
+    return hip_fail(e, "hipMemcpy(pos)");
+  }
+  // The track runs along x, so image rows (azimuth, i) step along x and
+  // columns (ground range, j) along y: pixel (j, i) = (Az0 + i - Na/2,
+  // Rg0 + j - Nr/2, 0).
+  sarbp_grid g;
+  std::memset(&g, 0, sizeof g);
+  g.origin[0] = (double)sp.Az0 - (double)(Na / 2);
+  g.origin[1] = (double)sp.Rg0 - (double)(Nr / 2);
+  g.ux[1] = 1.0;
+  g.uy[0] = 1.0;
+  g.nx = Nr;
+  g.ny = Na;
+  sarbp_params p;
+  std::memset(&p, 0, sizeof p);
+  p.c = sp.c;
+  p.fc = sp.fc;
+  p.r0 = 0.0;
+  p.dr = (double)sp.c / (2.0 * sp.Fr);
+  p.kaiser_beta = 0.0;
+  p.n_pulses = M;
+  p.n_samples = sp.Nfast;
+  p.interp = SARBP_INTERP_NEAREST;
+  p.precision = SARBP_PREC_REF;
+  p.dtype = SARBP_DTYPE_C64;
+  p.accumulate = 0;
+  sarbp_status st = sarbp_backproject_sync(ctx, Srnm, d_pos, &g, &p, Image, nullptr);
+  (void)hipFree(d_pos);
+  return st;
+}
+
 sarbp_status sarbp_sinc8_coefficients(double beta, float* out) {
   if (!out) return fail(SARBP_ERR_INVALID_ARGUMENT, "out is NULL");
   if (!(std::isfinite(beta) && beta >= 0 && beta <= 50)) return fail(SARBP_ERR_INVALID_ARGUMENT, "beta");

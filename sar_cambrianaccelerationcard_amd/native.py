@@ -68,6 +68,25 @@ class AutotuneResult(C.Structure):
                 ("n_candidates", C.c_int32), ("crop_nx", C.c_int32), ("crop_ny", C.c_int32)]
 
 
+class SarParams(C.Structure):
+    """The reference's SARParams (code/SAR_BP/CUDA/BPA_CUDA.h:28-50)."""
+    _fields_ = [(n, C.c_float) for n in ("c", "fc", "lambda_", "Kr", "Tr", "Br", "Fr", "vr", "H",
+                                          "La", "Rg0", "Az0")] + \
+               [("Nfast", C.c_int32), ("Mslow", C.c_int32), ("Ntarget", C.c_int32)]
+
+    @classmethod
+    def reference_defaults(cls) -> "SarParams":
+        """The defaults of BPA_CUDA.h:29-48."""
+        p = cls()
+        p.c, p.fc, p.Kr, p.Tr, p.Br, p.Fr = 3e8, 5.3e9, 4e12, 2.5e-6, 1e8, 1.2e8
+        p.vr, p.H, p.La, p.Rg0, p.Az0 = 150.0, 5000.0, 4.0, 10000.0, 0.0
+        p.lambda_ = p.c / p.fc
+        p.Nfast = int(p.Tr * p.Fr)
+        p.Mslow = 1024
+        p.Ntarget = 5
+        return p
+
+
 class Job(C.Structure):
     _fields_ = [("d_profiles", C.c_void_p), ("d_pos", C.c_void_p), ("h_pos", C.c_void_p),
                 ("grid", Grid), ("params", Params), ("d_image", C.c_void_p)]
@@ -135,6 +154,8 @@ _SIGS = {
     "sarbp_get_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(Tuning)]),
     "sarbp_backproject_batch": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_int32, C.c_void_p]),
     "sarbp_sinc8_coefficients": (C.c_int, [C.c_double, C.POINTER(C.c_float)]),
+    "sarbp_launch_backprojection": (C.c_int, [C.c_void_p, C.c_void_p, SarParams, C.c_int32,
+                                              C.c_int32]),
     "sarbp_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
     "sarbp_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_uint64)]),
     "sarbp_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(JobInfo)]),
