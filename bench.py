@@ -111,7 +111,7 @@ def leg_mixed(torch, ctx, native, W, args) -> dict:
             "batched_ms": t_batch * 1e3, "batched_equals_plain_bitwise": bool(same)}
 
 
-def leg_served(native, W, args, rate: float) -> dict:
+def served_inputs(W, args):
     jobs = W.mixed_jobs(args.served_jobs, seed=20260041, large_fraction=0.1)
     small_max = max((j.n_pulses * j.n_samples for j in jobs if not j.large), default=1)
     large_max = max((j.n_pulses * j.n_samples for j in jobs if j.large), default=0)
@@ -119,7 +119,12 @@ def leg_served(native, W, args, rate: float) -> dict:
     small_prof = W.random_profiles(small_max, 41)
     large_prof = W.random_profiles(large_max, 42) if large_max else None
     log(f"served: {len(jobs)} jobs ({sum(j.large for j in jobs)} large), host profiles in "
-        f"{time.time() - t0:.1f}s, rate {rate}/s")
+        f"{time.time() - t0:.1f}s")
+    return jobs, small_prof, large_prof
+
+
+def leg_served(native, W, args, rate: float, inputs) -> dict:
+    jobs, small_prof, large_prof = inputs
     srv = native.Server(interp="sinc8", hbm_budget_bytes=args.hbm_budget,
                         pulses_per_epoch=args.pulses_per_epoch)
     rng = np.random.default_rng(7)
@@ -152,7 +157,7 @@ def leg_served(native, W, args, rate: float) -> dict:
            "pulses_per_epoch": args.pulses_per_epoch, "hbm_budget_bytes": args.hbm_budget}
     log(f"served @ {rate}/s: {out['W_per_s']:.3e} W/s over {span:.1f}s, p50 {out['p50_latency_s']:.3f}s "
         f"p99 {out['p99_latency_s']:.3f}s, {st['epochs']} epochs, failed {st['failed']}")
-    out["_images"] = (jobs, poss, images, small_prof, large_prof)
+    out["_images"] = (jobs, poss, images)
     return out
 
 
@@ -173,7 +178,8 @@ def main() -> int:
     ap.add_argument("--mixed-jobs", type=int, default=64, help="batched leg jobs (0: skip)")
     ap.add_argument("--mixed-reps", type=int, default=2)
     ap.add_argument("--served-jobs", type=int, default=96, help="served leg jobs (0: skip)")
-    ap.add_argument("--served-rate", type=float, default=6.0, help="Poisson arrivals per second")
+    ap.add_argument("--served-rates", default="6,12,24",
+                    help="Poisson arrival rates (jobs/s) to sweep, comma separated")
     ap.add_argument("--hbm-budget", type=int, default=200 * 10**9)
     ap.add_argument("--pulses-per-epoch", type=int, default=1024)
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -273,10 +279,17 @@ def main() -> int:
     batched = leg_mixed(torch, ctx, native, W, args) if args.mixed_jobs > 0 else None
     served = None
     if args.served_jobs > 0:
-        served = leg_served(native, W, args, args.served_rate)
-        jobs, poss, images, small_prof, large_prof = served.pop("_images")
+        inputs = served_inputs(W, args)
+        small_prof = inputs[1]
+        runs = []
+        for rate in [float(r) for r in args.served_rates.split(",") if r]:
+            runs.append(leg_served(native, W, args, rate, inputs))
+            jobs, poss, images = runs[-1].pop("_images")
+        served = {"runs": runs, "peak_W_per_s": max(r["W_per_s"] for r in runs),
+                  "p50_latency_s_at_lowest_rate": runs[0]["p50_latency_s"],
+                  "p99_latency_s_at_lowest_rate": runs[0]["p99_latency_s"]}
         if batched:
-            served["speedup_vs_plain_sequential"] = served["W_per_s"] / batched["plain_sequential_W_per_s"]
+            served["peak_speedup_vs_plain_sequential"] = served["peak_W_per_s"] / batched["plain_sequential_W_per_s"]
         if args.cpu_rows > 0:
             import oracle.oracle as O
             errs = []
