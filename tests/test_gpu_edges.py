@@ -1,0 +1,172 @@
+"""Edge cases of the plain entry point through the C ABI, each against the
+FP64 oracle where there is an image to compare: zero pulses, empty images,
+ragged sizes (not multiples of the tile), accumulation and pulse chunking,
+bf16 storage, tilted and non-orthogonal 3-D grids, every tile shape, samples
+off both ends of the profile, explicit streams, and argument validation."""
+import numpy as np
+import pytest
+
+from sar_cambrianaccelerationcard_amd import device as D
+from sar_cambrianaccelerationcard_amd import metrics as M
+from sar_cambrianaccelerationcard_amd import native
+from sar_cambrianaccelerationcard_amd import scene as S
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def base():
+    cfg = S.scaled_config(S.load_config("config0"), n_pulses=96, nx=100, ny=70)
+    sc = S.make_scene(cfg)
+    return sc, D.DeviceScene(sc, bf16=True)
+
+
+def _oracle(sc, interp="sinc8", grid=None, prof=None, pos=None):
+    p = O.params(sc.c, sc.fc, sc.r0, sc.dr, interp, sc.kaiser_beta)
+    return O.backproject(sc.profiles if prof is None else prof, sc.pos if pos is None else pos,
+                         grid or sc.grid, p)
+
+
+def _run(ctx, ds, grid, params, img=None, stream=0):
+    import torch
+    if img is None:
+        img = torch.zeros((grid.ny, grid.nx), dtype=torch.complex64, device="cuda")
+    ctx.backproject(ds.prof.data_ptr(), ds.pos.data_ptr(), grid, params, img.data_ptr(), stream, sync=True)
+    return img
+
+
+def test_zero_pulses_writes_zero_and_accumulate_keeps(ctx, base):
+    import torch
+    sc, ds = base
+    params = sc.native_params(interp="sinc8")
+    params.n_pulses = 0
+    img = torch.full((sc.grid["ny"], sc.grid["nx"]), 3 + 1j, dtype=torch.complex64, device="cuda")
+    _run(ctx, ds, sc.native_grid(), params, img)
+    assert torch.count_nonzero(img).item() == 0
+    img.fill_(2 - 1j)
+    params.accumulate = 1
+    _run(ctx, ds, sc.native_grid(), params, img)
+    assert torch.all(img == (2 - 1j)).item()
+
+
+def test_empty_image_is_a_no_op(ctx, base):
+    sc, ds = base
+    g = sc.native_grid()
+    g.nx = 0
+    ctx.backproject(ds.prof.data_ptr(), ds.pos.data_ptr(), g, sc.native_params(), 0, sync=True)
+
+
+@pytest.mark.parametrize("nx,ny", [(1, 1), (63, 5), (65, 33), (129, 3)])
+def test_ragged_sizes(ctx, base, nx, ny):
+    sc, ds = base
+    g = dict(sc.grid, nx=nx, ny=ny)
+    img = _run(ctx, ds, native.make_grid(g["origin"], g["ux"], g["uy"], nx, ny),
+               sc.native_params(interp="linear")).cpu().numpy()
+    assert M.rel_l2(img, _oracle(sc, "linear", g)) <= 1e-4
+
+
+def test_pulse_chunks_accumulate_to_the_whole(ctx, base):
+    """Pulse additivity on the device: three accumulated sub-apertures equal
+    the full aperture to FP32 rounding and the oracle to 1e-4."""
+    import torch
+    sc, ds = base
+    full = _run(ctx, ds, sc.native_grid(), sc.native_params(interp="sinc8")).cpu().numpy()
+    img = torch.zeros((sc.grid["ny"], sc.grid["nx"]), dtype=torch.complex64, device="cuda")
+    cuts = [0, 30, 61, sc.n_pulses]
+    for k, (a, b) in enumerate(zip(cuts[:-1], cuts[1:])):
+        p = sc.native_params(interp="sinc8", accumulate=k > 0)
+        p.n_pulses = b - a
+        ctx.backproject(ds.prof[a:].data_ptr(), ds.pos[a:].data_ptr(), sc.native_grid(), p, img.data_ptr(),
+                        sync=True)
+    got = img.cpu().numpy()
+    assert M.rel_l2(got, full) <= 1e-6
+    assert M.rel_l2(got, _oracle(sc)) <= 1e-4
+
+
+def test_bf16_storage_matches_oracle_on_bf16_values(ctx, base):
+    """bf16x2 storage is exact widening: against the oracle fed the same
+    bf16-rounded samples it holds 1e-4."""
+    sc, ds = base
+    img = D.backproject(ctx, ds, interp="sinc8", dtype="bf16x2").cpu().numpy()
+    rounded = D.from_bf16x2(D.to_bf16x2(sc.profiles))
+    assert M.rel_l2(img, _oracle(sc, prof=rounded)) <= 1e-4
+
+
+def test_tilted_non_orthogonal_grid(ctx, base):
+    """A slant-plane grid with a vertical component and non-orthogonal steps."""
+    sc, ds = base
+    o = sc.grid["origin"]
+    g = {"origin": [o[0], o[1], 5.0], "ux": [0.45, 0.05, 0.02], "uy": [0.08, 0.4, -0.15], "nx": 70, "ny": 50}
+    img = _run(ctx, ds, native.make_grid(g["origin"], g["ux"], g["uy"], g["nx"], g["ny"]),
+               sc.native_params(interp="sinc8")).cpu().numpy()
+    assert M.rel_l2(img, _oracle(sc, grid=g)) <= 1e-4
+
+
+@pytest.mark.parametrize("cols,rows", [(1, 2), (1, 4), (1, 8), (2, 2), (2, 4), (2, 8), (4, 2), (4, 4)])
+def test_every_tile_shape(base, cols, rows):
+    """Every compiled tile shape on a 0.25 m grid (the widest tile, 256 px,
+    spans 32 m from its centre)."""
+    sc, ds = base
+    o = sc.grid["origin"]
+    g = {"origin": o, "ux": [0.25, 0, 0], "uy": [0, 0.25, 0], "nx": 300, "ny": 90}
+    c = native.Context(0)
+    c.set_tuning(native.INTERP_SINC8, native.PREC_REF, native.DTYPE_C64, rows, cols, 5)
+    img = _run(c, ds, native.make_grid(g["origin"], g["ux"], g["uy"], g["nx"], g["ny"]),
+               sc.native_params(interp="sinc8")).cpu().numpy()
+    c.close()
+    assert M.rel_l2(img, _oracle(sc, grid=g)) <= 1e-4
+
+
+def test_tile_radius_limit(base):
+    """A tile reaching more than 48 m from its centre is refused (FP32 pp)."""
+    sc, ds = base
+    c = native.Context(0)
+    c.set_tuning(native.INTERP_SINC8, native.PREC_REF, native.DTYPE_C64, 2, 4, 5)  # 256 px at 0.5 m
+    with pytest.raises(native.SarbpError) as ei:
+        _run(c, ds, sc.native_grid(), sc.native_params(interp="sinc8"))
+    c.close()
+    assert ei.value.status == native.ERR_UNSUPPORTED
+
+
+def test_samples_beyond_the_profile_read_zero(ctx, base):
+    """Shift the range axis so pixels fall off both ends of the profile."""
+    sc, ds = base
+    for shift in (-sc.n_samples * sc.dr * 0.6, sc.n_samples * sc.dr * 0.6):
+        p = sc.native_params(interp="sinc8")
+        p.r0 = sc.r0 + shift
+        img = _run(ctx, ds, sc.native_grid(), p).cpu().numpy()
+        o = O.params(sc.c, sc.fc, sc.r0 + shift, sc.dr, "sinc8", sc.kaiser_beta)
+        ref = O.backproject(sc.profiles, sc.pos, sc.grid, o)
+        assert M.rel_l2(img, ref) <= 1e-4
+
+
+def test_explicit_stream(ctx, base):
+    import torch
+    sc, ds = base
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        img = torch.zeros((sc.grid["ny"], sc.grid["nx"]), dtype=torch.complex64, device="cuda")
+        ctx.backproject(ds.prof.data_ptr(), ds.pos.data_ptr(), sc.native_grid(),
+                        sc.native_params(interp="linear"), img.data_ptr(), s.cuda_stream)
+    s.synchronize()
+    assert M.rel_l2(img.cpu().numpy(), _oracle(sc, "linear")) <= 1e-4
+
+
+def test_invalid_arguments(ctx, base):
+    sc, ds = base
+    bad = [("n_samples", 0), ("interp", 7), ("precision", -1), ("dtype", 3), ("dr", 0.0),
+           ("c", -1.0), ("accumulate", 2)]
+    for field, value in bad:
+        p = sc.native_params()
+        setattr(p, field, value)
+        with pytest.raises(native.SarbpError) as ei:
+            ctx.backproject(ds.prof.data_ptr(), ds.pos.data_ptr(), sc.native_grid(), p, 0)
+        assert ei.value.status == native.ERR_INVALID_ARGUMENT, field
+    g = sc.native_grid()
+    g.ux[0] = g.ux[1] = g.ux[2] = 0.0
+    with pytest.raises(native.SarbpError):
+        ctx.backproject(ds.prof.data_ptr(), ds.pos.data_ptr(), g, sc.native_params(), 0)
+    with pytest.raises(native.SarbpError):
+        ctx.set_tuning(native.INTERP_SINC8, native.PREC_F64, native.DTYPE_C64, 4, 2, 8)  # F64 has cols 1 only
