@@ -354,6 +354,54 @@ sarbp_status sarbp_server_drain(sarbp_server* server, double timeout_s);
 /* Stops accepting work, cancels queued jobs, waits for in-flight epochs. */
 void sarbp_server_destroy(sarbp_server* server);
 
+/* ---- range compression ---------------------------------------------------------
+ *
+ * The step before BP (code/SAR_BP/matlab/BPA_SAR.m:93-100 matched filter,
+ * :106-111 upsampling by zero insertion): raw LFM echoes -> range-compressed,
+ * L-times upsampled, cropped profiles that sarbp_backproject consumes.  The
+ * reference does this in MATLAB only (its CUDA driver feeds raw echoes to the
+ * BP kernel, BPA_CUDA.cpp:70-84); there is no compiled counterpart.
+ *
+ * Per pulse, with nf = smallest power of two >= n_raw + ceil(Tp fs):
+ *   h[n]   = exp(j pi Kr (n/fs - Tp/2)^2) for 0 < n/fs < Tp (and n < n_raw),
+ *            else 0; Kr = B / Tp (chirp centred at baseband)
+ *   S      = FFT_nf(raw zero-padded to nf);  Y = S . conj(FFT_nf(h))
+ *   Z      = Y with bins [0, (nf+1)/2) kept at the front and the rest moved to
+ *            the end of an L*nf spectrum (zeros between)
+ *   out[i] = IFFT_{L nf}(Z)[crop_offset + i] / nf,   i in [0, n_out)
+ * The 1/nf scale keeps the compressed amplitude (a unit chirp compresses to
+ * Tp*fs at its delay) and makes out[L q - crop_offset] the unupsampled
+ * compressed sample q.  Output sample i lies at slant range r0 + i*dr with
+ * r0 = c/2 (t0 + crop_offset / (L fs)), dr = c / (2 L fs).
+ */
+typedef struct sarbp_range_params {
+  double c;             /* propagation speed, m/s                          */
+  double bandwidth;     /* chirp bandwidth B, Hz                           */
+  double pulse_width;   /* chirp duration Tp, s                            */
+  double fs;            /* raw complex sampling rate, Hz                   */
+  double t0;            /* fast time of raw sample 0, s (range axis only)  */
+  int32_t n_raw;        /* raw samples per pulse                           */
+  int32_t upsample;     /* L, 1..16                                        */
+  int32_t crop_offset;  /* first upsampled sample kept, >= 0               */
+  int32_t n_out;        /* samples kept per pulse, crop_offset + n_out <= L nf */
+  int32_t dtype;        /* sarbp_dtype of the output profiles              */
+  int32_t reserved;     /* 0                                               */
+} sarbp_range_params;
+
+/* FFT size nf and output range axis (r0, dr) of a parameter set; validates it
+ * (nf <= 8192 on the device path).  Host-only; NULL outputs are skipped. */
+sarbp_status sarbp_range_info(const sarbp_range_params* rp, int32_t* nf,
+                              double* r0, double* dr);
+
+/* Range-compresses d_raw [n_pulses][n_raw] complex64 into d_profiles
+ * [n_pulses][n_out] (complex64, or bf16x2 words rounded to nearest even).
+ * One workgroup per pulse, all FFTs in LDS, FP32 arithmetic with FP64-derived
+ * tables.  The tables are built once per parameter set and kept by the
+ * context. */
+sarbp_status sarbp_range_compress(sarbp_context* ctx, const sarbp_cf32* d_raw,
+                                  int32_t n_pulses, const sarbp_range_params* rp,
+                                  void* d_profiles, void* stream);
+
 /* ---- diagnostics -------------------------------------------------------------- */
 
 /* The 64 coefficients the SINC8 interpolator uses for Kaiser shape `beta`:

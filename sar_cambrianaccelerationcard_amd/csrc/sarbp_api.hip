@@ -104,13 +104,6 @@ double norm3(const double* v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2
 
 using namespace sarbp;
 
-struct sarbp_context {
-  int device = 0;
-  int n_cu = 256;
-  std::mutex mu;
-  std::map<int, sarbp_tuning> tuning;
-  BatchExec* batch = nullptr;  // graph-replayed batched launches
-};
 
 struct sarbp_plan {
   sarbp_context* ctx;
@@ -355,6 +348,7 @@ void sarbp_destroy(sarbp_context* c) {
   {
     DeviceGuard dg(c->device);
     delete c->batch;
+    range_cache_destroy(c->range);
   }
   delete c;
 }

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -84,4 +85,18 @@ class BatchExec {
   int captured_ = 0;
 };
 
+// Device tables of range compression (twiddles, matched filter, phase ramp),
+// one set per (FFT size, upsampling, chirp) seen by a context.
+struct RangeCache;
+void range_cache_destroy(RangeCache* rc);
+
 }  // namespace sarbp
+
+struct sarbp_context {
+  int device = 0;
+  int n_cu = 256;
+  std::mutex mu;
+  std::map<int, sarbp_tuning> tuning;
+  sarbp::BatchExec* batch = nullptr;     // graph-replayed batched launches
+  sarbp::RangeCache* range = nullptr;    // range-compression tables
+};

@@ -1,0 +1,409 @@
+// sarbp_range.hip -- range compression on the device: FFT matched filter and
+// upsampling by spectral zero insertion (code/SAR_BP/matlab/BPA_SAR.m:93-100,
+// :106-111), the step that turns raw LFM echoes into the profiles BP reads.
+//
+// One workgroup per pulse (256 threads, up to 1024 for the largest FFTs);
+// every transform runs in LDS (nf complex64 words,
+// nf <= 8192 -> at most 64 KiB), so a pulse costs one HBM read of its raw
+// echo and one write of its output window.
+//
+// The L-times upsampled inverse transform is split into L polyphase inverse
+// FFTs of size nf instead of one of size L*nf: with k~ the signed bin of the
+// zero-inserted spectrum Y (k~ = k for k < (nf+1)/2, k - nf otherwise),
+//   x[L q + r] = sum_k Y[k] e^{j 2 pi k~ (L q + r) / (L nf)}
+//              = IFFT_nf( Y[k] e^{j 2 pi k~ r / (L nf)} )[q],
+// so phase r is an nf-point inverse FFT of the spectrum times a ramp.  The
+// FFTs are radix-4 Stockham passes (a radix-2 pass first when log2 nf is
+// odd), in place in LDS: every thread reads its butterfly inputs into
+// registers, the workgroup synchronises, then writes the outputs.
+//
+// Twiddles, the matched filter conj(FFT(h))/nf and the ramp are FP32 tables
+// computed in FP64 on the host and kept per context.
+#include "sarbp.h"
+#include "sarbp_host.h"
+
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <tuple>
+#include <vector>
+
+namespace sarbp {
+
+namespace {
+
+constexpr int kRcMaxLog2 = 13;  // nf <= 8192: 64 KiB of LDS
+
+// Threads per workgroup for nf = 2^LG: 256, or nf / 8 above 2048 points so
+// that a thread keeps at most 8 spectrum bins and 2 butterflies per pass in
+// registers.
+constexpr int rc_threads(int lg) { return lg >= 11 ? (1 << lg) / 8 : 256; }
+constexpr double kPiD = 3.14159265358979323846;
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+// In-place FFT of NF = 2^LG points held in LDS.  tw[k] = exp(-j 2 pi k / NF);
+// INV uses the conjugate twiddles (unscaled inverse).  Ends with a barrier.
+template <int LG, bool INV>
+__device__ __forceinline__ void fft_lds(float2* buf, const float2* __restrict__ tw) {
+  constexpr int NF = 1 << LG;
+  constexpr int kRcThreads = rc_threads(LG);
+  const int tid = threadIdx.x;
+  if constexpr (LG & 1) {
+    constexpr int NB = NF / 2;
+    constexpr int PER = (NB + kRcThreads - 1) / kRcThreads;
+    float2 y0[PER], y1[PER];
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+      const int j = tid + b * kRcThreads;
+      if (NB % kRcThreads == 0 || j < NB) {
+        const float2 v0 = buf[j], v1 = buf[j + NB];
+        y0[b] = cadd(v0, v1);
+        y1[b] = csub(v0, v1);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+      const int j = tid + b * kRcThreads;
+      if (NB % kRcThreads == 0 || j < NB) {
+        buf[2 * j] = y0[b];
+        buf[2 * j + 1] = y1[b];
+      }
+    }
+    __syncthreads();
+  }
+  constexpr int NB = NF / 4;
+  constexpr int PER = NB > 0 ? (NB + kRcThreads - 1) / kRcThreads : 0;
+#pragma unroll
+  for (int pass = 0; pass < LG / 2; ++pass) {
+    const int lns = (LG & 1) + 2 * pass;  // log2 of the sub-transform size
+    const int ns = 1 << lns;
+    float2 y[PER > 0 ? PER : 1][4];
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+      const int j = tid + b * kRcThreads;
+      if (NB % kRcThreads == 0 || j < NB) {
+        float2 v0 = buf[j], v1 = buf[j + NB], v2 = buf[j + 2 * NB], v3 = buf[j + 3 * NB];
+        if (lns > 0) {
+          const int k = j & (ns - 1);
+          const int st = k << (LG - 2 - lns);  // k * NF / (4 ns)
+          float2 w1 = tw[st];
+          if (INV) w1.y = -w1.y;
+          const float2 w2 = cmul(w1, w1), w3 = cmul(w2, w1);
+          v1 = cmul(v1, w1);
+          v2 = cmul(v2, w2);
+          v3 = cmul(v3, w3);
+        }
+        const float2 a0 = cadd(v0, v2), a1 = csub(v0, v2), a2 = cadd(v1, v3), d = csub(v1, v3);
+        // forward: a3 = -j d; inverse: a3 = +j d
+        const float2 a3 = INV ? make_float2(-d.y, d.x) : make_float2(d.y, -d.x);
+        y[b][0] = cadd(a0, a2);
+        y[b][1] = cadd(a1, a3);
+        y[b][2] = csub(a0, a2);
+        y[b][3] = csub(a1, a3);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+      const int j = tid + b * kRcThreads;
+      if (NB % kRcThreads == 0 || j < NB) {
+        const int k = j & (ns - 1);
+        const int dst = ((j - k) << 2) + k;
+        buf[dst] = y[b][0];
+        buf[dst + ns] = y[b][1];
+        buf[dst + 2 * ns] = y[b][2];
+        buf[dst + 3 * ns] = y[b][3];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ uint32_t bf16_rne(float x) {
+  const uint32_t u = __float_as_uint(x);
+  return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+
+struct RcArgs {
+  const float2* raw;
+  const float2* tw;
+  const float2* hc;
+  const float2* ramp;
+  void* out;
+  int n_raw, L, crop, n_out, dtype;
+};
+
+// Block m compresses pulse m: raw[m][0, n_raw) -> out[m][0, n_out).
+template <int LG>
+__global__ __launch_bounds__(rc_threads(LG)) void rc_kernel(RcArgs a) {
+  constexpr int NF = 1 << LG;
+  constexpr int kRcThreads = rc_threads(LG);
+  constexpr int E = (NF + kRcThreads - 1) / kRcThreads;
+  extern __shared__ float2 buf[];
+  const int tid = threadIdx.x;
+  const size_t m = blockIdx.x;
+  const float2* row = a.raw + m * (size_t)a.n_raw;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int i = tid + e * kRcThreads;
+    if (NF % kRcThreads == 0 || i < NF) buf[i] = i < a.n_raw ? row[i] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft_lds<LG, false>(buf, a.tw);
+  float2 spec[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int i = tid + e * kRcThreads;
+    if (NF % kRcThreads == 0 || i < NF) spec[e] = cmul(buf[i], a.hc[i]);
+  }
+  const size_t obase = m * (size_t)a.n_out;
+  for (int r = 0; r < a.L; ++r) {
+    // outputs o = L q + r - crop in [0, n_out) for q in [q_lo, q_hi]
+    const int num_lo = a.crop - r;
+    const int q_lo = num_lo <= 0 ? 0 : (num_lo + a.L - 1) / a.L;
+    const int q_hi = min(NF - 1, (a.n_out - 1 + a.crop - r) / a.L);
+    const bool live = a.n_out - 1 + a.crop - r >= 0 && q_lo <= q_hi;  // uniform over the block
+    if (live) {
+      __syncthreads();  // previous phase's reads of buf are done
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int i = tid + e * kRcThreads;
+        if (NF % kRcThreads == 0 || i < NF) buf[i] = spec[e];
+      }
+      __syncthreads();
+      fft_lds<LG, true>(buf, a.tw);
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int q = tid + e * kRcThreads;
+        if (q >= q_lo && q <= q_hi) {
+          const size_t o = obase + (size_t)(a.L * q + r - a.crop);
+          const float2 v = buf[q];
+          if (a.dtype == SARBP_DTYPE_C64)
+            static_cast<float2*>(a.out)[o] = v;
+          else
+            static_cast<uint32_t*>(a.out)[o] = bf16_rne(v.x) | (bf16_rne(v.y) << 16);
+        }
+      }
+    }
+    if (r + 1 < a.L) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int i = tid + e * kRcThreads;
+        if (NF % kRcThreads == 0 || i < NF) spec[e] = cmul(spec[e], a.ramp[i]);
+      }
+    }
+  }
+}
+
+template <int LG>
+hipError_t launch_rc(const RcArgs& a, int n_pulses, hipStream_t s) {
+  hipLaunchKernelGGL(rc_kernel<LG>, dim3((unsigned)n_pulses), dim3(rc_threads(LG)),
+                     (size_t)(1 << LG) * sizeof(float2), s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_rc_lg(int lg, const RcArgs& a, int n_pulses, hipStream_t s) {
+  switch (lg) {
+    case 1: return launch_rc<1>(a, n_pulses, s);
+    case 2: return launch_rc<2>(a, n_pulses, s);
+    case 3: return launch_rc<3>(a, n_pulses, s);
+    case 4: return launch_rc<4>(a, n_pulses, s);
+    case 5: return launch_rc<5>(a, n_pulses, s);
+    case 6: return launch_rc<6>(a, n_pulses, s);
+    case 7: return launch_rc<7>(a, n_pulses, s);
+    case 8: return launch_rc<8>(a, n_pulses, s);
+    case 9: return launch_rc<9>(a, n_pulses, s);
+    case 10: return launch_rc<10>(a, n_pulses, s);
+    case 11: return launch_rc<11>(a, n_pulses, s);
+    case 12: return launch_rc<12>(a, n_pulses, s);
+    case 13: return launch_rc<13>(a, n_pulses, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// ---- host tables ---------------------------------------------------------------
+
+using cd = std::complex<double>;
+
+// Iterative radix-2 FFT, FP64, forward (exp(-j ...)), n a power of two.
+void fft_host(std::vector<cd>& a) {
+  const size_t n = a.size();
+  for (size_t i = 1, j = 0; i < n; ++i) {
+    size_t bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) std::swap(a[i], a[j]);
+  }
+  for (size_t len = 2; len <= n; len <<= 1) {
+    const double ang = -2.0 * kPiD / (double)len;
+    for (size_t i = 0; i < n; i += len)
+      for (size_t k = 0; k < len / 2; ++k) {
+        const cd w(std::cos(ang * (double)k), std::sin(ang * (double)k));
+        const cd u = a[i + k], v = a[i + k + len / 2] * w;
+        a[i + k] = u + v;
+        a[i + k + len / 2] = u - v;
+      }
+  }
+}
+
+int chirp_samples(const sarbp_range_params* rp) { return (int)std::ceil(rp->pulse_width * rp->fs); }
+
+sarbp_status check_params(const sarbp_range_params* rp, int* lg_out) {
+  if (!rp) return fail(SARBP_ERR_INVALID_ARGUMENT, "range params is NULL");
+  if (!(rp->c > 0.0) || !(rp->fs > 0.0) || !(rp->pulse_width > 0.0) || !std::isfinite(rp->c) ||
+      !std::isfinite(rp->fs) || !std::isfinite(rp->pulse_width) || !std::isfinite(rp->bandwidth) ||
+      !std::isfinite(rp->t0))
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "c, fs, pulse_width must be finite and > 0; bandwidth, t0 finite");
+  if (rp->n_raw < 1) return fail(SARBP_ERR_INVALID_ARGUMENT, "n_raw %d < 1", rp->n_raw);
+  if (rp->upsample < 1 || rp->upsample > 16)
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "upsample %d outside [1, 16]", rp->upsample);
+  if (rp->crop_offset < 0 || rp->n_out < 0)
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "crop_offset %d, n_out %d", rp->crop_offset, rp->n_out);
+  if (rp->dtype != SARBP_DTYPE_C64 && rp->dtype != SARBP_DTYPE_BF16X2)
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "dtype %d", rp->dtype);
+  const double need = (double)rp->n_raw + (double)chirp_samples(rp);
+  if (need > (double)(1 << 30)) return fail(SARBP_ERR_UNSUPPORTED, "n_raw + chirp samples too large");
+  int lg = 1;
+  while ((double)(1 << lg) < need) ++lg;
+  if ((int64_t)rp->crop_offset + rp->n_out > ((int64_t)rp->upsample << lg))
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "crop_offset + n_out = %lld exceeds L * nf = %lld",
+                (long long)rp->crop_offset + rp->n_out, (long long)rp->upsample << lg);
+  if (lg > kRcMaxLog2)
+    return fail(SARBP_ERR_UNSUPPORTED, "FFT size %d exceeds %d (n_raw + chirp samples too long)", 1 << lg,
+                1 << kRcMaxLog2);
+  *lg_out = lg;
+  return SARBP_OK;
+}
+
+}  // namespace
+
+struct RangeTables {
+  float2* d = nullptr;  // [tw | hc | ramp], nf entries each
+  int nf = 0;
+};
+
+struct RangeCache {
+  std::map<std::tuple<int, int, int, double, double, double>, RangeTables> tables;
+};
+
+void range_cache_destroy(RangeCache* rc) {
+  if (!rc) return;
+  for (auto& kv : rc->tables) (void)hipFree(kv.second.d);
+  delete rc;
+}
+
+namespace {
+
+// Builds (or finds) the tables of a parameter set.  Caller holds ctx->mu and
+// the device guard.
+sarbp_status get_tables(sarbp_context* ctx, const sarbp_range_params* rp, int lg, const float2** out) {
+  if (!ctx->range) {
+    ctx->range = new (std::nothrow) RangeCache;
+    if (!ctx->range) return fail(SARBP_ERR_OUT_OF_MEMORY, "range cache");
+  }
+  const int nf = 1 << lg, L = rp->upsample;
+  const auto key = std::make_tuple(nf, L, rp->n_raw, rp->bandwidth, rp->pulse_width, rp->fs);
+  auto it = ctx->range->tables.find(key);
+  if (it != ctx->range->tables.end()) {
+    *out = it->second.d;
+    return SARBP_OK;
+  }
+  std::vector<float2> h((size_t)3 * nf);
+  for (int k = 0; k < nf; ++k) {
+    const double ang = -2.0 * kPiD * (double)k / (double)nf;
+    h[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+  }
+  // Reference chirp starting at the window start, centred at baseband
+  // (exp(j pi Kr (t - Tp/2)^2) for 0 < t < Tp), and its spectrum.
+  const double kr = rp->bandwidth / rp->pulse_width;
+  std::vector<cd> H((size_t)nf, cd(0.0, 0.0));
+  for (int n = 0; n < nf && n < rp->n_raw; ++n) {
+    const double t = (double)n / rp->fs;
+    if (!(t > 0.0 && t < rp->pulse_width)) continue;
+    const double tc = t - 0.5 * rp->pulse_width;
+    const double ph = 0.5 * kr * tc * tc;  // revolutions
+    const double phf = ph - std::floor(ph);
+    H[n] = cd(std::cos(2.0 * kPiD * phf), std::sin(2.0 * kPiD * phf));
+  }
+  fft_host(H);
+  const double inv_nf = 1.0 / (double)nf;
+  const int half = (nf + 1) / 2;
+  for (int k = 0; k < nf; ++k) {
+    const cd v = std::conj(H[k]) * inv_nf;
+    h[nf + k] = make_float2((float)v.real(), (float)v.imag());
+    const double ks = k < half ? (double)k : (double)k - nf;
+    const double ang = 2.0 * kPiD * ks / ((double)L * nf);
+    h[2 * (size_t)nf + k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+  }
+  RangeTables t;
+  t.nf = nf;
+  hipError_t e = hipMalloc(&t.d, h.size() * sizeof(float2));
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc(range tables)");
+  if ((e = hipMemcpy(t.d, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice)) != hipSuccess) {
+    (void)hipFree(t.d);
+    return hip_fail(e, "hipMemcpy(range tables)");
+  }
+  ctx->range->tables[key] = t;
+  *out = t.d;
+  return SARBP_OK;
+}
+
+}  // namespace
+}  // namespace sarbp
+
+using namespace sarbp;
+
+extern "C" {
+
+sarbp_status sarbp_range_info(const sarbp_range_params* rp, int32_t* nf, double* r0, double* dr) {
+  int lg = 0;
+  const sarbp_status st = check_params(rp, &lg);
+  if (st != SARBP_OK) return st;
+  const double L = (double)rp->upsample;
+  if (nf) *nf = 1 << lg;
+  if (r0) *r0 = 0.5 * rp->c * (rp->t0 + (double)rp->crop_offset / (L * rp->fs));
+  if (dr) *dr = 0.5 * rp->c / (L * rp->fs);
+  return SARBP_OK;
+}
+
+sarbp_status sarbp_range_compress(sarbp_context* ctx, const sarbp_cf32* d_raw, int32_t n_pulses,
+                                  const sarbp_range_params* rp, void* d_profiles, void* stream) {
+  if (!ctx) return fail(SARBP_ERR_INVALID_ARGUMENT, "context is NULL");
+  int lg = 0;
+  sarbp_status st = check_params(rp, &lg);
+  if (st != SARBP_OK) return st;
+  if (n_pulses < 0) return fail(SARBP_ERR_INVALID_ARGUMENT, "n_pulses %d < 0", n_pulses);
+  if (n_pulses == 0 || rp->n_out == 0) return SARBP_OK;
+  if (!d_raw || !d_profiles) return fail(SARBP_ERR_INVALID_ARGUMENT, "raw or profiles pointer is NULL");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  DeviceGuard dg(ctx->device);
+  if (dg.err != hipSuccess) return hip_fail(dg.err, "hipSetDevice");
+  const float2* tab = nullptr;
+  if ((st = get_tables(ctx, rp, lg, &tab)) != SARBP_OK) return st;
+  const size_t nf = (size_t)1 << lg;
+  RcArgs a;
+  a.raw = reinterpret_cast<const float2*>(d_raw);
+  a.tw = tab;
+  a.hc = tab + nf;
+  a.ramp = tab + 2 * nf;
+  a.out = d_profiles;
+  a.n_raw = rp->n_raw;
+  a.L = rp->upsample;
+  a.crop = rp->crop_offset;
+  a.n_out = rp->n_out;
+  a.dtype = rp->dtype;
+  const hipError_t e = launch_rc_lg(lg, a, n_pulses, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(e, "range compression launch");
+  return SARBP_OK;
+}
+
+}  // extern "C"

@@ -88,3 +88,34 @@ def backproject(ctx: native.Context, ds: DeviceScene, *, interp: Optional[str] =
     if sync:
         torch.cuda.synchronize(ds.device)
     return img
+
+
+def range_params(radar: dict, dtype: str = "c64", **override) -> native.RangeParams:
+    """RangeParams from a scene's `radar` dict (keys of make_range_params)."""
+    kw = dict(radar)
+    kw.update(override)
+    kw["dtype"] = dtype
+    return native.make_range_params(**kw)
+
+
+def range_compress(ctx: native.Context, raw, radar: dict, *, dtype: str = "c64", device: int = 0,
+                   out=None, sync: bool = True, **override):
+    """Range-compresses raw echoes [n_pulses][n_raw] (numpy complex or a
+    complex64 device tensor) on the device; returns the profiles tensor
+    ([n_pulses][n_out] complex64, or int32 bf16x2 words)."""
+    torch = require_gpu()
+    dev = torch.device("cuda", device)
+    if not isinstance(raw, torch.Tensor):
+        raw = torch.from_numpy(np.ascontiguousarray(raw, dtype=np.complex64)).to(dev)
+    rp = range_params(radar, dtype, **override)
+    if raw.dim() != 2 or raw.shape[1] != rp.n_raw or raw.dtype != torch.complex64 or not raw.is_contiguous():
+        raise ValueError("raw must be a contiguous [n_pulses][n_raw] complex64 tensor")
+    n_pulses = raw.shape[0]
+    if out is None:
+        out = torch.empty((n_pulses, rp.n_out), dtype=torch.complex64 if dtype == "c64" else torch.int32,
+                          device=dev)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    ctx.range_compress(raw.data_ptr(), n_pulses, rp, out.data_ptr(), stream)
+    if sync:
+        torch.cuda.synchronize(dev)
+    return out

@@ -119,6 +119,13 @@ class ServerStats(C.Structure):
                 ("uptime_s", C.c_double), ("hbm_in_use", C.c_size_t), ("hbm_peak", C.c_size_t)]
 
 
+class RangeParams(C.Structure):
+    _fields_ = [("c", C.c_double), ("bandwidth", C.c_double), ("pulse_width", C.c_double),
+                ("fs", C.c_double), ("t0", C.c_double), ("n_raw", C.c_int32),
+                ("upsample", C.c_int32), ("crop_offset", C.c_int32), ("n_out", C.c_int32),
+                ("dtype", C.c_int32), ("reserved", C.c_int32)]
+
+
 JOB_QUEUED, JOB_RESIDENT, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = range(6)
 
 
@@ -165,6 +172,10 @@ _SIGS = {
     "sarbp_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
     "sarbp_server_drain": (C.c_int, [C.c_void_p, C.c_double]),
     "sarbp_server_destroy": (None, [C.c_void_p]),
+    "sarbp_range_info": (C.c_int, [C.POINTER(RangeParams), C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "sarbp_range_compress": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(RangeParams),
+                                       C.c_void_p, C.c_void_p]),
     "sarbp_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Grid),
                                  C.POINTER(Params), C.c_void_p, C.c_size_t, C.c_int32, C.c_void_p,
                                  C.POINTER(AutotuneResult)]),
@@ -219,6 +230,26 @@ def make_grid(origin, ux, uy, nx: int, ny: int) -> Grid:
     return g
 
 
+def make_range_params(*, c: float, bandwidth: float, pulse_width: float, fs: float,
+                      t0: float, n_raw: int, upsample: int, crop_offset: int, n_out: int,
+                      dtype="c64") -> RangeParams:
+    rp = RangeParams()
+    rp.c, rp.bandwidth, rp.pulse_width, rp.fs, rp.t0 = (float(c), float(bandwidth),
+                                                        float(pulse_width), float(fs), float(t0))
+    rp.n_raw, rp.upsample, rp.crop_offset, rp.n_out = int(n_raw), int(upsample), int(crop_offset), int(n_out)
+    rp.dtype = DTYPE_NAMES[dtype] if isinstance(dtype, str) else int(dtype)
+    rp.reserved = 0
+    return rp
+
+
+def range_info(rp: RangeParams) -> tuple:
+    """(nf, r0, dr) of a range-compression parameter set (host only)."""
+    lib = load()
+    nf, r0, dr = C.c_int32(), C.c_double(), C.c_double()
+    check(lib.sarbp_range_info(C.byref(rp), C.byref(nf), C.byref(r0), C.byref(dr)), "sarbp_range_info")
+    return nf.value, r0.value, dr.value
+
+
 def make_params(*, c: float, fc: float, r0: float, dr: float, n_pulses: int,
                 n_samples: int, interp="linear", precision="ref", dtype="c64",
                 kaiser_beta: float = 6.0, accumulate: bool = False) -> Params:
@@ -258,6 +289,12 @@ class Context:
         f = self.lib.sarbp_backproject_sync if sync else self.lib.sarbp_backproject
         check(f(self.handle, C.c_void_p(d_profiles), C.c_void_p(d_pos), C.byref(grid),
                 C.byref(params), C.c_void_p(d_image), C.c_void_p(stream)), "sarbp_backproject")
+
+    def range_compress(self, d_raw: int, n_pulses: int, rp: RangeParams, d_profiles: int,
+                       stream: int = 0) -> None:
+        check(self.lib.sarbp_range_compress(self.handle, C.c_void_p(d_raw), int(n_pulses),
+                                            C.byref(rp), C.c_void_p(d_profiles), C.c_void_p(stream)),
+              "sarbp_range_compress")
 
     def set_tuning(self, interp: int, precision: int, dtype: int, rows: int, cols: int,
                    pulses: int) -> None:

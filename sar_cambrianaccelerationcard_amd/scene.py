@@ -100,6 +100,8 @@ class Scene:
     target_pixels: np.ndarray       # [K][2] (ix, iy) nearest pixel of each target
     config: dict = field(default_factory=dict)
     profiles64: Optional[np.ndarray] = None  # complex128 before the cast
+    raw: Optional[np.ndarray] = None         # [n_pulses][n_raw] complex128 raw echoes (+ noise)
+    radar: Optional[dict] = None             # range-compression parameters of `raw`
 
     @property
     def n_pulses(self) -> int:
@@ -166,8 +168,11 @@ def _targets(cfg: dict, grid: dict, rng: np.random.Generator) -> np.ndarray:
     return np.array(out, dtype=np.float64).reshape(-1, 5)
 
 
-def make_scene(cfg: dict | str, *, keep_f64: bool = False, n_threads: int = 0) -> Scene:
-    """Simulates one scene from a config dict (or config name under configs/)."""
+def make_scene(cfg: dict | str, *, keep_f64: bool = False, keep_raw: bool = False,
+               n_threads: int = 0) -> Scene:
+    """Simulates one scene from a config dict (or config name under configs/).
+    keep_raw keeps the raw echoes (before range compression; clutter is added
+    to the compressed profiles only) and their compression parameters."""
     if isinstance(cfg, str):
         cfg = load_config(cfg)
     L = sim()
@@ -216,6 +221,7 @@ def make_scene(cfg: dict | str, *, keep_f64: bool = False, n_threads: int = 0) -
     if L.sarsim_compress(_dp(raw.view(np.float64)), n_p, C.byref(radar),
                          _dp(prof64.view(np.float64)), None, n_threads) != 0:
         raise ValueError("bad crop window")
+    raw_kept = raw if keep_raw else None
     del raw
     r0 = float(L.sarsim_out_r0(C.byref(radar)))
     dr = float(L.sarsim_out_dr(C.byref(radar)))
@@ -246,7 +252,10 @@ def make_scene(cfg: dict | str, *, keep_f64: bool = False, n_threads: int = 0) -
     return Scene(name=cfg.get("name", "scene"), profiles=prof64.astype(np.complex64), pos=pos,
                  grid=grid, c=c, fc=fc, r0=r0, dr=dr, interp=im["interp"],
                  kaiser_beta=float(im.get("kaiser_beta", 6.0)), targets=targets,
-                 target_pixels=tpix, config=cfg, profiles64=prof64 if keep_f64 else None)
+                 target_pixels=tpix, config=cfg, profiles64=prof64 if keep_f64 else None,
+                 raw=raw_kept,
+                 radar={"c": c, "bandwidth": B, "pulse_width": Tp, "fs": fs, "t0": t0,
+                        "n_raw": n_raw, "upsample": up, "crop_offset": 0, "n_out": n_out})
 
 
 def scaled_config(cfg: dict, *, n_pulses=None, nx=None, ny=None, n_samples=None,
