@@ -14,12 +14,19 @@ Extra legs (reported in the same JSON line):
   batched   64 mixed-size jobs: the persistent graph-replayed batched launch
             against the same jobs run one plain launch after another
   served    a Poisson stream of mixed jobs (10% large) through the
-            continuous-batching server: throughput and p50 / p99 latency
+            continuous-batching server: throughput and p50 / p99 latency; the
+            same stream as raw echoes (range-compressed in HBM, 3.3x fewer
+            host-to-device bytes); and the reference's host flow per job
+            (copy in, BP, copy out) as the end-to-end baseline
   roofline  FP32 VALU op roofline of the plain kernel (ops per pixel-pulse:
             64 sinc8, 33 linear; peak 78.6 Top/s = 256 CU x 64 lanes x 2 x
             2.4 GHz) and the conventional FLOP view (157.3 TFLOP/s)
   cpu_baseline  the FP64 CPU oracle on a bounded row sample of the image
   accuracy  rel-L2 of GPU rows against those oracle rows
+  range_compression  the step before BP on the device: raw echoes of the
+            same scene size (4096 pulses x 2448 samples) -> 8192 4x-upsampled
+            samples, against the HBM roofline (8 TB/s), with the FP64 oracle
+            timed on a pulse sample
 """
 from __future__ import annotations
 
@@ -55,6 +62,54 @@ def timed(torch, stream, fn, reps: int) -> float:
     e1.record(stream)
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / 1e3 / reps
+
+
+def leg_range(torch, ctx, native, D, args) -> dict:
+    """Device range compression of raw echoes of the headline scene's size."""
+    M_, n_raw, n_out = args.range_pulses, 2448, 8192
+    rd = dict(c=299792458.0, bandwidth=150e6, pulse_width=2e-6, fs=200e6, t0=3.2e-5,
+              n_raw=n_raw, upsample=4, crop_offset=0, n_out=n_out)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    raw = torch.randn((M_, n_raw), dtype=torch.complex64, device="cuda", generator=g)
+    out = torch.empty((M_, n_out), dtype=torch.complex64, device="cuda")
+    out16 = torch.empty((M_, n_out), dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    rp, rp16 = D.range_params(rd, "c64"), D.range_params(rd, "bf16x2")
+    nf = native.range_info(rp)[0]
+
+    def run(p, o):
+        return lambda: ctx.range_compress(raw.data_ptr(), M_, p, o.data_ptr(), sp)
+
+    res = {"workload": f"{M_} pulses x {n_raw} raw complex64 samples -> {n_out} samples "
+                       f"(FFT {nf}, upsampling 4)", "n_pulses": M_, "fft_size": nf}
+    for name, p, o, ob in [("c64", rp, out, 8), ("bf16x2", rp16, out16, 4)]:
+        fn = run(p, o)
+        for _ in range(2):
+            fn()
+        t = timed(torch, stream, fn, 10)
+        byts = M_ * (n_raw * 8 + n_out * ob)
+        res[name] = {"ms": t * 1e3, "GB_per_s": byts / t / 1e9, "frac_hbm": byts / t / 8e12,
+                     "bytes": byts, "pulses_per_s": M_ / t}
+        log(f"range compression ({name}): {t * 1e3:.3f} ms, {byts / t / 1e9:.0f} GB/s "
+            f"({byts / t / 8e12:.1%} of 8 TB/s), {M_ / t:.3e} pulses/s")
+    if args.cpu_rows > 0:
+        import oracle.range_oracle as R
+        n_s = 256
+        h_raw = raw[:n_s].cpu().numpy().astype(np.complex128)
+        tc = time.perf_counter()
+        ref = R.range_compress(h_raw, chunk=64, **rd)
+        tcpu = time.perf_counter() - tc
+        got = out[:n_s].cpu().numpy()
+        res["rel_l2_sample"] = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+        res["cpu_baseline"] = {"value": n_s / tcpu, "unit": "pulses/s", "cores": 1, "kind": "port",
+                               "sample": f"FP64 numpy oracle (oracle/range_oracle.py), {n_s} pulses, "
+                                         f"{tcpu:.2f} s"}
+        log(f"range oracle {n_s} pulses: {tcpu:.2f}s ({n_s / tcpu:.1f} pulses/s), rel-L2 "
+            f"{res['rel_l2_sample']:.2e}")
+    del raw, out, out16
+    torch.cuda.empty_cache()
+    return res
 
 
 def leg_mixed(torch, ctx, native, W, args) -> dict:
@@ -153,7 +208,18 @@ def served_inputs(W, args):
     return jobs, small_prof, large_prof
 
 
-def leg_served(native, W, args, rate: float, inputs) -> dict:
+def served_raw_inputs(W, jobs):
+    """Seeded raw echoes (one buffer per size class) for raw-echo submissions."""
+    small_max = max((j.n_pulses * j.n_raw for j in jobs if not j.large), default=1)
+    large_max = max((j.n_pulses * j.n_raw for j in jobs if j.large), default=0)
+    return W.random_profiles(small_max, 43), (W.random_profiles(large_max, 44) if large_max else None)
+
+
+def leg_served(native, W, args, rate: float, inputs, raw_inputs=None) -> dict:
+    """A Poisson stream (rate jobs/s; 0 = all at once) through the server.  With
+    raw_inputs the jobs are submitted as raw echoes and range-compressed on
+    the device at admission."""
+    from sar_cambrianaccelerationcard_amd import device as D
     jobs, small_prof, large_prof = inputs
     srv = native.Server(interp="sinc8", hbm_budget_bytes=args.hbm_budget,
                         pulses_per_epoch=args.pulses_per_epoch)
@@ -168,8 +234,13 @@ def leg_served(native, W, args, rate: float, inputs) -> dict:
         if at > now:
             time.sleep(at - now)
         img = np.empty((j.side, j.side), dtype=np.complex64)
-        src = large_prof if j.large else small_prof
-        ids.append(srv.submit(src[: j.n_pulses * j.n_samples], pos, j.grid(), j.params(), img))
+        if raw_inputs is None:
+            src = large_prof if j.large else small_prof
+            ids.append(srv.submit(src[: j.n_pulses * j.n_samples], pos, j.grid(), j.params(), img))
+        else:
+            src = raw_inputs[1] if j.large else raw_inputs[0]
+            ids.append(srv.submit_raw(src[: j.n_pulses * j.n_raw], D.range_params(j.radar()), pos,
+                                      j.grid(), j.params(), img))
         images.append(img)
     infos = [srv.wait(i, timeout_s=600) for i in ids]
     st = srv.stats()
@@ -184,8 +255,11 @@ def leg_served(native, W, args, rate: float, inputs) -> dict:
            "p50_latency_small_s": float(np.percentile(lat_small, 50)) if lat_small.size else None,
            "p99_latency_small_s": float(np.percentile(lat_small, 99)) if lat_small.size else None,
            "epochs": st["epochs"], "hbm_peak_bytes": st["hbm_peak"], "failed": st["failed"],
-           "pulses_per_epoch": args.pulses_per_epoch, "hbm_budget_bytes": args.hbm_budget}
-    log(f"served @ {rate if rate > 0 else 'burst'}/s: {out['W_per_s']:.3e} W/s over {span:.1f}s, p50 {out['p50_latency_s']:.3f}s "
+           "pulses_per_epoch": args.pulses_per_epoch, "hbm_budget_bytes": args.hbm_budget,
+           "input": "raw echoes" if raw_inputs is not None else "profiles",
+           "h2d_input_bytes": int(sum(j.n_pulses * (j.n_raw if raw_inputs is not None else j.n_samples) * 8
+                                      for j in jobs))}
+    log(f"served{' raw' if raw_inputs is not None else ''} @ {rate if rate > 0 else 'burst'}/s: {out['W_per_s']:.3e} W/s over {span:.1f}s, p50 {out['p50_latency_s']:.3f}s "
         f"p99 {out['p99_latency_s']:.3f}s, {st['epochs']} epochs, failed {st['failed']}")
     out["_images"] = (jobs, poss, images)
     return out
@@ -205,6 +279,7 @@ def main() -> int:
     ap.add_argument("--rows", type=int, default=0, help="rows per lane (skips the autotuner)")
     ap.add_argument("--cols", type=int, default=0, help="columns per lane (skips the autotuner)")
     ap.add_argument("--pulses-per-stage", type=int, default=0)
+    ap.add_argument("--range-pulses", type=int, default=4096, help="range-compression leg (0: skip)")
     ap.add_argument("--mixed-jobs", type=int, default=64, help="batched leg jobs (0: skip)")
     ap.add_argument("--mixed-reps", type=int, default=2)
     ap.add_argument("--served-jobs", type=int, default=96, help="served leg jobs (0: skip)")
@@ -212,6 +287,8 @@ def main() -> int:
                     help="Poisson arrival rates (jobs/s) to sweep, comma separated; 0 = every "
                          "job submitted at once")
     ap.add_argument("--hbm-budget", type=int, default=200 * 10**9)
+    ap.add_argument("--served-raw", type=int, default=1,
+                    help="also serve the stream as raw echoes (burst), range-compressed on the device")
     ap.add_argument("--pulses-per-epoch", type=int, default=1024)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (counter calibration)")
@@ -307,6 +384,7 @@ def main() -> int:
     del ds, img
     torch.cuda.empty_cache()
 
+    rc = leg_range(torch, ctx, native, D, args) if args.range_pulses > 0 else None
     batched = leg_mixed(torch, ctx, native, W, args) if args.mixed_jobs > 0 else None
     served = None
     if args.served_jobs > 0:
@@ -319,6 +397,12 @@ def main() -> int:
         served = {"runs": runs, "peak_W_per_s": max(r["W_per_s"] for r in runs),
                   "p50_latency_s_at_lowest_rate": runs[0]["p50_latency_s"],
                   "p99_latency_s_at_lowest_rate": runs[0]["p99_latency_s"]}
+        if args.served_raw:
+            raw_inputs = served_raw_inputs(W, jobs)
+            rr = leg_served(native, W, args, 0.0, inputs, raw_inputs)
+            rr.pop("_images")
+            served["raw_echo_burst"] = rr
+            del raw_inputs
         e2e = plain_e2e(torch, ctx, native, W, inputs)
         served["plain_end_to_end"] = e2e
         served["peak_speedup_vs_plain_end_to_end"] = served["peak_W_per_s"] / e2e["W_per_s"]
@@ -381,6 +465,7 @@ def main() -> int:
                                          "frac": wps * FLOP_W[interp] / 1e12 / PEAK_TFLOPS}},
         "cpu_baseline": cpu,
         "accuracy": acc,
+        "range_compression": rc,
         "batched": batched,
         "served": served,
     }

@@ -339,6 +339,14 @@ sarbp_status sarbp_server_create(const sarbp_server_config* cfg, sarbp_server** 
 /* Rejects requests that can never fit the arena with SARBP_ERR_OVER_BUDGET. */
 sarbp_status sarbp_server_submit(sarbp_server* server, const sarbp_request* req,
                                  uint64_t* job_id);
+/* A request that carries raw echoes instead of range-compressed profiles.
+ * The server copies the raw echoes (n_raw samples per pulse instead of n_out)
+ * and range-compresses them in HBM (sarbp_range_compress) before imaging.
+ * params.n_samples, r0 and dr are set from `range` (sarbp_range_info);
+ * range.dtype is set to the server's dtype.  Declared further down. */
+typedef struct sarbp_raw_request sarbp_raw_request;
+sarbp_status sarbp_server_submit_raw(sarbp_server* server, const sarbp_raw_request* req,
+                                     uint64_t* job_id);
 sarbp_status sarbp_server_poll(sarbp_server* server, uint64_t job_id,
                                sarbp_job_info* info);
 /* Waits until the job is done, failed or cancelled (SARBP_ERR_TIMEOUT after
@@ -401,6 +409,17 @@ sarbp_status sarbp_range_info(const sarbp_range_params* rp, int32_t* nf,
 sarbp_status sarbp_range_compress(sarbp_context* ctx, const sarbp_cf32* d_raw,
                                   int32_t n_pulses, const sarbp_range_params* rp,
                                   void* d_profiles, void* stream);
+
+/* Served job from raw echoes (sarbp_server_submit_raw).  Host buffers must
+ * stay valid until the job is done, failed or cancelled. */
+struct sarbp_raw_request {
+  const sarbp_cf32* h_raw;   /* [n_pulses][range.n_raw] complex64 raw echoes */
+  sarbp_range_params range;  /* their range compression                     */
+  const double* h_pos;       /* [n_pulses][3]                                */
+  sarbp_grid grid;
+  sarbp_params params;       /* as sarbp_request; n_samples, r0, dr ignored  */
+  sarbp_cf32* h_image;       /* [ny][nx] output                              */
+};
 
 /* ---- diagnostics -------------------------------------------------------------- */
 

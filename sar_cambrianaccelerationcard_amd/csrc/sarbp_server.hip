@@ -91,7 +91,12 @@ struct Job {
   uint64_t id = 0;
   sarbp_request req{};
   BpArgs args{};
-  size_t prof_bytes = 0, pos_bytes = 0, img_bytes = 0, total = 0;
+  // Raw-echo jobs: range-compressed in HBM at admission; the raw echoes sit
+  // after the image in the job's arena block.
+  bool raw = false;
+  const sarbp_cf32* h_raw = nullptr;
+  sarbp_range_params rp{};
+  size_t prof_bytes = 0, pos_bytes = 0, img_bytes = 0, raw_bytes = 0, total = 0;
   size_t off = 0;
   bool has_mem = false;
   int32_t state = SARBP_JOB_QUEUED;
@@ -137,6 +142,7 @@ struct sarbp_server {
   std::thread th_admit, th_sched, th_complete;
   hipStream_t s_admit = nullptr, s_compute = nullptr, s_complete = nullptr;
   BatchExec exec[2];
+  sarbp_context* rctx = nullptr;    // range-compression tables of raw-echo jobs
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   int64_t n_submitted = 0, n_completed = 0, n_failed = 0, n_cancelled = 0, n_epochs = 0;
   double work_done = 0, gpu_busy = 0;
@@ -179,18 +185,30 @@ void sarbp_server::admit_loop() {
     ++admitting;
     char* base = arena.base() + j->off;
     const sarbp_request req = j->req;
-    const size_t pb = j->prof_bytes, qb = j->pos_bytes;
+    const size_t pb = j->prof_bytes, qb = j->pos_bytes, ib = j->img_bytes;
+    const bool raw = j->raw;
+    const sarbp_cf32* h_raw = j->h_raw;
+    const sarbp_range_params rp = j->rp;
+    const size_t rb = j->raw_bytes;
     lk.unlock();
     hipError_t e = hipSuccess;
-    if (pb) e = hipMemcpyAsync(base, req.h_profiles, pb, hipMemcpyHostToDevice, s_admit);
+    sarbp_status rst = SARBP_OK;
+    if (raw) {
+      sarbp_cf32* d_raw = reinterpret_cast<sarbp_cf32*>(base + align_up(pb) + align_up(qb) + align_up(ib));
+      if (rb) e = hipMemcpyAsync(d_raw, h_raw, rb, hipMemcpyHostToDevice, s_admit);
+      if (e == hipSuccess && rb)
+        rst = sarbp_range_compress(rctx, d_raw, req.params.n_pulses, &rp, base, s_admit);
+    } else if (pb) {
+      e = hipMemcpyAsync(base, req.h_profiles, pb, hipMemcpyHostToDevice, s_admit);
+    }
     if (e == hipSuccess && qb)
       e = hipMemcpyAsync(base + align_up(pb), req.h_pos, qb, hipMemcpyHostToDevice, s_admit);
     if (e == hipSuccess) e = hipStreamSynchronize(s_admit);
     lk.lock();
     --admitting;
-    if (e != hipSuccess) {
+    if (e != hipSuccess || rst != SARBP_OK) {
       j->state = SARBP_JOB_FAILED;
-      j->status = hip_fail(e, "server H2D");
+      j->status = e != hipSuccess ? hip_fail(e, "server H2D") : rst;
       j->t_done = now();
       ++n_failed;
       release_job(*j);
@@ -443,10 +461,20 @@ sarbp_status sarbp_server_create(const sarbp_server_config* cfg, sarbp_server** 
     s->n_cu = cu;
   sarbp_status st = s->arena.init(cfg->hbm_budget_bytes);
   if (st != SARBP_OK) return st;
+  auto undo = [&s] {  // releases what a failed create made before its threads
+    if (s->s_admit) (void)hipStreamDestroy(s->s_admit);
+    if (s->s_compute) (void)hipStreamDestroy(s->s_compute);
+    if (s->s_complete) (void)hipStreamDestroy(s->s_complete);
+    sarbp_destroy(s->rctx);
+  };
+  if ((st = sarbp_create(&s->rctx, cfg->device)) != SARBP_OK) return st;
   if ((e = hipStreamCreateWithFlags(&s->s_admit, hipStreamNonBlocking)) != hipSuccess ||
       (e = hipStreamCreateWithFlags(&s->s_compute, hipStreamNonBlocking)) != hipSuccess ||
-      (e = hipStreamCreateWithFlags(&s->s_complete, hipStreamNonBlocking)) != hipSuccess)
-    return hip_fail(e, "hipStreamCreate");
+      (e = hipStreamCreateWithFlags(&s->s_complete, hipStreamNonBlocking)) != hipSuccess) {
+    st = hip_fail(e, "hipStreamCreate");
+    undo();
+    return st;
+  }
   sarbp_server* p = s.get();
   try {
     p->th_admit = std::thread([p] { p->admit_loop(); });
@@ -463,14 +491,19 @@ sarbp_status sarbp_server_create(const sarbp_server_config* cfg, sarbp_server** 
     if (p->th_admit.joinable()) p->th_admit.join();
     if (p->th_sched.joinable()) p->th_sched.join();
     if (p->th_complete.joinable()) p->th_complete.join();
+    undo();
     return fail(SARBP_ERR_OUT_OF_MEMORY, "server threads");
   }
   *out = s.release();
   return SARBP_OK;
 }
 
-sarbp_status sarbp_server_submit(sarbp_server* s, const sarbp_request* req, uint64_t* job_id) {
-  if (!s || !req || !job_id) return fail(SARBP_ERR_INVALID_ARGUMENT, "NULL argument");
+namespace {
+
+// Queues one job; `raw` (NULL for a profile request) holds the raw echoes and
+// their range compression.
+sarbp_status submit_job(sarbp_server* s, const sarbp_request* req, const sarbp_raw_request* raw,
+                        uint64_t* job_id) {
   const sarbp_params& p = req->params;
   sarbp_status st = validate(&req->grid, &p);
   if (st != SARBP_OK) return st;
@@ -478,7 +511,8 @@ sarbp_status sarbp_server_submit(sarbp_server* s, const sarbp_request* req, uint
     return fail(SARBP_ERR_INVALID_ARGUMENT, "request interp/precision/dtype differ from the server's");
   if (p.accumulate) return fail(SARBP_ERR_INVALID_ARGUMENT, "served jobs cannot accumulate into a host image");
   const size_t npix = (size_t)req->grid.nx * req->grid.ny;
-  if ((p.n_pulses > 0 && (!req->h_profiles || !req->h_pos)) || (npix > 0 && !req->h_image))
+  if ((p.n_pulses > 0 && ((!raw && !req->h_profiles) || (raw && !raw->h_raw) || !req->h_pos)) ||
+      (npix > 0 && !req->h_image))
     return fail(SARBP_ERR_INVALID_ARGUMENT, "host pointer is NULL");
   std::unique_ptr<Job> j(new (std::nothrow) Job);
   if (!j) return fail(SARBP_ERR_OUT_OF_MEMORY, "job");
@@ -487,7 +521,13 @@ sarbp_status sarbp_server_submit(sarbp_server* s, const sarbp_request* req, uint
   j->prof_bytes = (size_t)p.n_pulses * p.n_samples * (p.dtype == SARBP_DTYPE_C64 ? 8 : 4);
   j->pos_bytes = (size_t)p.n_pulses * 24;
   j->img_bytes = npix * 8;
-  j->total = align_up(j->prof_bytes) + align_up(j->pos_bytes) + align_up(j->img_bytes);
+  if (raw) {
+    j->raw = true;
+    j->h_raw = raw->h_raw;
+    j->rp = raw->range;
+    j->raw_bytes = (size_t)p.n_pulses * raw->range.n_raw * 8;
+  }
+  j->total = align_up(j->prof_bytes) + align_up(j->pos_bytes) + align_up(j->img_bytes) + align_up(j->raw_bytes);
   j->work = (double)npix * p.n_pulses;
   if (j->total > s->arena.size())
     return fail(SARBP_ERR_OVER_BUDGET, "job needs %zu bytes, the arena holds %zu", j->total, s->arena.size());
@@ -510,6 +550,32 @@ sarbp_status sarbp_server_submit(sarbp_server* s, const sarbp_request* req, uint
   s->queue.push_back(id);
   s->cv_admit.notify_all();
   return SARBP_OK;
+}
+
+}  // namespace
+
+sarbp_status sarbp_server_submit(sarbp_server* s, const sarbp_request* req, uint64_t* job_id) {
+  if (!s || !req || !job_id) return fail(SARBP_ERR_INVALID_ARGUMENT, "NULL argument");
+  return submit_job(s, req, nullptr, job_id);
+}
+
+sarbp_status sarbp_server_submit_raw(sarbp_server* s, const sarbp_raw_request* raw, uint64_t* job_id) {
+  if (!s || !raw || !job_id) return fail(SARBP_ERR_INVALID_ARGUMENT, "NULL argument");
+  sarbp_raw_request r = *raw;
+  r.range.dtype = s->cfg.dtype;
+  double r0 = 0.0, dr = 0.0;
+  sarbp_status st = sarbp_range_info(&r.range, nullptr, &r0, &dr);
+  if (st != SARBP_OK) return st;
+  sarbp_request req{};
+  req.h_profiles = nullptr;
+  req.h_pos = r.h_pos;
+  req.grid = r.grid;
+  req.params = r.params;
+  req.params.n_samples = r.range.n_out;
+  req.params.r0 = r0;
+  req.params.dr = dr;
+  req.h_image = r.h_image;
+  return submit_job(s, &req, &r, job_id);
 }
 
 sarbp_status sarbp_server_poll(sarbp_server* s, uint64_t id, sarbp_job_info* info) {
@@ -644,6 +710,7 @@ void sarbp_server_destroy(sarbp_server* s) {
     if (s->s_compute) (void)hipStreamDestroy(s->s_compute);
     if (s->s_complete) (void)hipStreamDestroy(s->s_complete);
   }
+  sarbp_destroy(s->rctx);
   delete s;
 }
 

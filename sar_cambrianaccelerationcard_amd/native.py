@@ -126,6 +126,11 @@ class RangeParams(C.Structure):
                 ("dtype", C.c_int32), ("reserved", C.c_int32)]
 
 
+class RawRequest(C.Structure):
+    _fields_ = [("h_raw", C.c_void_p), ("range", RangeParams), ("h_pos", C.c_void_p),
+                ("grid", Grid), ("params", Params), ("h_image", C.c_void_p)]
+
+
 JOB_QUEUED, JOB_RESIDENT, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = range(6)
 
 
@@ -165,6 +170,7 @@ _SIGS = {
                                               C.c_int32]),
     "sarbp_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
     "sarbp_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+    "sarbp_server_submit_raw": (C.c_int, [C.c_void_p, C.POINTER(RawRequest), C.POINTER(C.c_uint64)]),
     "sarbp_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(JobInfo)]),
     "sarbp_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.c_double, C.POINTER(JobInfo)]),
     "sarbp_server_cancel": (C.c_int, [C.c_void_p, C.c_uint64]),
@@ -423,6 +429,23 @@ class Server:
         check(self.lib.sarbp_server_submit(self.handle, C.byref(r), C.byref(jid)),
               "sarbp_server_submit")
         self._keep[jid.value] = (profiles, pos, image)
+        return jid.value
+
+    def submit_raw(self, raw, range_params: RangeParams, pos, grid: Grid, params: Params, image) -> int:
+        """raw: host complex64 [n_pulses][n_raw] echoes, range-compressed on the
+        device after admission (params.n_samples / r0 / dr come from
+        range_params)."""
+        r = RawRequest()
+        r.h_raw = raw.ctypes.data if raw.size else None
+        r.range = range_params
+        r.h_pos = pos.ctypes.data if pos.size else None
+        r.grid = grid
+        r.params = params
+        r.h_image = image.ctypes.data if image.size else None
+        jid = C.c_uint64()
+        check(self.lib.sarbp_server_submit_raw(self.handle, C.byref(r), C.byref(jid)),
+              "sarbp_server_submit_raw")
+        self._keep[jid.value] = (raw, pos, image)
         return jid.value
 
     def poll(self, jid: int) -> dict:

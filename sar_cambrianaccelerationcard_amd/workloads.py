@@ -58,6 +58,19 @@ class JobSpec:
         o = [-(self.side // 2) * self.spacing, YC - (self.side // 2) * self.spacing, 0.0]
         return native.make_grid(o, [self.spacing, 0, 0], [0, self.spacing, 0], self.side, self.side)
 
+    @property
+    def n_raw(self) -> int:
+        """Raw samples per pulse whose compression covers the n_samples window:
+        the window at 1/4 the upsampled rate plus one 2 us chirp (400 samples)."""
+        return -(-self.n_samples // 4) + 400
+
+    def radar(self) -> dict:
+        """Range compression of this job's raw echoes (LFM 150 MHz / 2 us, 200 MHz
+        complex sampling, 4x upsampling): the output axis is params()' r0, dr."""
+        r0 = float(np.hypot(YC, ALT)) - 0.5 * self.n_samples * DR
+        return dict(c=C_LIGHT, bandwidth=150e6, pulse_width=2e-6, fs=200e6, t0=2.0 * r0 / C_LIGHT,
+                    n_raw=self.n_raw, upsample=4, crop_offset=0, n_out=self.n_samples)
+
     def params(self, interp="sinc8", precision="ref", dtype="c64") -> native.Params:
         r0 = float(np.hypot(YC, ALT)) - 0.5 * self.n_samples * DR
         return native.make_params(c=C_LIGHT, fc=FC, r0=r0, dr=DR, n_pulses=self.n_pulses,

@@ -137,3 +137,63 @@ def test_request_validation(scenes):
         srv.submit(sc.profiles, sc.pos, sc.native_grid(),
                    sc.native_params(interp="sinc8", accumulate=True), img)
     srv.close()
+
+
+def test_raw_echo_jobs_match_profile_jobs(ctx, scenes):
+    """Jobs submitted as raw echoes (range-compressed in HBM at admission) in a
+    mix with a profile job: each image holds the oracle bound against the BP of
+    the FP64-compressed profiles and is bitwise the image of the same job
+    submitted as its device-compressed profiles on a fresh server."""
+    from sar_cambrianaccelerationcard_amd import device as D
+    raws = []
+    for i, size in enumerate([(300, 160, 96), (128, 130, 70)]):
+        cfg = S.scaled_config(S.load_config("config0"), n_pulses=size[0], nx=size[1], ny=size[2])
+        cfg["seed"] = 3000 + i
+        cfg["image"]["interp"] = "sinc8"
+        raws.append(S.make_scene(cfg, keep_f64=True, keep_raw=True))
+    print(f"[server raw] {len(raws)} raw-echo jobs + 1 profile job", flush=True)
+    srv = native.Server(interp="sinc8", hbm_budget_bytes=1 << 30, pulses_per_epoch=100)
+    subs = []
+    for sc in raws:
+        img = np.zeros((sc.grid["ny"], sc.grid["nx"]), dtype=np.complex64)
+        raw = np.ascontiguousarray(sc.raw, dtype=np.complex64)
+        jid = srv.submit_raw(raw, D.range_params(sc.radar), sc.pos, sc.native_grid(),
+                             sc.native_params(interp="sinc8"), img)
+        subs.append((jid, img, raw))
+    pj, pimg = _submit(srv, scenes[1])
+    for jid, _, _ in subs:
+        assert srv.wait(jid, timeout_s=120)["state"] == native.JOB_DONE
+    assert srv.wait(pj, timeout_s=120)["state"] == native.JOB_DONE
+    srv.close()
+    for sc, (jid, img, raw) in zip(raws, subs):
+        p = O.params(sc.c, sc.fc, sc.r0, sc.dr, "sinc8", sc.kaiser_beta)
+        ref = O.backproject(sc.profiles64, sc.pos, sc.grid, p)
+        err = M.rel_l2(img, ref)
+        print(f"  raw job {jid}: rel_l2={err:.2e}", flush=True)
+        assert err <= 1e-4
+        prof = D.range_compress(ctx, raw, sc.radar).cpu().numpy()
+        alone = native.Server(interp="sinc8", hbm_budget_bytes=1 << 30, pulses_per_epoch=100)
+        a = np.zeros_like(img)
+        j2 = alone.submit(prof, sc.pos, sc.native_grid(), sc.native_params(interp="sinc8"), a)
+        alone.wait(j2, timeout_s=120)
+        alone.close()
+        assert np.array_equal(a, img)
+
+
+def test_raw_request_validation():
+    srv = native.Server(interp="sinc8", hbm_budget_bytes=1 << 28, pulses_per_epoch=100)
+    sc = _scene(4000, 16, 32, 32)
+    rp = native.make_range_params(c=299792458.0, bandwidth=150e6, pulse_width=2e-6, fs=200e6,
+                                  t0=3.2e-5, n_raw=624, upsample=0, crop_offset=0, n_out=2048)
+    raw = np.zeros((16, 624), dtype=np.complex64)
+    img = np.zeros((32, 32), dtype=np.complex64)
+    with pytest.raises(native.SarbpError) as e:
+        srv.submit_raw(raw, rp, sc.pos, sc.native_grid(), sc.native_params(interp="sinc8"), img)
+    assert e.value.status == native.ERR_INVALID_ARGUMENT
+    rp.upsample = 4
+    rp.n_raw = 9000
+    with pytest.raises(native.SarbpError) as e:
+        srv.submit_raw(np.zeros((16, 9000), dtype=np.complex64), rp, sc.pos, sc.native_grid(),
+                       sc.native_params(interp="sinc8"), img)
+    assert e.value.status == native.ERR_UNSUPPORTED
+    srv.close()
