@@ -58,29 +58,40 @@ struct BatchHeader {
   int32_t reserved;
 };
 
-// LDS layout of one workgroup (byte offsets):
+// Pulses whose per-(tile, pulse) setup is computed together: the largest
+// multiple of the stage size P up to 64 (one wave's lanes), so the FP64 setup
+// runs once per SB / P stages.
+__host__ __device__ inline int setup_batch(int P) { return P >= 64 ? P : P * (64 / P); }
+
+// LDS layout of one workgroup (byte offsets), SB = setup_batch(P):
 //   raw   [P][win] float2         staged range windows
 //   coef  [P][win-7][8] float2    sinc8 only: polynomial coefficients D_k of
 //                                 the interpolant at each window position
-//   par   [P] 48-B pulse records
-//   start [P] int window starts, order int
+//   par   [SB] 48-B pulse records
+//   rng   [SB] int2: first staged sample, first sinc8 position of each pulse
+//   start [SB] int window starts
+//   stage [SB/P][3] int per stage: range-expansion order, staged samples and
+//                   sinc8 positions per pulse
 struct LdsLayout {
-  size_t raw, coef, par, start, order, total;
+  size_t raw, coef, par, rng, start, stage, total;
 };
 
 __host__ __device__ inline LdsLayout lds_layout(int interp, int win, int P) {
   LdsLayout L;
+  const int SB = setup_batch(P);
   size_t o = 0;
   L.raw = o;
   o += (size_t)P * win * 8;
   L.coef = o;
   if (interp == kSinc8) o += (size_t)P * (win - 7) * kSincCoef * 8;
   L.par = o;
-  o += (size_t)P * kParBytes;
+  o += (size_t)SB * kParBytes;
+  L.rng = o;
+  o += (size_t)SB * 8;
   L.start = o;
-  o += (size_t)P * 4;
-  L.order = o;
-  o += 4;
+  o += (size_t)SB * 4;
+  L.stage = o;
+  o += (size_t)(SB / P) * 12;
   L.total = (o + 15) & ~(size_t)15;
   return L;
 }
