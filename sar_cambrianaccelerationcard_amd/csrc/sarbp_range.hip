@@ -142,8 +142,19 @@ struct RcArgs {
   int n_raw, L, crop, n_out, dtype;
 };
 
+__device__ __forceinline__ void store_out(void* out, size_t o, float2 v, int dtype) {
+  if (dtype == SARBP_DTYPE_C64)
+    static_cast<float2*>(out)[o] = v;
+  else
+    static_cast<uint32_t*>(out)[o] = bf16_rne(v.x) | (bf16_rne(v.y) << 16);
+}
+
 // Block m compresses pulse m: raw[m][0, n_raw) -> out[m][0, n_out).
-template <int LG>
+// LT > 0 (= L in {1, 2, 4}): every phase's samples are kept in registers and
+// the L interleaved phases are written back through LDS in contiguous chunks,
+// so each store instruction covers whole cache lines.  LT = 0: any L, each
+// phase stored as it is computed (L-strided stores).
+template <int LG, int LT>
 __global__ __launch_bounds__(rc_threads(LG)) void rc_kernel(RcArgs a) {
   constexpr int NF = 1 << LG;
   constexpr int kRcThreads = rc_threads(LG);
@@ -166,13 +177,10 @@ __global__ __launch_bounds__(rc_threads(LG)) void rc_kernel(RcArgs a) {
     if (NF % kRcThreads == 0 || i < NF) spec[e] = cmul(buf[i], a.hc[i]);
   }
   const size_t obase = m * (size_t)a.n_out;
-  for (int r = 0; r < a.L; ++r) {
-    // outputs o = L q + r - crop in [0, n_out) for q in [q_lo, q_hi]
-    const int num_lo = a.crop - r;
-    const int q_lo = num_lo <= 0 ? 0 : (num_lo + a.L - 1) / a.L;
-    const int q_hi = min(NF - 1, (a.n_out - 1 + a.crop - r) / a.L);
-    const bool live = a.n_out - 1 + a.crop - r >= 0 && q_lo <= q_hi;  // uniform over the block
-    if (live) {
+  if constexpr (LT > 0) {
+    float2 keep[E][LT];
+#pragma unroll
+    for (int r = 0; r < LT; ++r) {
       __syncthreads();  // previous phase's reads of buf are done
 #pragma unroll
       for (int e = 0; e < E; ++e) {
@@ -184,48 +192,106 @@ __global__ __launch_bounds__(rc_threads(LG)) void rc_kernel(RcArgs a) {
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const int q = tid + e * kRcThreads;
-        if (q >= q_lo && q <= q_hi) {
-          const size_t o = obase + (size_t)(a.L * q + r - a.crop);
-          const float2 v = buf[q];
-          if (a.dtype == SARBP_DTYPE_C64)
-            static_cast<float2*>(a.out)[o] = v;
-          else
-            static_cast<uint32_t*>(a.out)[o] = bf16_rne(v.x) | (bf16_rne(v.y) << 16);
+        if (NF % kRcThreads == 0 || q < NF) keep[e][r] = buf[q];
+      }
+      if (r + 1 < LT) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int i = tid + e * kRcThreads;
+          if (NF % kRcThreads == 0 || i < NF) spec[e] = cmul(spec[e], a.ramp[i]);
         }
       }
     }
-    if (r + 1 < a.L) {
+    // Upsampled samples [c NF, (c + 1) NF) = q in [c NF / LT, (c + 1) NF / LT):
+    // interleave the phases in LDS, then store the chunk's part of the window.
+    constexpr int QC = NF / LT;  // q per chunk
+#pragma unroll
+    for (int c = 0; c < LT; ++c) {
+      const int u0 = c * NF;  // first upsampled index of the chunk
+      if (u0 - a.crop >= a.n_out || u0 + NF - a.crop <= 0) continue;  // uniform
+      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int q = tid + e * kRcThreads;
+        if (q >= c * QC && q < (c + 1) * QC) {
+#pragma unroll
+          for (int r = 0; r < LT; ++r) buf[(q - c * QC) * LT + r] = keep[e][r];
+        }
+      }
+      __syncthreads();
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const int i = tid + e * kRcThreads;
-        if (NF % kRcThreads == 0 || i < NF) spec[e] = cmul(spec[e], a.ramp[i]);
+        const int o = u0 + i - a.crop;
+        if ((NF % kRcThreads == 0 || i < NF) && o >= 0 && o < a.n_out)
+          store_out(a.out, obase + (size_t)o, buf[i], a.dtype);
+      }
+    }
+  } else {
+    for (int r = 0; r < a.L; ++r) {
+      // outputs o = L q + r - crop in [0, n_out) for q in [q_lo, q_hi]
+      const int num_lo = a.crop - r;
+      const int q_lo = num_lo <= 0 ? 0 : (num_lo + a.L - 1) / a.L;
+      const int q_hi = min(NF - 1, (a.n_out - 1 + a.crop - r) / a.L);
+      const bool live = a.n_out - 1 + a.crop - r >= 0 && q_lo <= q_hi;  // uniform over the block
+      if (live) {
+        __syncthreads();  // previous phase's reads of buf are done
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int i = tid + e * kRcThreads;
+          if (NF % kRcThreads == 0 || i < NF) buf[i] = spec[e];
+        }
+        __syncthreads();
+        fft_lds<LG, true>(buf, a.tw);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int q = tid + e * kRcThreads;
+          if (q >= q_lo && q <= q_hi)
+            store_out(a.out, obase + (size_t)(a.L * q + r - a.crop), buf[q], a.dtype);
+        }
+      }
+      if (r + 1 < a.L) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int i = tid + e * kRcThreads;
+          if (NF % kRcThreads == 0 || i < NF) spec[e] = cmul(spec[e], a.ramp[i]);
+        }
       }
     }
   }
 }
 
-template <int LG>
+template <int LG, int LT>
 hipError_t launch_rc(const RcArgs& a, int n_pulses, hipStream_t s) {
-  hipLaunchKernelGGL(rc_kernel<LG>, dim3((unsigned)n_pulses), dim3(rc_threads(LG)),
+  hipLaunchKernelGGL((rc_kernel<LG, LT>), dim3((unsigned)n_pulses), dim3(rc_threads(LG)),
                      (size_t)(1 << LG) * sizeof(float2), s, a);
   return hipGetLastError();
 }
 
+template <int LG>
+hipError_t launch_rc_l(const RcArgs& a, int n_pulses, hipStream_t s) {
+  // The gathered path needs NF / L whole q per chunk.
+  if (a.L == 1) return launch_rc<LG, 1>(a, n_pulses, s);
+  if (a.L == 2 && LG >= 1) return launch_rc<LG, 2>(a, n_pulses, s);
+  if (a.L == 4 && LG >= 2) return launch_rc<LG, 4>(a, n_pulses, s);
+  return launch_rc<LG, 0>(a, n_pulses, s);
+}
+
 hipError_t launch_rc_lg(int lg, const RcArgs& a, int n_pulses, hipStream_t s) {
   switch (lg) {
-    case 1: return launch_rc<1>(a, n_pulses, s);
-    case 2: return launch_rc<2>(a, n_pulses, s);
-    case 3: return launch_rc<3>(a, n_pulses, s);
-    case 4: return launch_rc<4>(a, n_pulses, s);
-    case 5: return launch_rc<5>(a, n_pulses, s);
-    case 6: return launch_rc<6>(a, n_pulses, s);
-    case 7: return launch_rc<7>(a, n_pulses, s);
-    case 8: return launch_rc<8>(a, n_pulses, s);
-    case 9: return launch_rc<9>(a, n_pulses, s);
-    case 10: return launch_rc<10>(a, n_pulses, s);
-    case 11: return launch_rc<11>(a, n_pulses, s);
-    case 12: return launch_rc<12>(a, n_pulses, s);
-    case 13: return launch_rc<13>(a, n_pulses, s);
+    case 1: return launch_rc_l<1>(a, n_pulses, s);
+    case 2: return launch_rc_l<2>(a, n_pulses, s);
+    case 3: return launch_rc_l<3>(a, n_pulses, s);
+    case 4: return launch_rc_l<4>(a, n_pulses, s);
+    case 5: return launch_rc_l<5>(a, n_pulses, s);
+    case 6: return launch_rc_l<6>(a, n_pulses, s);
+    case 7: return launch_rc_l<7>(a, n_pulses, s);
+    case 8: return launch_rc_l<8>(a, n_pulses, s);
+    case 9: return launch_rc_l<9>(a, n_pulses, s);
+    case 10: return launch_rc_l<10>(a, n_pulses, s);
+    case 11: return launch_rc_l<11>(a, n_pulses, s);
+    case 12: return launch_rc_l<12>(a, n_pulses, s);
+    case 13: return launch_rc_l<13>(a, n_pulses, s);
     default: return hipErrorInvalidValue;
   }
 }
