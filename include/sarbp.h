@@ -213,7 +213,13 @@ typedef struct sarbp_tuning {
 } sarbp_tuning;
 
 /* Sets the configuration used by later calls with this (interp, precision,
- * dtype) triple on this context; all-zero restores the built-in default. */
+ * dtype) triple on this context; all-zero restores the built-in default.
+ * With SARBP_PREC_REF the tile shape is an upper bound: a call whose grid
+ * would put tile pixels more than 48 m from the tile centre (FP32
+ * differential range) runs with the largest compiled shape within it that
+ * fits (plan info reports the shape used), in a batch too.  Only a grid
+ * too coarse for every shape is refused
+ * (SARBP_ERR_UNSUPPORTED). */
 sarbp_status sarbp_set_tuning(sarbp_context* ctx, int32_t interp,
                               int32_t precision, int32_t dtype,
                               const sarbp_tuning* tuning);
@@ -260,11 +266,14 @@ typedef struct sarbp_job {
 
 /*
  * Back-projects n_jobs independent images of arbitrary, uneven sizes in one
- * persistent launch: a device work list of (job, tile, pulse-chunk) items is
- * drained by a fixed grid of workgroups.  All jobs must share interp,
- * precision and dtype.  Each job's pixel sums are formed in a fixed pulse
- * order, so a job's image is bitwise identical whatever else is in the batch.
- * The work list lives in context-owned device memory.
+ * persistent launch per tile shape: a device work list of (job, tile) items
+ * is drained by a fixed grid of workgroups.  Each job uses the tile shape a
+ * plain launch of it would (sarbp_set_tuning); jobs whose grids need a
+ * smaller shape form their own launch, queued on the same stream.  All jobs
+ * must share interp, precision and dtype.  Each job's pixel sums are formed
+ * in a fixed pulse order, so a job's image is bitwise identical to its plain
+ * launch whatever else is in the batch.  The work list lives in
+ * context-owned device memory.
  */
 sarbp_status sarbp_backproject_batch(sarbp_context* ctx, const sarbp_job* jobs,
                                      int32_t n_jobs, void* stream);
@@ -423,9 +432,10 @@ struct sarbp_raw_request {
 
 /* ---- diagnostics -------------------------------------------------------------- */
 
-/* The 64 coefficients the SINC8 interpolator uses for Kaiser shape `beta`:
- * out[k * 8 + t] multiplies f^k in the weight of tap t (sample floor(u) - 3 + t)
- * at fractional bin f = u - floor(u).  Host-only, no device needed. */
+/* The 96 coefficients the SINC8 interpolator uses for Kaiser shape `beta`:
+ * out[(h * 6 + k) * 8 + t] multiplies g^k in the weight of tap t (sample
+ * floor(u) - 3 + t) at fractional bin f = u - floor(u) in half h = floor(2 f)
+ * of the bin, g = 2 f - h.  Host-only, no device needed. */
 sarbp_status sarbp_sinc8_coefficients(double beta, float* out);
 
 #ifdef __cplusplus

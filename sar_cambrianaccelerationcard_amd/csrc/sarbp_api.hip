@@ -68,12 +68,13 @@ void sinc8_poly(double beta, float* out) {
   const double inv_i0b = 1.0 / bessel_i0(beta);
   double nodes[N];
   for (int n = 0; n < N; ++n) nodes[n] = 0.5 + 0.5 * std::cos(kPi * (n + 0.5) / N);
-  for (int t = 0; t < kSincTaps; ++t) {
+  for (int ht = 0; ht < kSincHalves * kSincTaps; ++ht) {
+    const int h = ht / kSincTaps, t = ht % kSincTaps;
     double A[N][N + 1];
     for (int n = 0; n < N; ++n) {
       double v = 1.0;
       for (int k = 0; k < N; ++k, v *= nodes[n]) A[n][k] = v;
-      A[n][N] = kaiser_sinc8(nodes[n] + 3.0 - t, beta, inv_i0b);
+      A[n][N] = kaiser_sinc8(0.5 * (h + nodes[n]) + 3.0 - t, beta, inv_i0b);
     }
     for (int c = 0; c < N; ++c) {
       int piv = c;
@@ -86,7 +87,7 @@ void sinc8_poly(double beta, float* out) {
         for (int k = c; k <= N; ++k) A[r][k] -= m * A[c][k];
       }
     }
-    for (int k = 0; k < N; ++k) out[k * kSincTaps + t] = (float)(A[k][N] / A[k][k]);
+    for (int k = 0; k < N; ++k) out[(h * N + k) * kSincTaps + t] = (float)(A[k][N] / A[k][k]);
   }
 }
 
@@ -158,6 +159,45 @@ sarbp_tuning default_tuning(int interp) {
 // - s^2 carries ~ulp(dmax^2) of rounding into q = pp / (2R), which must stay
 // far below a wavelength (at 48 m, ~1e-7 m for R >= 3 km).
 constexpr double kMaxTileRadius = 48.0;
+
+// Largest |pixel - tile reference pixel| over a tile of px x py pixels per lane.
+double tile_radius(const sarbp_grid* g, int px, int py) {
+  const int tx = kTileX * px, ty = kWaves * py;
+  double r = 0.0;
+  for (int ci = 0; ci < 4; ++ci) {
+    const int di = (ci & 1) ? tx / 2 - 1 : -tx / 2, dj = (ci & 2) ? ty / 2 - 1 : -ty / 2;
+    double d[3];
+    for (int k = 0; k < 3; ++k) d[k] = di * g->ux[k] + dj * g->uy[k];
+    r = std::max(r, norm3(d));
+  }
+  return r;
+}
+
+// The tuning with its tile shape reduced, if needed, to the largest compiled
+// shape no wider and no taller than the requested one whose tiles stay within
+// kMaxTileRadius on every grid (SARBP_PREC_REF only).  A grid too coarse for
+// every shape keeps the request, which plan_args then refuses.
+sarbp_tuning fit_tuning(sarbp_tuning t, int precision, const sarbp_grid* const* grids, int n) {
+  if (precision != SARBP_PREC_REF) return t;
+  auto fits = [&](int px, int py) {
+    for (int i = 0; i < n; ++i)
+      if (tile_radius(grids[i], px, py) > kMaxTileRadius) return false;
+    return true;
+  };
+  if (fits(t.cols_per_lane, t.rows_per_lane)) return t;
+  int best_px = 0, best_py = 0;
+  for (int px = t.cols_per_lane; px >= 1; px /= 2)
+    for (int py = t.rows_per_lane; py >= 2; py /= 2)
+      if (tile_shape_supported(precision, px, py) && px * py > best_px * best_py && fits(px, py)) {
+        best_px = px;
+        best_py = py;
+      }
+  if (best_px) {
+    t.cols_per_lane = best_px;
+    t.rows_per_lane = best_py;
+  }
+  return t;
+}
 
 // Range extent (metres, half-width) of one tile seen from any pulse, and the
 // minimum slant range over the scene.  Without positions: triangle inequality
@@ -347,7 +387,7 @@ void sarbp_destroy(sarbp_context* c) {
   if (!c) return;
   {
     DeviceGuard dg(c->device);
-    delete c->batch;
+    for (BatchExec* b : c->batch) delete b;
     range_cache_destroy(c->range);
   }
   delete c;
@@ -391,7 +431,8 @@ sarbp_status sarbp_backproject(sarbp_context* ctx, const void* d_profiles, const
   std::lock_guard<std::mutex> lk(ctx->mu);
   DeviceGuard dg(ctx->device);
   if (dg.err != hipSuccess) return hip_fail(dg.err, "hipSetDevice");
-  const sarbp_tuning tune = get_tuning(ctx, params->interp, params->precision, params->dtype);
+  const sarbp_tuning tune =
+      fit_tuning(get_tuning(ctx, params->interp, params->precision, params->dtype), params->precision, &grid, 1);
   BpArgs a;
   st = plan_args(nullptr, grid, params, tune, &a, nullptr);
   if (st != SARBP_OK) return st;
@@ -428,7 +469,8 @@ sarbp_status sarbp_plan_create(sarbp_context* ctx, const double* h_pos, const sa
     delete pl;
     return hip_fail(dg.err, "hipSetDevice");
   }
-  const sarbp_tuning tune = get_tuning(ctx, params->interp, params->precision, params->dtype);
+  const sarbp_tuning tune =
+      fit_tuning(get_tuning(ctx, params->interp, params->precision, params->dtype), params->precision, &grid, 1);
   std::memset(&pl->info, 0, sizeof pl->info);
   st = plan_args(h_pos, grid, params, tune, &pl->args, &pl->info);
   if (st != SARBP_OK) {
@@ -490,7 +532,7 @@ sarbp_status sarbp_backproject_batch(sarbp_context* ctx, const sarbp_job* jobs, 
   std::lock_guard<std::mutex> lk(ctx->mu);
   DeviceGuard dg(ctx->device);
   if (dg.err != hipSuccess) return hip_fail(dg.err, "hipSetDevice");
-  const sarbp_tuning tune = get_tuning(ctx, p0.interp, p0.precision, p0.dtype);
+  const sarbp_tuning base = get_tuning(ctx, p0.interp, p0.precision, p0.dtype);
   // Largest per-tile cost first (pulses per tile), ties by submission order:
   // the persistent grid drains items in this order.
   std::vector<int32_t> order((size_t)n_jobs);
@@ -498,24 +540,39 @@ sarbp_status sarbp_backproject_batch(sarbp_context* ctx, const sarbp_job* jobs, 
   std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
     return jobs[x].params.n_pulses > jobs[y].params.n_pulses;
   });
-  std::vector<JobDesc> desc;
-  desc.reserve((size_t)n_jobs);
+  // Each job runs with the tile shape a plain launch of it would use (the
+  // tuned shape, reduced to fit its grid), so its image is bitwise the plain
+  // launch's; jobs that share a shape share one persistent launch.
+  std::vector<std::pair<sarbp_tuning, std::vector<JobDesc>>> groups;
   for (int32_t j : order) {
     const sarbp_job& jb = jobs[j];
     if (jb.grid.nx == 0 || jb.grid.ny == 0) continue;
+    const sarbp_grid* gp = &jb.grid;
+    const sarbp_tuning t = fit_tuning(base, p0.precision, &gp, 1);
     JobDesc d;
-    sarbp_status st = plan_args(jb.h_pos, &jb.grid, &jb.params, tune, &d.a, nullptr);
+    sarbp_status st = plan_args(jb.h_pos, &jb.grid, &jb.params, t, &d.a, nullptr);
     if (st != SARBP_OK) return fail(st, "job %d: %s", j, g_detail.c_str());
     d.a.prof = jb.d_profiles;
     d.a.pos = jb.d_pos;
     d.a.img = reinterpret_cast<float2*>(jb.d_image);
     d.item_begin = 0;
-    desc.push_back(d);
+    size_t g = 0;
+    while (g < groups.size() && !(groups[g].first.cols_per_lane == t.cols_per_lane &&
+                                  groups[g].first.rows_per_lane == t.rows_per_lane))
+      ++g;
+    if (g == groups.size()) groups.emplace_back(t, std::vector<JobDesc>());
+    groups[g].second.push_back(d);
   }
-  if (desc.empty()) return SARBP_OK;
-  if (!ctx->batch && !(ctx->batch = new (std::nothrow) BatchExec))
-    return fail(SARBP_ERR_OUT_OF_MEMORY, "batch executor");
-  return ctx->batch->run(desc, p0.interp, p0.precision, p0.dtype, tune, ctx->n_cu, (hipStream_t)stream);
+  if (groups.size() > sizeof ctx->batch / sizeof ctx->batch[0])
+    return fail(SARBP_ERR_UNSUPPORTED, "%zu tile shapes in one batch", groups.size());
+  for (size_t g = 0; g < groups.size(); ++g) {
+    if (!ctx->batch[g] && !(ctx->batch[g] = new (std::nothrow) BatchExec))
+      return fail(SARBP_ERR_OUT_OF_MEMORY, "batch executor");
+    const sarbp_status st = ctx->batch[g]->run(groups[g].second, p0.interp, p0.precision, p0.dtype,
+                                               groups[g].first, ctx->n_cu, (hipStream_t)stream);
+    if (st != SARBP_OK) return st;
+  }
+  return SARBP_OK;
 }
 
 sarbp_status sarbp_autotune(sarbp_context* ctx, const void* d_profiles, const double* d_pos,

@@ -55,7 +55,6 @@ BatchExec::~BatchExec() {
     (void)hipEventDestroy(done_);
   }
   drop_graphs();
-  if (capture_stream_) (void)hipStreamDestroy(capture_stream_);
   if (d_jobs_) (void)hipFree(d_jobs_);
   if (h_jobs_) (void)hipHostFree(h_jobs_);
   if (d_hdr_) (void)hipFree(d_hdr_);
@@ -133,18 +132,26 @@ sarbp_status BatchExec::run(std::vector<JobDesc>& jobs, int interp, int precisio
   const auto key = std::make_tuple(interp, precision, dtype, px, py, shm_class);
   auto it = graphs_.find(key);
   if (it == graphs_.end()) {
-    if (!capture_stream_ && (e = hipStreamCreateWithFlags(&capture_stream_, hipStreamNonBlocking)) != hipSuccess)
+    // The capture stream lives only for the capture: a process's streams share
+    // a few hardware queues, and a long-lived idle stream shifts the queue
+    // assignment of streams created later (a server's copy stream could land
+    // on its compute stream's queue and wait behind every kernel).
+    hipStream_t cs = nullptr;
+    if ((e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking)) != hipSuccess)
       return hip_fail(e, "hipStreamCreate(capture)");
     const int per_cu = batch_blocks_per_cu(interp, precision, dtype, px, py, shm_class);
     const int n_blocks = std::max(1, n_cu * per_cu);
-    if ((e = hipStreamBeginCapture(capture_stream_, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+    if ((e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal)) != hipSuccess) {
+      (void)hipStreamDestroy(cs);
       return hip_fail(e, "hipStreamBeginCapture");
-    hipError_t e1 = reset_counter(d_counter_, capture_stream_);
+    }
+    hipError_t e1 = reset_counter(d_counter_, cs);
     hipError_t e2 = e1 == hipSuccess ? launch_batch(d_jobs_, d_hdr_, d_counter_, interp, precision, dtype, px,
-                                                    py, shm_class, n_blocks, capture_stream_)
+                                                    py, shm_class, n_blocks, cs)
                                      : e1;
     hipGraph_t g = nullptr;
-    e = hipStreamEndCapture(capture_stream_, &g);
+    e = hipStreamEndCapture(cs, &g);
+    (void)hipStreamDestroy(cs);
     if (e2 != hipSuccess) {
       if (g) (void)hipGraphDestroy(g);
       return hip_fail(e2, "batched launch capture");

@@ -39,6 +39,9 @@ struct DeviceGuard {
 
 sarbp_status validate(const sarbp_grid* g, const sarbp_params* p);
 sarbp_tuning default_tuning(int interp);
+// The tuning with its tile shape reduced, if needed, to the largest compiled
+// shape within it whose tiles fit the REF tile-radius limit on every grid.
+sarbp_tuning fit_tuning(sarbp_tuning t, int precision, const sarbp_grid* const* grids, int n);
 // Fills a launch descriptor (device pointers left NULL) for one image.
 sarbp_status plan_args(const double* h_pos, const sarbp_grid* g, const sarbp_params* p,
                        const sarbp_tuning& tune, BpArgs* a, sarbp_plan_info* info);
@@ -80,7 +83,6 @@ class BatchExec {
   unsigned long long* d_counter_ = nullptr;
   size_t cap_ = 0;
   hipEvent_t done_ = nullptr;
-  hipStream_t capture_stream_ = nullptr;
   std::map<std::tuple<int, int, int, int, int, size_t>, hipGraphExec_t> graphs_;
   int captured_ = 0;
 };
@@ -97,6 +99,8 @@ struct sarbp_context {
   int n_cu = 256;
   std::mutex mu;
   std::map<int, sarbp_tuning> tuning;
-  sarbp::BatchExec* batch = nullptr;     // graph-replayed batched launches
+  // Graph-replayed batched launches, one executor per tile shape of a call
+  // (jobs whose grids need different shapes run as separate launches).
+  sarbp::BatchExec* batch[8] = {};
   sarbp::RangeCache* range = nullptr;    // range-compression tables
 };

@@ -14,10 +14,12 @@ namespace sarbp {
 constexpr int kTileX = 64;
 constexpr int kWaves = 4;
 constexpr int kThreads = kTileX * kWaves;
-// Sinc8: the weight of tap t (sample i0 - 3 + t) at fractional bin f is the
-// degree-7 polynomial sum_k C[k][t] f^k.
+// Sinc8: the weight of tap t (sample i0 - 3 + t) at fractional bin f, in
+// half h = floor(2 f) of the bin, is the degree-5 polynomial
+// sum_k C[h][k][t] g^k of g = 2 f - h.
 constexpr int kSincTaps = 8;
-constexpr int kSincCoef = 8;
+constexpr int kSincHalves = 2;
+constexpr int kSincCoef = 6;
 // Per-pulse parameter record in LDS (12 floats or 4 doubles).
 constexpr int kParBytes = 48;
 
@@ -41,7 +43,7 @@ struct BpArgs {
   int32_t pulses_per_stage;
   int32_t pulse_begin, pulse_end;  // pulses [begin, end) are summed
   int32_t accumulate;
-  float sinc_poly[kSincCoef * kSincTaps];  // C[k * 8 + t]
+  float sinc_poly[kSincHalves * kSincCoef * kSincTaps];  // C[(h * 6 + k) * 8 + t]
 };
 
 // Batched (persistent) launch: one work item = one tile of one job.
@@ -65,15 +67,18 @@ __host__ __device__ inline int setup_batch(int P) { return P >= 64 ? P : P * (64
 
 // LDS layout of one workgroup (byte offsets), SB = setup_batch(P):
 //   raw   [P][win] float2         staged range windows
-//   coef  [P][win-7][8] float2    sinc8 only: polynomial coefficients D_k of
-//                                 the interpolant at each window position
+//   coef  [P][win-7][2][6] float2 sinc8 only: polynomial coefficients D_hk of
+//                                 the interpolant at each window position and
+//                                 half bin
 //   par   [SB] 48-B pulse records
 //   rng   [SB] int2: first staged sample, first sinc8 position of each pulse
 //   start [SB] int window starts
 //   stage [SB/P][3] int per stage: range-expansion order, staged samples and
 //                   sinc8 positions per pulse
+//   poly  [2][6][8] float sinc8 only: the interpolant's polynomial coefficients
+//                   (read as broadcasts; too many to stay in scalar registers)
 struct LdsLayout {
-  size_t raw, coef, par, rng, start, stage, total;
+  size_t raw, coef, par, rng, start, stage, poly, total;
 };
 
 __host__ __device__ inline LdsLayout lds_layout(int interp, int win, int P) {
@@ -83,7 +88,7 @@ __host__ __device__ inline LdsLayout lds_layout(int interp, int win, int P) {
   L.raw = o;
   o += (size_t)P * win * 8;
   L.coef = o;
-  if (interp == kSinc8) o += (size_t)P * (win - 7) * kSincCoef * 8;
+  if (interp == kSinc8) o += (size_t)P * (win - 7) * kSincHalves * kSincCoef * 8;
   L.par = o;
   o += (size_t)SB * kParBytes;
   L.rng = o;
@@ -92,6 +97,9 @@ __host__ __device__ inline LdsLayout lds_layout(int interp, int win, int P) {
   o += (size_t)SB * 4;
   L.stage = o;
   o += (size_t)(SB / P) * 12;
+  o = (o + 15) & ~(size_t)15;
+  L.poly = o;
+  if (interp == kSinc8) o += (size_t)kSincHalves * kSincCoef * kSincTaps * 4;
   L.total = (o + 15) & ~(size_t)15;
   return L;
 }

@@ -91,6 +91,7 @@ struct Job {
   uint64_t id = 0;
   sarbp_request req{};
   BpArgs args{};
+  sarbp_tuning tune{};  // the server's tuning, tile shape fitted to this job's grid
   // Raw-echo jobs: range-compressed in HBM at admission; the raw echoes sit
   // after the image in the job's arena block.
   bool raw = false;
@@ -141,7 +142,7 @@ struct sarbp_server {
   int admitting = 0;                // jobs between queue pop and residency
   std::thread th_admit, th_sched, th_complete;
   hipStream_t s_admit = nullptr, s_compute = nullptr, s_complete = nullptr;
-  BatchExec exec[2];
+  BatchExec exec[2][8];             // [epoch parity][tile-shape group of the epoch]
   sarbp_context* rctx = nullptr;    // range-compression tables of raw-echo jobs
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   int64_t n_submitted = 0, n_completed = 0, n_failed = 0, n_cancelled = 0, n_epochs = 0;
@@ -289,7 +290,8 @@ void sarbp_server::sched_loop() {
   std::deque<Epoch> inflight;
   int k = 0;
   for (;;) {
-    std::vector<JobDesc> desc;
+    // chunks grouped by tile shape: one batched launch per shape
+    std::vector<std::pair<sarbp_tuning, std::vector<JobDesc>>> groups;
     Epoch ep;
     {
       std::unique_lock<std::mutex> lk(mu);
@@ -325,7 +327,12 @@ void sarbp_server::sched_loop() {
         d.a.pulse_end = (int32_t)std::min<int64_t>(j.args.n_pulses, (int64_t)j.next_pulse + chunk);
         d.a.accumulate = j.next_pulse > 0 ? 1 : 0;
         d.item_begin = 0;
-        desc.push_back(d);
+        size_t g = 0;
+        while (g < groups.size() && !(groups[g].first.cols_per_lane == j.tune.cols_per_lane &&
+                                      groups[g].first.rows_per_lane == j.tune.rows_per_lane))
+          ++g;
+        if (g == groups.size()) groups.emplace_back(j.tune, std::vector<JobDesc>());
+        groups[g].second.push_back(d);
         j.next_pulse = d.a.pulse_end;
         j.started = true;
         ++j.inflight;
@@ -346,9 +353,11 @@ void sarbp_server::sched_loop() {
     hipError_t e = hipEventCreate(&ep.ev0);
     if (e == hipSuccess) e = hipEventCreate(&ep.ev1);
     if (e == hipSuccess) e = hipEventRecord(ep.ev0, s_compute);
-    sarbp_status st = e == hipSuccess
-                          ? exec[k & 1].run(desc, cfg.interp, cfg.precision, cfg.dtype, tune, n_cu, s_compute)
-                          : hip_fail(e, "epoch events");
+    sarbp_status st = e == hipSuccess ? SARBP_OK : hip_fail(e, "epoch events");
+    for (size_t g = 0; st == SARBP_OK && g < groups.size(); ++g)
+      st = g < 8 ? exec[k & 1][g].run(groups[g].second, cfg.interp, cfg.precision, cfg.dtype, groups[g].first,
+                                      n_cu, s_compute)
+                 : fail(SARBP_ERR_UNSUPPORTED, "more than 8 tile shapes in an epoch");
     if (st == SARBP_OK && (e = hipEventRecord(ep.ev1, s_compute)) != hipSuccess) st = hip_fail(e, "hipEventRecord");
     if (st != SARBP_OK) {
       // Nothing was enqueued after ev0 (or the launch failed): fail the parts.
@@ -517,7 +526,9 @@ sarbp_status submit_job(sarbp_server* s, const sarbp_request* req, const sarbp_r
   std::unique_ptr<Job> j(new (std::nothrow) Job);
   if (!j) return fail(SARBP_ERR_OUT_OF_MEMORY, "job");
   j->req = *req;
-  if ((st = plan_args(req->h_pos, &req->grid, &p, s->tune, &j->args, nullptr)) != SARBP_OK) return st;
+  const sarbp_grid* gp = &req->grid;
+  j->tune = fit_tuning(s->tune, p.precision, &gp, 1);
+  if ((st = plan_args(req->h_pos, &req->grid, &p, j->tune, &j->args, nullptr)) != SARBP_OK) return st;
   j->prof_bytes = (size_t)p.n_pulses * p.n_samples * (p.dtype == SARBP_DTYPE_C64 ? 8 : 4);
   j->pos_bytes = (size_t)p.n_pulses * 24;
   j->img_bytes = npix * 8;

@@ -343,11 +343,11 @@ class Context:
 
 
 def sinc8_coefficients(beta: float):
-    """The library's sinc8 polynomial coefficients as a numpy [8 powers][8 taps]."""
+    """The library's sinc8 polynomial coefficients as a numpy [2 halves][6 powers][8 taps]."""
     import numpy as np
-    out = (C.c_float * 64)()
+    out = (C.c_float * 96)()
     check(load().sarbp_sinc8_coefficients(float(beta), out), "sarbp_sinc8_coefficients")
-    return np.frombuffer(out, dtype=np.float32).reshape(8, 8).copy()
+    return np.frombuffer(out, dtype=np.float32).reshape(2, 6, 8).copy()
 
 
 class Plan:

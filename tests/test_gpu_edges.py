@@ -119,14 +119,29 @@ def test_every_tile_shape(base, cols, rows):
     assert M.rel_l2(img, _oracle(sc, grid=g)) <= 1e-4
 
 
-def test_tile_radius_limit(base):
-    """A tile reaching more than 48 m from its centre is refused (FP32 pp)."""
+def test_tile_shape_falls_back_within_radius_limit(base):
+    """A tuned tile shape whose tiles would reach more than 48 m from their
+    centre (FP32 pp) runs with the largest compiled shape within it that fits."""
     sc, ds = base
     c = native.Context(0)
     c.set_tuning(native.INTERP_SINC8, native.PREC_REF, native.DTYPE_C64, 2, 4, 5)  # 256 px at 0.5 m
-    with pytest.raises(native.SarbpError) as ei:
-        _run(c, ds, sc.native_grid(), sc.native_params(interp="sinc8"))
+    info = c.plan(sc.pos, sc.native_grid(), sc.native_params(interp="sinc8")).info()
+    assert (info["cols_per_lane"], info["rows_per_lane"]) == (2, 2)
+    img = _run(c, ds, sc.native_grid(), sc.native_params(interp="sinc8")).cpu().numpy()
     c.close()
+    o = O.params(sc.c, sc.fc, sc.r0, sc.dr, "sinc8", sc.kaiser_beta)
+    assert M.rel_l2(img, O.backproject(sc.profiles, sc.pos, sc.grid, o)) <= 1e-4
+
+
+def test_grid_too_coarse_for_every_tile_is_refused(ctx, base):
+    """At 1.6 m pixels even the smallest tile (64 x 8) spans 51 m."""
+    sc, ds = base
+    g = sc.native_grid()
+    for k in range(3):
+        g.ux[k] *= 3.2
+        g.uy[k] *= 3.2
+    with pytest.raises(native.SarbpError) as ei:
+        _run(ctx, ds, g, sc.native_params(interp="sinc8"))
     assert ei.value.status == native.ERR_UNSUPPORTED
 
 

@@ -1,7 +1,8 @@
-"""The sinc8 interpolator evaluates each Kaiser-sinc tap weight as a degree-7
-polynomial in the fractional bin f (host-fitted coefficients, exported by
-sarbp_sinc8_coefficients).  These CPU tests pin that fit against the oracle's
-direct evaluation K(x) = sinc(x) I0(beta sqrt(1-(x/4)^2)) / I0(beta)."""
+"""The sinc8 interpolator evaluates each Kaiser-sinc tap weight, over each
+half of the fractional bin, as a degree-5 polynomial (host-fitted
+coefficients, exported by sarbp_sinc8_coefficients).  These CPU tests pin that
+fit against the oracle's direct evaluation
+K(x) = sinc(x) I0(beta sqrt(1-(x/4)^2)) / I0(beta)."""
 import numpy as np
 import pytest
 
@@ -10,21 +11,36 @@ from sar_cambrianaccelerationcard_amd import native
 import oracle as O
 
 
+def _weights(C, f):
+    """Tap weights [t][n] at fractional bins f (the kernel's half-bin rule)."""
+    h = np.minimum((2 * f).astype(int), 1)
+    g = 2 * f - h
+    powers = np.stack([g ** k for k in range(6)])             # [k][n]
+    return np.stack([(C[h[n]].T @ powers[:, n]) for n in range(f.size)], 1)
+
+
 @pytest.mark.parametrize("beta", [0.0, 2.5, 6.0, 8.0])
 def test_poly_matches_kaiser_sinc(beta):
-    C = native.sinc8_coefficients(beta).astype(np.float64)   # [k][t]
-    f = np.linspace(0.0, 1.0, 1001)
-    powers = np.stack([f ** k for k in range(8)])              # [k][n]
-    w = C.T @ powers                                           # [t][n]
+    C = native.sinc8_coefficients(beta).astype(np.float64)   # [h][k][t]
+    f = np.linspace(0.0, 1.0, 1001, endpoint=False)
+    w = _weights(C, f)
     ref = np.array([[O.kaiser_sinc8(fi + 3 - t, beta) for fi in f] for t in range(8)])
     err = np.abs(w - ref).max()
     assert err < (5e-6 if beta <= 6 else 2e-5), err
 
 
+def test_halves_join_continuously():
+    """Half 0 at g = 1 and half 1 at g = 0 are the same bin f = 1/2."""
+    C = native.sinc8_coefficients(6.0).astype(np.float64)
+    end0 = C[0].sum(axis=0)                                    # g = 1
+    start1 = C[1][0]                                           # g = 0
+    assert np.abs(end0 - start1).max() < 5e-6
+
+
 def test_poly_partition_of_sample_at_integer_bins():
     """At f = 0 the interpolator returns the centre sample: weights (0,0,0,1,0,0,0,0)."""
     C = native.sinc8_coefficients(6.0).astype(np.float64)
-    w = C.T @ np.array([0.0 ** k for k in range(8)])
+    w = _weights(C, np.array([0.0]))[:, 0]
     expect = np.zeros(8)
     expect[3] = 1.0
     assert np.abs(w - expect).max() < 5e-6
