@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define SARBP_ABI_VERSION 1
+#define SARBP_ABI_VERSION 2
 
 typedef enum sarbp_status {
   SARBP_OK = 0,
@@ -184,6 +184,10 @@ typedef struct sarbp_plan_info {
                                    the whole scene, else 0 (chosen per stage) */
   int32_t lds_bytes;            /* dynamic LDS per workgroup                 */
   double max_phase_err_rad;     /* bound on the expansion truncation error   */
+  int32_t sinc8_halves;         /* SINC8 kernel: 2 = half-bin degree-5 tap
+                                   polynomials, 1 = whole-bin degree 7; 0 for
+                                   other interpolators (chosen per launch)   */
+  int32_t reserved;
 } sarbp_plan_info;
 
 sarbp_status sarbp_plan_create(sarbp_context* ctx, const double* h_pos,
@@ -432,11 +436,14 @@ struct sarbp_raw_request {
 
 /* ---- diagnostics -------------------------------------------------------------- */
 
-/* The 96 coefficients the SINC8 interpolator uses for Kaiser shape `beta`:
- * out[(h * 6 + k) * 8 + t] multiplies g^k in the weight of tap t (sample
- * floor(u) - 3 + t) at fractional bin f = u - floor(u) in half h = floor(2 f)
- * of the bin, g = 2 f - h.  Host-only, no device needed. */
+/* The 96 coefficients of the SINC8 interpolator's half-bin kernel for Kaiser
+ * shape `beta`: out[(h * 6 + k) * 8 + t] multiplies g^k in the weight of tap t
+ * (sample floor(u) - 3 + t) at fractional bin f = u - floor(u) in half
+ * h = floor(2 f) of the bin, g = 2 f - h.  Host-only, no device needed. */
 sarbp_status sarbp_sinc8_coefficients(double beta, float* out);
+/* The 64 coefficients of its whole-bin kernel: out[k * 8 + t] multiplies f^k
+ * in the weight of tap t.  Host-only. */
+sarbp_status sarbp_sinc8_coefficients_whole(double beta, float* out);
 
 #ifdef __cplusplus
 }

@@ -35,6 +35,7 @@ HIP_FLAGS = [
 
 
 SARBP_SOURCES = ("sarbp_kernels_nearest.hip", "sarbp_kernels_linear.hip", "sarbp_kernels_sinc8.hip",
+                 "sarbp_kernels_sinc8w.hip",
                  "sarbp_api.hip", "sarbp_batch.hip", "sarbp_server.hip",
                  "sarbp_range.hip")
 

@@ -63,18 +63,22 @@ double kaiser_sinc8(double x, double beta, double inv_i0b) {
 // interpolates K(f + 3 - t) at the 8 Chebyshev nodes of [0, 1]
 // (near-minimax; max error ~2e-6 for beta = 6).  Solved as an 8x8 Vandermonde
 // system in FP64 with partial pivoting.
-void sinc8_poly(double beta, float* out) {
-  constexpr int N = kSincCoef;
+// Fits the tap polynomials of the sinc8 interpolator: `halves` = 2 gives the
+// half-bin degree-5 set C[(h * 6 + k) * 8 + t] in g = 2 f - h, `halves` = 1
+// the whole-bin degree-7 set C[k * 8 + t] in f.  Interpolation at Chebyshev
+// nodes, FP64 Vandermonde solve with partial pivoting.
+template <int N>
+void sinc8_poly_fit(double beta, int halves, float* out) {
   const double inv_i0b = 1.0 / bessel_i0(beta);
   double nodes[N];
   for (int n = 0; n < N; ++n) nodes[n] = 0.5 + 0.5 * std::cos(kPi * (n + 0.5) / N);
-  for (int ht = 0; ht < kSincHalves * kSincTaps; ++ht) {
+  for (int ht = 0; ht < halves * kSincTaps; ++ht) {
     const int h = ht / kSincTaps, t = ht % kSincTaps;
     double A[N][N + 1];
     for (int n = 0; n < N; ++n) {
       double v = 1.0;
       for (int k = 0; k < N; ++k, v *= nodes[n]) A[n][k] = v;
-      A[n][N] = kaiser_sinc8(0.5 * (h + nodes[n]) + 3.0 - t, beta, inv_i0b);
+      A[n][N] = kaiser_sinc8((h + nodes[n]) / halves + 3.0 - t, beta, inv_i0b);
     }
     for (int c = 0; c < N; ++c) {
       int piv = c;
@@ -90,6 +94,9 @@ void sinc8_poly(double beta, float* out) {
     for (int k = 0; k < N; ++k) out[(h * N + k) * kSincTaps + t] = (float)(A[k][N] / A[k][k]);
   }
 }
+
+void sinc8_poly(double beta, float* out) { sinc8_poly_fit<kSincCoef>(beta, kSincHalves, out); }
+void sinc8_poly_whole(double beta, float* out) { sinc8_poly_fit<kSincCoefW>(beta, 1, out); }
 
 int tuning_key(int interp, int prec, int dtype) { return interp * 100 + prec * 10 + dtype; }
 
@@ -284,8 +291,17 @@ sarbp_status plan_args(const double* h_pos, const sarbp_grid* g,
     return fail(SARBP_ERR_UNSUPPORTED, "range window of a tile too long (%.0f samples): pixel spacing is too coarse for dr", 2 * half_bins);
   const int win = ((2 * (int)half_bins + 2) + 3) & ~3;  // multiple of 4 samples
   int P = std::max(1, tune.pulses_per_stage);
-  while (P > 1 && lds_layout(p->interp, win, P).total > kLdsBudget) --P;
-  if (lds_layout(p->interp, win, P).total > kLdsBudget)
+  // Sinc8 kernel: half bins unless the tile is coarse against the range bins.
+  // Per pixel-pulse the half-bin kernel issues ~4 VALU instructions fewer in
+  // the pixel loop and ~0.6 * 76 / (tile pixels) more per window position
+  // (about 60% of the window's positions are touched per pulse).
+  int kinterp = p->interp;
+  if (p->interp == SARBP_INTERP_SINC8) {
+    const double tile_px = (double)kTileX * px * kWaves * py;
+    kinterp = 0.6 * 76.0 * (win - 7) < 4.0 * tile_px ? kSinc8 : kSinc8W;
+  }
+  while (P > 1 && lds_layout(kinterp, win, P).total > kLdsBudget) --P;
+  if (lds_layout(kinterp, win, P).total > kLdsBudget)
     return fail(SARBP_ERR_UNSUPPORTED, "range window of %d samples does not fit LDS", win);
 
   std::memset(a, 0, sizeof *a);
@@ -315,7 +331,9 @@ sarbp_status plan_args(const double* h_pos, const sarbp_grid* g,
   a->accumulate = p->accumulate;
   a->dmax = ext.dmax;
   a->order_tol = 1e-5;
-  if (p->interp == SARBP_INTERP_SINC8) sinc8_poly(p->kaiser_beta, a->sinc_poly);
+  a->kernel_interp = kinterp;
+  if (kinterp == kSinc8) sinc8_poly(p->kaiser_beta, a->sinc_poly);
+  if (kinterp == kSinc8W) sinc8_poly_whole(p->kaiser_beta, a->sinc_poly);
   if (info) {
     info->tiles_x = a->tiles_x;
     info->tiles_y = a->tiles_y;
@@ -325,7 +343,8 @@ sarbp_status plan_args(const double* h_pos, const sarbp_grid* g,
     info->pulses_per_stage = P;
     info->rows_per_lane = py;
     info->cols_per_lane = px;
-    info->lds_bytes = (int32_t)lds_layout(p->interp, win, P).total;
+    info->lds_bytes = (int32_t)lds_layout(kinterp, win, P).total;
+    info->sinc8_halves = kinterp == kSinc8 ? 2 : (kinterp == kSinc8W ? 1 : 0);
     const double rm = ext.rmin > ext.dmax ? ext.rmin - ext.dmax : 0.0;
     const double d2 = ext.dmax * ext.dmax;
     info->range_order = rm > 0 && 2 * kPi * a->k_rev * 0.625 * d2 * d2 / (rm * rm * rm) <= a->order_tol ? 1 : 0;
@@ -336,7 +355,7 @@ sarbp_status plan_args(const double* h_pos, const sarbp_grid* g,
 
 sarbp_status launch(const BpArgs& a, const sarbp_params* p, const sarbp_tuning& t, hipStream_t s) {
   if (a.nx == 0 || a.ny == 0) return SARBP_OK;
-  hipError_t e = launch_plain(a, p->interp, p->precision, p->dtype, t.cols_per_lane, t.rows_per_lane, s);
+  hipError_t e = launch_plain(a, a.kernel_interp, p->precision, p->dtype, t.cols_per_lane, t.rows_per_lane, s);
   if (e != hipSuccess) return hip_fail(e, "BP kernel launch");
   return SARBP_OK;
 }
@@ -543,7 +562,12 @@ sarbp_status sarbp_backproject_batch(sarbp_context* ctx, const sarbp_job* jobs, 
   // Each job runs with the tile shape a plain launch of it would use (the
   // tuned shape, reduced to fit its grid), so its image is bitwise the plain
   // launch's; jobs that share a shape share one persistent launch.
-  std::vector<std::pair<sarbp_tuning, std::vector<JobDesc>>> groups;
+  struct Group {
+    sarbp_tuning tune;
+    int kinterp;
+    std::vector<JobDesc> desc;
+  };
+  std::vector<Group> groups;
   for (int32_t j : order) {
     const sarbp_job& jb = jobs[j];
     if (jb.grid.nx == 0 || jb.grid.ny == 0) continue;
@@ -557,19 +581,20 @@ sarbp_status sarbp_backproject_batch(sarbp_context* ctx, const sarbp_job* jobs, 
     d.a.img = reinterpret_cast<float2*>(jb.d_image);
     d.item_begin = 0;
     size_t g = 0;
-    while (g < groups.size() && !(groups[g].first.cols_per_lane == t.cols_per_lane &&
-                                  groups[g].first.rows_per_lane == t.rows_per_lane))
+    while (g < groups.size() && !(groups[g].tune.cols_per_lane == t.cols_per_lane &&
+                                  groups[g].tune.rows_per_lane == t.rows_per_lane &&
+                                  groups[g].kinterp == d.a.kernel_interp))
       ++g;
-    if (g == groups.size()) groups.emplace_back(t, std::vector<JobDesc>());
-    groups[g].second.push_back(d);
+    if (g == groups.size()) groups.push_back(Group{t, d.a.kernel_interp, {}});
+    groups[g].desc.push_back(d);
   }
   if (groups.size() > sizeof ctx->batch / sizeof ctx->batch[0])
     return fail(SARBP_ERR_UNSUPPORTED, "%zu tile shapes in one batch", groups.size());
   for (size_t g = 0; g < groups.size(); ++g) {
     if (!ctx->batch[g] && !(ctx->batch[g] = new (std::nothrow) BatchExec))
       return fail(SARBP_ERR_OUT_OF_MEMORY, "batch executor");
-    const sarbp_status st = ctx->batch[g]->run(groups[g].second, p0.interp, p0.precision, p0.dtype,
-                                               groups[g].first, ctx->n_cu, (hipStream_t)stream);
+    const sarbp_status st = ctx->batch[g]->run(groups[g].desc, groups[g].kinterp, p0.precision, p0.dtype,
+                                               groups[g].tune, ctx->n_cu, (hipStream_t)stream);
     if (st != SARBP_OK) return st;
   }
   return SARBP_OK;
@@ -756,6 +781,13 @@ sarbp_status sarbp_sinc8_coefficients(double beta, float* out) {
   if (!out) return fail(SARBP_ERR_INVALID_ARGUMENT, "out is NULL");
   if (!(std::isfinite(beta) && beta >= 0 && beta <= 50)) return fail(SARBP_ERR_INVALID_ARGUMENT, "beta");
   sinc8_poly(beta, out);
+  return SARBP_OK;
+}
+
+sarbp_status sarbp_sinc8_coefficients_whole(double beta, float* out) {
+  if (!out) return fail(SARBP_ERR_INVALID_ARGUMENT, "out is NULL");
+  if (!(std::isfinite(beta) && beta >= 0 && beta <= 50)) return fail(SARBP_ERR_INVALID_ARGUMENT, "beta");
+  sinc8_poly_whole(beta, out);
   return SARBP_OK;
 }
 

@@ -22,6 +22,7 @@ hipError_t launch_plain(const BpArgs& a, int interp, int precision, int dtype, i
     case kNearest: return launch_plain_nearest(a, precision, dtype, px, py, stream);
     case kLinear: return launch_plain_linear(a, precision, dtype, px, py, stream);
     case kSinc8: return launch_plain_sinc8(a, precision, dtype, px, py, stream);
+    case kSinc8W: return launch_plain_sinc8w(a, precision, dtype, px, py, stream);
     default: return hipErrorInvalidValue;
   }
 }
@@ -36,6 +37,8 @@ hipError_t launch_batch(const JobDesc* d_jobs, const BatchHeader* d_hdr, unsigne
       return launch_batch_linear(d_jobs, d_hdr, d_counter, precision, dtype, px, py, shm, n_blocks, stream);
     case kSinc8:
       return launch_batch_sinc8(d_jobs, d_hdr, d_counter, precision, dtype, px, py, shm, n_blocks, stream);
+    case kSinc8W:
+      return launch_batch_sinc8w(d_jobs, d_hdr, d_counter, precision, dtype, px, py, shm, n_blocks, stream);
     default: return hipErrorInvalidValue;
   }
 }
@@ -45,6 +48,7 @@ int batch_blocks_per_cu(int interp, int precision, int dtype, int px, int py, si
     case kNearest: return blocks_per_cu_nearest(precision, dtype, px, py, shm);
     case kLinear: return blocks_per_cu_linear(precision, dtype, px, py, shm);
     case kSinc8: return blocks_per_cu_sinc8(precision, dtype, px, py, shm);
+    case kSinc8W: return blocks_per_cu_sinc8w(precision, dtype, px, py, shm);
     default: return 1;
   }
 }

@@ -14,16 +14,26 @@ namespace sarbp {
 constexpr int kTileX = 64;
 constexpr int kWaves = 4;
 constexpr int kThreads = kTileX * kWaves;
-// Sinc8: the weight of tap t (sample i0 - 3 + t) at fractional bin f, in
-// half h = floor(2 f) of the bin, is the degree-5 polynomial
-// sum_k C[h][k][t] g^k of g = 2 f - h.
+// Sinc8 comes in two kernels.  kSinc8 (half bins): the weight of tap t
+// (sample i0 - 3 + t) at fractional bin f, in half h = floor(2 f) of the bin,
+// is the degree-5 polynomial sum_k C[h][k][t] g^k of g = 2 f - h.  kSinc8W
+// (whole bins): the degree-7 polynomial sum_k C[k][t] f^k.  Half bins cost a
+// pixel 10 FMA instead of 14 and a window position 192 FMA instead of 128 of
+// coefficient precompute; the planner picks per launch.
 constexpr int kSincTaps = 8;
 constexpr int kSincHalves = 2;
 constexpr int kSincCoef = 6;
+constexpr int kSincCoefW = 8;
 // Per-pulse parameter record in LDS (12 floats or 4 doubles).
 constexpr int kParBytes = 48;
 
-enum { kNearest = 0, kLinear = 1, kSinc8 = 2 };
+enum { kNearest = 0, kLinear = 1, kSinc8 = 2, kSinc8W = 3 };
+
+// Polynomial rows (of kSincTaps coefficients) per window position: the
+// interpolant's complex coefficients D per position, 8 B each.
+__host__ __device__ constexpr int sinc_rows(int interp) {
+  return interp == kSinc8 ? kSincHalves * kSincCoef : (interp == kSinc8W ? kSincCoefW : 0);
+}
 enum { kRef = 0, kF64 = 1, kF32 = 2 };
 
 // Everything one plain launch needs; passed by value as the kernel argument.
@@ -43,7 +53,9 @@ struct BpArgs {
   int32_t pulses_per_stage;
   int32_t pulse_begin, pulse_end;  // pulses [begin, end) are summed
   int32_t accumulate;
-  float sinc_poly[kSincHalves * kSincCoef * kSincTaps];  // C[(h * 6 + k) * 8 + t]
+  int32_t kernel_interp;  // kNearest, kLinear, kSinc8 or kSinc8W
+  // kSinc8: C[(h * 6 + k) * 8 + t]; kSinc8W: C[k * 8 + t]
+  float sinc_poly[kSincHalves * kSincCoef * kSincTaps];
 };
 
 // Batched (persistent) launch: one work item = one tile of one job.
@@ -67,16 +79,16 @@ __host__ __device__ inline int setup_batch(int P) { return P >= 64 ? P : P * (64
 
 // LDS layout of one workgroup (byte offsets), SB = setup_batch(P):
 //   raw   [P][win] float2         staged range windows
-//   coef  [P][win-7][2][6] float2 sinc8 only: polynomial coefficients D_hk of
-//                                 the interpolant at each window position and
-//                                 half bin
+//   coef  [P][win-7][R] float2    sinc8 only: the R = sinc_rows polynomial
+//                                 coefficients of the interpolant at each
+//                                 window position ([2][6] half bins or [8])
 //   par   [SB] 48-B pulse records
 //   rng   [SB] int2: first staged sample, first sinc8 position of each pulse
 //   start [SB] int window starts
 //   stage [SB/P][3] int per stage: range-expansion order, staged samples and
 //                   sinc8 positions per pulse
-//   poly  [2][6][8] float sinc8 only: the interpolant's polynomial coefficients
-//                   (read as broadcasts; too many to stay in scalar registers)
+//   poly  [R][8] float  sinc8 only: the tap polynomials C (read as broadcasts;
+//                       too many to stay in scalar registers)
 struct LdsLayout {
   size_t raw, coef, par, rng, start, stage, poly, total;
 };
@@ -88,7 +100,7 @@ __host__ __device__ inline LdsLayout lds_layout(int interp, int win, int P) {
   L.raw = o;
   o += (size_t)P * win * 8;
   L.coef = o;
-  if (interp == kSinc8) o += (size_t)P * (win - 7) * kSincHalves * kSincCoef * 8;
+  o += (size_t)P * (win - 7) * sinc_rows(interp) * 8;
   L.par = o;
   o += (size_t)SB * kParBytes;
   L.rng = o;
@@ -99,7 +111,7 @@ __host__ __device__ inline LdsLayout lds_layout(int interp, int win, int P) {
   o += (size_t)(SB / P) * 12;
   o = (o + 15) & ~(size_t)15;
   L.poly = o;
-  if (interp == kSinc8) o += (size_t)kSincHalves * kSincCoef * kSincTaps * 4;
+  o += (size_t)sinc_rows(interp) * kSincTaps * 4;
   L.total = (o + 15) & ~(size_t)15;
   return L;
 }
@@ -123,6 +135,7 @@ inline bool tile_shape_supported(int precision, int px, int py) {
 SARBP_DECLARE_TU(nearest)
 SARBP_DECLARE_TU(linear)
 SARBP_DECLARE_TU(sinc8)
+SARBP_DECLARE_TU(sinc8w)
 #undef SARBP_DECLARE_TU
 
 // Launches the plain tile kernel.  Returns hipSuccess or the launch error.

@@ -290,8 +290,13 @@ void sarbp_server::sched_loop() {
   std::deque<Epoch> inflight;
   int k = 0;
   for (;;) {
-    // chunks grouped by tile shape: one batched launch per shape
-    std::vector<std::pair<sarbp_tuning, std::vector<JobDesc>>> groups;
+    // chunks grouped by tile shape and kernel: one batched launch per group
+    struct Group {
+      sarbp_tuning tune;
+      int kinterp;
+      std::vector<JobDesc> desc;
+    };
+    std::vector<Group> groups;
     Epoch ep;
     {
       std::unique_lock<std::mutex> lk(mu);
@@ -328,11 +333,12 @@ void sarbp_server::sched_loop() {
         d.a.accumulate = j.next_pulse > 0 ? 1 : 0;
         d.item_begin = 0;
         size_t g = 0;
-        while (g < groups.size() && !(groups[g].first.cols_per_lane == j.tune.cols_per_lane &&
-                                      groups[g].first.rows_per_lane == j.tune.rows_per_lane))
+        while (g < groups.size() && !(groups[g].tune.cols_per_lane == j.tune.cols_per_lane &&
+                                      groups[g].tune.rows_per_lane == j.tune.rows_per_lane &&
+                                      groups[g].kinterp == j.args.kernel_interp))
           ++g;
-        if (g == groups.size()) groups.emplace_back(j.tune, std::vector<JobDesc>());
-        groups[g].second.push_back(d);
+        if (g == groups.size()) groups.push_back(Group{j.tune, j.args.kernel_interp, {}});
+        groups[g].desc.push_back(d);
         j.next_pulse = d.a.pulse_end;
         j.started = true;
         ++j.inflight;
@@ -355,8 +361,8 @@ void sarbp_server::sched_loop() {
     if (e == hipSuccess) e = hipEventRecord(ep.ev0, s_compute);
     sarbp_status st = e == hipSuccess ? SARBP_OK : hip_fail(e, "epoch events");
     for (size_t g = 0; st == SARBP_OK && g < groups.size(); ++g)
-      st = g < 8 ? exec[k & 1][g].run(groups[g].second, cfg.interp, cfg.precision, cfg.dtype, groups[g].first,
-                                      n_cu, s_compute)
+      st = g < 8 ? exec[k & 1][g].run(groups[g].desc, groups[g].kinterp, cfg.precision, cfg.dtype,
+                                      groups[g].tune, n_cu, s_compute)
                  : fail(SARBP_ERR_UNSUPPORTED, "more than 8 tile shapes in an epoch");
     if (st == SARBP_OK && (e = hipEventRecord(ep.ev1, s_compute)) != hipSuccess) st = hip_fail(e, "hipEventRecord");
     if (st != SARBP_OK) {

@@ -55,7 +55,8 @@ class PlanInfo(C.Structure):
                 ("window", C.c_int32), ("pulses_per_stage", C.c_int32),
                 ("rows_per_lane", C.c_int32), ("cols_per_lane", C.c_int32),
                 ("range_order", C.c_int32),
-                ("lds_bytes", C.c_int32), ("max_phase_err_rad", C.c_double)]
+                ("lds_bytes", C.c_int32), ("max_phase_err_rad", C.c_double),
+                ("sinc8_halves", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Tuning(C.Structure):
@@ -166,6 +167,7 @@ _SIGS = {
     "sarbp_get_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(Tuning)]),
     "sarbp_backproject_batch": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_int32, C.c_void_p]),
     "sarbp_sinc8_coefficients": (C.c_int, [C.c_double, C.POINTER(C.c_float)]),
+    "sarbp_sinc8_coefficients_whole": (C.c_int, [C.c_double, C.POINTER(C.c_float)]),
     "sarbp_launch_backprojection": (C.c_int, [C.c_void_p, C.c_void_p, SarParams, C.c_int32,
                                               C.c_int32]),
     "sarbp_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
@@ -348,6 +350,14 @@ def sinc8_coefficients(beta: float):
     out = (C.c_float * 96)()
     check(load().sarbp_sinc8_coefficients(float(beta), out), "sarbp_sinc8_coefficients")
     return np.frombuffer(out, dtype=np.float32).reshape(2, 6, 8).copy()
+
+
+def sinc8_coefficients_whole(beta: float):
+    """The whole-bin sinc8 kernel's coefficients as a numpy [8 powers][8 taps]."""
+    import numpy as np
+    out = (C.c_float * 64)()
+    check(load().sarbp_sinc8_coefficients_whole(float(beta), out), "sarbp_sinc8_coefficients_whole")
+    return np.frombuffer(out, dtype=np.float32).reshape(8, 8).copy()
 
 
 class Plan:

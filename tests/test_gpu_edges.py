@@ -145,6 +145,41 @@ def test_grid_too_coarse_for_every_tile_is_refused(ctx, base):
     assert ei.value.status == native.ERR_UNSUPPORTED
 
 
+def test_sinc8_kernel_variants(ctx, base):
+    """The planner runs the half-bin kernel on a fine grid and the whole-bin
+    kernel where tiles are coarse against the range bins; both hold the
+    oracle bound, and one batch of both jobs reproduces their planned (host
+    positions known, like the batch) launches bit for bit."""
+    import torch
+    sc, ds = base
+    o = sc.grid["origin"]
+    grids = [{"origin": o, "ux": [0.5, 0, 0], "uy": [0, 0.5, 0], "nx": 90, "ny": 60},
+             {"origin": o, "ux": [1.25, 0, 0], "uy": [0, 1.25, 0], "nx": 70, "ny": 50}]
+    halves = []
+    imgs = []
+    jobs = []
+    for g in grids:
+        ng = native.make_grid(g["origin"], g["ux"], g["uy"], g["nx"], g["ny"])
+        plan = ctx.plan(sc.pos, ng, sc.native_params(interp="sinc8"))
+        halves.append(plan.info()["sinc8_halves"])
+        out0 = torch.zeros((g["ny"], g["nx"]), dtype=torch.complex64, device="cuda")
+        plan.execute(ds.prof.data_ptr(), ds.pos.data_ptr(), out0.data_ptr(), torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        img = out0.cpu().numpy()
+        imgs.append(img)
+        assert M.rel_l2(img, _oracle(sc, grid=g)) <= 1e-4
+        jb = native.Job()
+        out = torch.zeros((g["ny"], g["nx"]), dtype=torch.complex64, device="cuda")
+        jb.d_profiles, jb.d_pos, jb.h_pos = ds.prof.data_ptr(), ds.pos.data_ptr(), sc.pos.ctypes.data
+        jb.grid, jb.params, jb.d_image = ng, sc.native_params(interp="sinc8"), out.data_ptr()
+        jobs.append((jb, out))
+    assert sorted(halves) == [1, 2], halves
+    ctx.backproject_batch([j for j, _ in jobs], torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    for (_, out), img in zip(jobs, imgs):
+        assert np.array_equal(out.cpu().numpy(), img)
+
+
 def test_samples_beyond_the_profile_read_zero(ctx, base):
     """Shift the range axis so pixels fall off both ends of the profile."""
     sc, ds = base

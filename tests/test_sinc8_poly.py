@@ -49,3 +49,15 @@ def test_poly_partition_of_sample_at_integer_bins():
 def test_bad_beta_rejected():
     with pytest.raises(native.SarbpError):
         native.sinc8_coefficients(float("nan"))
+
+
+@pytest.mark.parametrize("beta", [0.0, 6.0, 8.0])
+def test_whole_bin_poly_matches_kaiser_sinc(beta):
+    """The whole-bin kernel's degree-7 tap polynomials (used for tiles coarse
+    against the range bins)."""
+    C = native.sinc8_coefficients_whole(beta).astype(np.float64)   # [k][t]
+    f = np.linspace(0.0, 1.0, 1001)
+    w = C.T @ np.stack([f ** k for k in range(8)])
+    ref = np.array([[O.kaiser_sinc8(fi + 3 - t, beta) for fi in f] for t in range(8)])
+    err = np.abs(w - ref).max()
+    assert err < (5e-6 if beta <= 6 else 2e-5), err
