@@ -162,10 +162,56 @@ sarbp_tuning default_tuning(int interp) {
   return t;
 }
 
-// Largest |pixel - tile reference| the FP32 differential supports: pp = |d|^2
-// - s^2 carries ~ulp(dmax^2) of rounding into q = pp / (2R), which must stay
-// far below a wavelength (at 48 m, ~1e-7 m for R >= 3 km).
+// Largest |pixel - tile reference| the FP32 differential supports.  Two FP32
+// roundings grow with it: pp = |d|^2 - s^2 carries ~ulp(|d|^2) into
+// q = pp / (2R), a phase error ~ k ulp(|d|^2) / (2R) (at 48 m and R = 5 km,
+// 5e-6 rad at X band), and s = e.d carries ~ulp(|d|) into the range bin
+// (at 48 m and dr = 0.19 m, 1.5e-5 bins).  The limit is 48 m, or, when the
+// antenna positions bound the range from below, what keeps both errors at
+// those levels: 48 m x sqrt(R_min / 5 km), at most 256 range bins.
 constexpr double kMaxTileRadius = 48.0;
+
+double radius_limit(double rmin, double dr) {
+  if (!(rmin > 5000.0)) return kMaxTileRadius;
+  return std::max(kMaxTileRadius, std::min(kMaxTileRadius * std::sqrt(rmin / 5000.0), 256.0 * dr));
+}
+
+double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// Smallest distance from an antenna position to the imaged area: the grid's
+// parallelogram extended by one tile of px x py pixels per lane on every side
+// (tiles overhang ragged edges).  0 without positions or for a degenerate grid.
+double min_range(const sarbp_grid* g, int px, int py, const double* h_pos, int n) {
+  if (!h_pos || n <= 0) return 0.0;
+  const double tx = kTileX * px, ty = kWaves * py;
+  const double a11 = dot3(g->ux, g->ux), a12 = dot3(g->ux, g->uy), a22 = dot3(g->uy, g->uy);
+  const double det = a11 * a22 - a12 * a12;
+  if (!(det > 0.0)) return 0.0;
+  const double i0 = -tx, i1 = std::max(g->nx - 1, 0) + tx, j0 = -ty, j1 = std::max(g->ny - 1, 0) + ty;
+  // in-plane distance from plane coordinates (i, j) to the segment (pi, pj) + t (qi, qj), t in [0, 1]
+  auto seg = [&](double i, double j, double pi, double pj, double qi, double qj) {
+    const double di = i - pi, dj = j - pj;
+    const double qq = qi * qi * a11 + 2 * qi * qj * a12 + qj * qj * a22;
+    double t = qq > 0 ? (di * qi * a11 + (di * qj + dj * qi) * a12 + dj * qj * a22) / qq : 0.0;
+    t = std::min(1.0, std::max(0.0, t));
+    const double ei = di - t * qi, ej = dj - t * qj;
+    return std::sqrt(std::max(0.0, ei * ei * a11 + 2 * ei * ej * a12 + ej * ej * a22));
+  };
+  double best = INFINITY;
+  for (int m = 0; m < n; ++m) {
+    const double* a = h_pos + 3 * (size_t)m;
+    const double w[3] = {a[0] - g->origin[0], a[1] - g->origin[1], a[2] - g->origin[2]};
+    const double b1 = dot3(w, g->ux), b2 = dot3(w, g->uy);
+    const double i = (b1 * a22 - b2 * a12) / det, j = (a11 * b2 - a12 * b1) / det;
+    const double off2 = std::max(0.0, dot3(w, w) - (i * b1 + j * b2));  // squared distance to the plane
+    double in = 0.0;
+    if (i < i0 || i > i1 || j < j0 || j > j1)
+      in = std::min(std::min(seg(i, j, i0, j0, i1 - i0, 0), seg(i, j, i0, j1, i1 - i0, 0)),
+                    std::min(seg(i, j, i0, j0, 0, j1 - j0), seg(i, j, i1, j0, 0, j1 - j0)));
+    best = std::min(best, std::sqrt(off2 + in * in));
+  }
+  return std::isfinite(best) ? best : 0.0;
+}
 
 // Largest |pixel - tile reference pixel| over a tile of px x py pixels per lane.
 double tile_radius(const sarbp_grid* g, int px, int py) {
@@ -182,14 +228,13 @@ double tile_radius(const sarbp_grid* g, int px, int py) {
 
 // The tuning with its tile shape reduced, if needed, to the largest compiled
 // shape no wider and no taller than the requested one whose tiles stay within
-// kMaxTileRadius on every grid (SARBP_PREC_REF only).  A grid too coarse for
-// every shape keeps the request, which plan_args then refuses.
-sarbp_tuning fit_tuning(sarbp_tuning t, int precision, const sarbp_grid* const* grids, int n) {
+// the radius limit (SARBP_PREC_REF only).  A grid too coarse for every shape
+// keeps the request, which plan_args then refuses.
+sarbp_tuning fit_tuning(sarbp_tuning t, int precision, const sarbp_grid* g, const sarbp_params* p,
+                        const double* h_pos) {
   if (precision != SARBP_PREC_REF) return t;
   auto fits = [&](int px, int py) {
-    for (int i = 0; i < n; ++i)
-      if (tile_radius(grids[i], px, py) > kMaxTileRadius) return false;
-    return true;
+    return tile_radius(g, px, py) <= radius_limit(min_range(g, px, py, h_pos, p->n_pulses), p->dr);
   };
   if (fits(t.cols_per_lane, t.rows_per_lane)) return t;
   int best_px = 0, best_py = 0;
@@ -281,10 +326,11 @@ sarbp_status plan_args(const double* h_pos, const sarbp_grid* g,
   if (!tile_shape_supported(p->precision, px, py))
     return fail(SARBP_ERR_INVALID_ARGUMENT, "unsupported tile shape: %d cols x %d rows per lane", px, py);
   const Extent ext = tile_extent(g, px, py, h_pos, p->n_pulses);
-  if (p->precision == SARBP_PREC_REF && ext.dmax > kMaxTileRadius)
+  const double rlim = radius_limit(min_range(g, px, py, h_pos, p->n_pulses), p->dr);
+  if (p->precision == SARBP_PREC_REF && ext.dmax > rlim)
     return fail(SARBP_ERR_UNSUPPORTED,
                 "tile of %d x %d pixels spans %.1f m from its centre (limit %.0f m): use fewer pixels per lane",
-                kTileX * px, kWaves * py, ext.dmax, kMaxTileRadius);
+                kTileX * px, kWaves * py, ext.dmax, rlim);
   // half window in samples: extent + sinc taps (3 left / 4 right) + 2 margin
   const double half_bins = std::ceil(ext.half_m / p->dr) + 6.0;
   if (half_bins > 4096.0)
@@ -451,7 +497,8 @@ sarbp_status sarbp_backproject(sarbp_context* ctx, const void* d_profiles, const
   DeviceGuard dg(ctx->device);
   if (dg.err != hipSuccess) return hip_fail(dg.err, "hipSetDevice");
   const sarbp_tuning tune =
-      fit_tuning(get_tuning(ctx, params->interp, params->precision, params->dtype), params->precision, &grid, 1);
+      fit_tuning(get_tuning(ctx, params->interp, params->precision, params->dtype), params->precision, grid, params,
+                 nullptr);
   BpArgs a;
   st = plan_args(nullptr, grid, params, tune, &a, nullptr);
   if (st != SARBP_OK) return st;
@@ -489,7 +536,8 @@ sarbp_status sarbp_plan_create(sarbp_context* ctx, const double* h_pos, const sa
     return hip_fail(dg.err, "hipSetDevice");
   }
   const sarbp_tuning tune =
-      fit_tuning(get_tuning(ctx, params->interp, params->precision, params->dtype), params->precision, &grid, 1);
+      fit_tuning(get_tuning(ctx, params->interp, params->precision, params->dtype), params->precision, grid, params,
+                 h_pos);
   std::memset(&pl->info, 0, sizeof pl->info);
   st = plan_args(h_pos, grid, params, tune, &pl->args, &pl->info);
   if (st != SARBP_OK) {
@@ -571,8 +619,7 @@ sarbp_status sarbp_backproject_batch(sarbp_context* ctx, const sarbp_job* jobs, 
   for (int32_t j : order) {
     const sarbp_job& jb = jobs[j];
     if (jb.grid.nx == 0 || jb.grid.ny == 0) continue;
-    const sarbp_grid* gp = &jb.grid;
-    const sarbp_tuning t = fit_tuning(base, p0.precision, &gp, 1);
+    const sarbp_tuning t = fit_tuning(base, p0.precision, &jb.grid, &jb.params, jb.h_pos);
     JobDesc d;
     sarbp_status st = plan_args(jb.h_pos, &jb.grid, &jb.params, t, &d.a, nullptr);
     if (st != SARBP_OK) return fail(st, "job %d: %s", j, g_detail.c_str());

@@ -40,8 +40,10 @@ struct DeviceGuard {
 sarbp_status validate(const sarbp_grid* g, const sarbp_params* p);
 sarbp_tuning default_tuning(int interp);
 // The tuning with its tile shape reduced, if needed, to the largest compiled
-// shape within it whose tiles fit the REF tile-radius limit on every grid.
-sarbp_tuning fit_tuning(sarbp_tuning t, int precision, const sarbp_grid* const* grids, int n);
+// shape within it whose tiles fit the REF tile-radius limit on this grid
+// (h_pos, if given, bounds the range from below and relaxes the limit).
+sarbp_tuning fit_tuning(sarbp_tuning t, int precision, const sarbp_grid* g, const sarbp_params* p,
+                        const double* h_pos);
 // Fills a launch descriptor (device pointers left NULL) for one image.
 sarbp_status plan_args(const double* h_pos, const sarbp_grid* g, const sarbp_params* p,
                        const sarbp_tuning& tune, BpArgs* a, sarbp_plan_info* info);

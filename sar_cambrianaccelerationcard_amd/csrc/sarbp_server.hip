@@ -532,8 +532,7 @@ sarbp_status submit_job(sarbp_server* s, const sarbp_request* req, const sarbp_r
   std::unique_ptr<Job> j(new (std::nothrow) Job);
   if (!j) return fail(SARBP_ERR_OUT_OF_MEMORY, "job");
   j->req = *req;
-  const sarbp_grid* gp = &req->grid;
-  j->tune = fit_tuning(s->tune, p.precision, &gp, 1);
+  j->tune = fit_tuning(s->tune, p.precision, &req->grid, &p, req->h_pos);
   if ((st = plan_args(req->h_pos, &req->grid, &p, j->tune, &j->args, nullptr)) != SARBP_OK) return st;
   j->prof_bytes = (size_t)p.n_pulses * p.n_samples * (p.dtype == SARBP_DTYPE_C64 ? 8 : 4);
   j->pos_bytes = (size_t)p.n_pulses * 24;

@@ -7,6 +7,8 @@ from __future__ import annotations
 import hashlib
 import json
 import os
+import struct
+import zlib
 from typing import Optional
 
 import numpy as np
@@ -57,3 +59,20 @@ def magnitude_db(image: np.ndarray, floor_db: float = -60.0) -> np.ndarray:
     if peak <= 0:
         return np.full(a.shape, floor_db)
     return np.maximum(20.0 * np.log10(np.maximum(a / peak, 1e-30)), floor_db)
+
+
+def save_png(path: str, image: np.ndarray, floor_db: float = -40.0) -> None:
+    """Writes |image| in dB (0 dB = the peak, clipped at floor_db) as an 8-bit
+    greyscale PNG, rows top to bottom.  Uses only zlib (no imaging library)."""
+    db = magnitude_db(np.asarray(image), floor_db)
+    g = np.clip(np.rint((db - floor_db) * (255.0 / -floor_db)), 0, 255).astype(np.uint8)
+    h, w = g.shape
+    raw = b"".join(b"\x00" + g[r].tobytes() for r in range(h))
+
+    def chunk(kind: bytes, data: bytes) -> bytes:
+        return struct.pack(">I", len(data)) + kind + data + struct.pack(">I", zlib.crc32(kind + data) & 0xFFFFFFFF)
+
+    png = b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 0, 0, 0, 0)) + \
+        chunk(b"IDAT", zlib.compress(raw, 6)) + chunk(b"IEND", b"")
+    with open(path, "wb") as f:
+        f.write(png)
