@@ -24,7 +24,8 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "_lib")
 INCLUDE = os.path.join(ROOT, "include")
 
-HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.environ.get("HIPCC", os.path.join(ROCM, "bin", "hipcc"))
 ARCH = "gfx950"
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
@@ -91,8 +92,25 @@ def build_sarsim(force: bool = True) -> str:
     return out
 
 
+def build_run(force: bool = True) -> str:
+    """The sarbp_run command-line imager (host code only, links both libraries)."""
+    out = os.path.join(LIB, "sarbp_run")
+    src = os.path.join(CSRC, "sarbp_run.cpp")
+    deps = [src, os.path.join(INCLUDE, "sarbp.h"), os.path.join(CSRC, "sarsim.h"),
+            os.path.join(LIB, "libsarbp.so"), os.path.join(LIB, "libsarsim.so")]
+    if not force and _newer(out, deps):
+        return out
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__",
+          "-I" + os.path.join(ROCM, "include"), "-I" + INCLUDE, "-I" + CSRC, src, "-o", tmp,
+          "-L" + LIB, "-lsarbp", "-lsarsim", "-L" + os.path.join(ROCM, "lib"), "-lamdhip64",
+          "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(ROCM, "lib")])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all(force: bool = True) -> list[str]:
-    return [build_sarbp(force), build_sarsim(force)]
+    return [build_sarbp(force), build_sarsim(force), build_run(force)]
 
 
 if __name__ == "__main__":

@@ -66,3 +66,21 @@ def test_slant_plane_bp_of_real_echoes(ctx, crop, interp):
     err = M.rel_l2(img.cpu().numpy(), ref)
     print(f"  tile {info['tile_nx']}x{info['tile_ny']} window {info['window']} rel_l2={err:.3e}", flush=True)
     assert err <= 1e-4
+
+
+def test_imaging_command_on_the_crop(tmp_path):
+    """python -m sar_cambrianaccelerationcard_amd.radarsat --crop ...: writes the
+    image (+ sidecar) and its dB PNG."""
+    import subprocess
+    import sys
+    from sar_cambrianaccelerationcard_amd import imageio as io
+    out = str(tmp_path / "cd")
+    r = subprocess.run([sys.executable, "-m", "sar_cambrianaccelerationcard_amd.radarsat", "--crop", CROP,
+                        "--out", out, "--nx", "128", "--ny", "96", "--dx", "6"],
+                       capture_output=True, text=True, timeout=300,
+                       cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    print(r.stdout, r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    img = io.load_image(out + ".bin")
+    assert img.shape == (96, 128) and np.all(np.isfinite(img)) and np.abs(img).max() > 0
+    assert os.path.getsize(out + ".png") > 0
