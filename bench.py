@@ -27,6 +27,12 @@ Extra legs (reported in the same JSON line):
             same scene size (4096 pulses x 2448 samples) -> 8192 4x-upsampled
             samples, against the HBM roofline (8 TB/s), with the FP64 oracle
             timed on a pulse sample
+  omega_k   the reference's second imaging path at its scene size: 3072 x
+            2048 raw echoes (synthetic point targets, RADARSAT-1 radar) ->
+            4096 x 3414 padded image (pad + down-conversion, 2-D FFT,
+            reference function + 6-tap Stolt, 2-D inverse FFT); the Stolt
+            kernel alone against the HBM roofline (16 B per sample); the FP64
+            oracle timed on the whole image
 """
 from __future__ import annotations
 
@@ -108,6 +114,65 @@ def leg_range(torch, ctx, native, D, args) -> dict:
         log(f"range oracle {n_s} pulses: {tcpu:.2f}s ({n_s / tcpu:.1f} pulses/s), rel-L2 "
             f"{res['rel_l2_sample']:.2e}")
     del raw, out, out16
+    torch.cuda.empty_cache()
+    return res
+
+
+def leg_omega_k(torch, ctx, native, args) -> dict:
+    """omega-K image formation at the reference's scene size."""
+    from sar_cambrianaccelerationcard_amd import device as D
+    from sar_cambrianaccelerationcard_amd import omegak as O
+    scene = O.radarsat_scene()
+    targets = [O.aperture_target(scene, a, s) for a, s in
+               ((900, 1400), (1300, 1500), (1700, 1600), (2100, 1450), (2500, 1650), (2900, 1550))]
+    echo_h = O.simulate(scene, targets)
+    na, nr = scene.padded
+    echo = torch.from_numpy(echo_h).to("cuda")
+    img = torch.empty((na, nr), dtype=torch.complex64, device="cuda")
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    wp = scene.params()
+    fn = lambda: ctx.omega_k(echo.data_ptr(), scene.na_in, scene.nr_in, wp, img.data_ptr(), sp)
+    tp = time.perf_counter()
+    fn()
+    torch.cuda.synchronize()
+    t_plan = time.perf_counter() - tp
+    fn()
+    t = timed(torch, stream, fn, 10)
+    n_pad = na * nr
+    res = {"workload": f"{scene.na_in} x {scene.nr_in} raw complex64 echoes -> {na} x {nr} padded image, "
+                       f"6-tap floor-window Stolt, synthetic point targets (RADARSAT-1 radar, squint "
+                       f"f_nc = {scene.f_nc:.0f} Hz)",
+           "ms": t * 1e3, "images_per_s": 1.0 / t, "echo_samples_per_s": scene.na_in * scene.nr_in / t,
+           "first_call_ms": t_plan * 1e3}
+    log(f"omega-K: {t * 1e3:.3f} ms per image ({scene.na_in * scene.nr_in / t:.3e} echo samples/s), "
+        f"first call {t_plan * 1e3:.0f} ms")
+    # The Stolt kernel alone on the padded spectrum (no reference function):
+    # 8 B read + 8 B written per sample.
+    fr = np.fft.fftshift(-nr / 2.0 + np.arange(nr)) * scene.fs / nr
+    fa = scene.f_nc + np.fft.fftshift(-na / 2.0 + np.arange(na)) * scene.prf / na
+    out = torch.empty_like(img)
+    fs_ = lambda: ctx.stolt_sinc(img.data_ptr(), out.data_ptr(), fr, fa, f0=scene.f0, c=scene.c, vr=scene.vr,
+                                 na=na, nr=nr, fs=scene.fs, taps=6, window=0, stream=sp)
+    fs_()
+    ts = timed(torch, stream, fs_, 10)
+    byts = 16 * n_pad
+    res["stolt"] = {"ms": ts * 1e3, "GB_per_s": byts / ts / 1e9, "frac_hbm": byts / ts / 8e12, "bytes": byts}
+    log(f"Stolt kernel: {ts * 1e3:.3f} ms, {byts / ts / 1e9:.0f} GB/s ({byts / ts / 8e12:.1%} of 8 TB/s)")
+    g = img.cpu().numpy()
+    res["focus"] = [O.peak_metrics(g, *tg) for tg in targets]
+    if args.cpu_rows > 0:
+        import oracle.wk_oracle as WK
+        tc = time.perf_counter()
+        ref = WK.omega_k(echo_h, c=wp.c, f0=wp.f0, fs=wp.fs, prf=wp.prf, kr=wp.kr, vr=wp.vr, r0=wp.r0,
+                         f_nc=wp.f_nc, taps=wp.taps, window=wp.window)
+        tcpu = time.perf_counter() - tc
+        res["rel_l2"] = float(np.linalg.norm(g - ref) / np.linalg.norm(ref))
+        res["cpu_baseline"] = {"value": scene.na_in * scene.nr_in / tcpu, "unit": "echo samples/s", "cores": 1,
+                               "kind": "port", "sample": f"FP64 numpy oracle (oracle/wk_oracle.py), the whole "
+                                                         f"image, {tcpu:.2f} s"}
+        log(f"omega-K oracle: {tcpu:.1f}s, rel-L2 {res['rel_l2']:.2e}")
+    del echo, img, out
     torch.cuda.empty_cache()
     return res
 
@@ -280,6 +345,9 @@ def main() -> int:
     ap.add_argument("--cols", type=int, default=0, help="columns per lane (skips the autotuner)")
     ap.add_argument("--pulses-per-stage", type=int, default=0)
     ap.add_argument("--range-pulses", type=int, default=4096, help="range-compression leg (0: skip)")
+    ap.add_argument("--omega-k", type=int, default=1, help="omega-K leg (0: skip)")
+    ap.add_argument("--omega-k-only", action="store_true",
+                    help="only run the omega-K leg and print its JSON (profiling)")
     ap.add_argument("--mixed-jobs", type=int, default=64, help="batched leg jobs (0: skip)")
     ap.add_argument("--mixed-reps", type=int, default=2)
     ap.add_argument("--served-jobs", type=int, default=96, help="served leg jobs (0: skip)")
@@ -319,6 +387,9 @@ def main() -> int:
     if not torch.cuda.is_available():
         print("no HIP device", file=sys.stderr)
         return 3
+    if args.omega_k_only:
+        print(json.dumps({"omega_k": leg_omega_k(torch, native.Context(0), native, args)}))
+        return 0
     t0 = time.time()
     cfg = S.load_config(args.config)
     sc = S.make_scene(cfg)
@@ -385,6 +456,7 @@ def main() -> int:
     torch.cuda.empty_cache()
 
     rc = leg_range(torch, ctx, native, D, args) if args.range_pulses > 0 else None
+    wk = leg_omega_k(torch, ctx, native, args) if args.omega_k else None
     batched = leg_mixed(torch, ctx, native, W, args) if args.mixed_jobs > 0 else None
     served = None
     if args.served_jobs > 0:
@@ -466,6 +538,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "accuracy": acc,
         "range_compression": rc,
+        "omega_k": wk,
         "batched": batched,
         "served": served,
     }

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define SARBP_ABI_VERSION 2
+#define SARBP_ABI_VERSION 3
 
 typedef enum sarbp_status {
   SARBP_OK = 0,
@@ -433,6 +433,73 @@ struct sarbp_raw_request {
   sarbp_params params;       /* as sarbp_request; n_samples, r0, dr ignored  */
   sarbp_cf32* h_image;       /* [ny][nx] output                              */
 };
+
+/* ---- omega-K image formation ---------------------------------------------------
+ *
+ * The reference's second imaging path (code/SAR_WK/wk_v5.2.3, process_wk at
+ * wk.cc:43-343; MATLAB prototype code/SAR_WK/matlab/mian_wka_sinc.m).  Rows are
+ * azimuth lines (Na), columns range samples (Nr), row-major complex64.
+ *   fr[n] = fftshift(-Nr/2 : Nr/2-1)[n] * fs / Nr           (wk.cc:197-208)
+ *   fa[m] = f_nc + fftshift(-Na/2 : Na/2-1)[m] * prf / Na   (wk.cc:211-221)
+ *   s     = sqrt(max(0, (f0 + fr)^2 - c^2 fa^2 / (4 vr^2)))
+ * Stolt interpolation (stolt.cc:324-727): output (m, n) is the sum over the taps
+ * idx of in(m, idx) sinc(cen - idx), sinc(x) = sin(pi x)/(pi x), sinc(0) = 1,
+ * at cen = n - (s - f0 - fr[n]) Nr / fs.
+ */
+typedef enum sarbp_stolt_window {
+  /* idx = floor(cen) + p, p = -P/2 .. P/2-1, each clamped to [0, Nr-1] with the
+   * distance taken to the clamped index: the MLU code (stolt.cc:522-583). */
+  SARBP_STOLT_FLOOR = 0,
+  /* every idx in [0, Nr-1] with |cen - idx| <= P/2: the MATLAB prototype
+   * (mian_wka_sinc.m:98-111). */
+  SARBP_STOLT_CENTERED = 1
+} sarbp_stolt_window;
+
+/* Stolt interpolation along range frequency of d_in [na][nr] into d_out (may be
+ * d_in).  Replaces StoltInterp_sinc (code/SAR_WK/wk_v5.2.3/task/stolt.cc:324,
+ * declared task/task.h:37-44): h_fr_axis [nr] and h_fa_axis [na] are the
+ * range- and azimuth-frequency axes (host, Hz), fs the range sampling rate
+ * (its argument `fr`), taps = P (even, 2..16), window a sarbp_stolt_window;
+ * nr <= 8192.  The axes are copied to the device before the call returns.
+ * One workgroup per azimuth line: the line is staged in LDS, the mapping is
+ * evaluated in FP64, the taps in FP32. */
+sarbp_status sarbp_stolt_sinc(sarbp_context* ctx, const sarbp_cf32* d_in, sarbp_cf32* d_out,
+                              const double* h_fr_axis, const double* h_fa_axis, double f0, double c,
+                              double vr, int32_t na, int32_t nr, double fs, int32_t taps,
+                              int32_t window, void* stream);
+
+typedef struct sarbp_wk_params {
+  double c;       /* propagation speed, m/s                                   */
+  double f0;      /* carrier frequency, Hz                                    */
+  double fs;      /* range sampling rate Fr, Hz                               */
+  double prf;     /* azimuth sampling rate, Hz                                */
+  double kr;      /* signed chirp FM rate, Hz/s (negative for a down-chirp:
+                     the paras.txt convention, read.cc:54-82)                 */
+  double vr;      /* effective radar velocity, m/s                            */
+  double r0;      /* reference slant range, m                                 */
+  double f_nc;    /* Doppler centroid, Hz                                     */
+  int32_t taps;   /* Stolt taps P (6 in the reference, wk.cc:273)             */
+  int32_t window; /* sarbp_stolt_window                                       */
+} sarbp_wk_params;
+
+/* Padded size of an na_in x nr_in echo: round(na_in/6) zero lines before and
+ * after, round(nr_in/3) zero samples before and after each line
+ * (code/SAR_WK/wk_v5.2.3/task/preprocess.cc:21-24).  Host-only. */
+sarbp_status sarbp_wk_padded_size(int32_t na_in, int32_t nr_in, int32_t* na, int32_t* nr);
+
+/* omega-K image of raw echoes d_echo [na_in][nr_in] into d_image [na][nr]
+ * (the padded size; the buffer is also the transform workspace).  The steps
+ * of process_wk (wk.cc:87-285): zero padding and azimuth down-conversion by
+ * exp(-j 2 pi f_nc ta), ta[m] = (m - na/2) / prf (preprocess.cc:17-166); 2-D
+ * FFT (fft.cc:18-198); reference function exp(j (4 pi r0/c s + pi fr^2 / kr))
+ * (compression.cc:17-58); Stolt interpolation (above); 2-D inverse FFT scaled
+ * by 1/(na nr) (fft.cc:200-359).  The reference function is applied as each
+ * line is staged for the Stolt step and the inverse scale as it is written,
+ * so the image makes four passes over HBM besides the transforms.  The 2-D
+ * transforms are hipFFT plans kept per (na, nr) by the context (the first call
+ * of a size builds its plan).  nr <= 8192. */
+sarbp_status sarbp_omega_k(sarbp_context* ctx, const sarbp_cf32* d_echo, int32_t na_in, int32_t nr_in,
+                           const sarbp_wk_params* wp, sarbp_cf32* d_image, void* stream);
 
 /* ---- diagnostics -------------------------------------------------------------- */
 

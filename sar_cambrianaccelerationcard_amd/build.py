@@ -4,7 +4,8 @@
     python -m sar_cambrianaccelerationcard_amd.build sarbp  # one target
 
 Targets (all written to sar_cambrianaccelerationcard_amd/_lib/):
-  libsarbp.so   gfx950 HIP back-projection kernels + C ABI (include/sarbp.h)
+  libsarbp.so   gfx950 HIP back-projection, range-compression and omega-K
+                kernels + C ABI (include/sarbp.h); links hipFFT
   libsarsim.so  FP64 CPU scene simulator (csrc/sarsim.h)
   sarbp_run     C++ command-line driver (mirror of the reference's main.cpp)
 
@@ -38,7 +39,7 @@ HIP_FLAGS = [
 SARBP_SOURCES = ("sarbp_kernels_nearest.hip", "sarbp_kernels_linear.hip", "sarbp_kernels_sinc8.hip",
                  "sarbp_kernels_sinc8w.hip",
                  "sarbp_api.hip", "sarbp_batch.hip", "sarbp_server.hip",
-                 "sarbp_range.hip")
+                 "sarbp_range.hip", "sarbp_wk.hip")
 
 
 def _run(cmd: list[str]) -> None:
@@ -72,7 +73,8 @@ def build_sarbp(force: bool = True) -> str:
         list(ex.map(lambda so: _run([HIPCC, *HIP_FLAGS, "-c", so[0], "-o", so[1]]),
                     zip(srcs, objs)))
     tmp = out + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
+          "-L" + os.path.join(ROCM, "lib"), "-lhipfft", "-Wl,-rpath," + os.path.join(ROCM, "lib")])
     os.replace(tmp, out)
     for o in objs:
         os.remove(o)

@@ -454,6 +454,7 @@ void sarbp_destroy(sarbp_context* c) {
     DeviceGuard dg(c->device);
     for (BatchExec* b : c->batch) delete b;
     range_cache_destroy(c->range);
+    wk_cache_destroy(c->wk);
   }
   delete c;
 }

@@ -94,6 +94,10 @@ class BatchExec {
 struct RangeCache;
 void range_cache_destroy(RangeCache* rc);
 
+// hipFFT plans per (na, nr) and the axis buffers of the omega-K entry points.
+struct WkCache;
+void wk_cache_destroy(WkCache* wc);
+
 }  // namespace sarbp
 
 struct sarbp_context {
@@ -105,4 +109,5 @@ struct sarbp_context {
   // (jobs whose grids need different shapes run as separate launches).
   sarbp::BatchExec* batch[8] = {};
   sarbp::RangeCache* range = nullptr;    // range-compression tables
+  sarbp::WkCache* wk = nullptr;          // omega-K transform plans
 };

@@ -119,3 +119,44 @@ def range_compress(ctx: native.Context, raw, radar: dict, *, dtype: str = "c64",
     if sync:
         torch.cuda.synchronize(dev)
     return out
+
+
+def omega_k(ctx: native.Context, echo, wp: native.WkParams, *, device: int = 0, out=None,
+            sync: bool = True):
+    """omega-K image of raw echoes [na_in][nr_in] (numpy complex or a complex64
+    device tensor); returns the padded complex64 image tensor [na][nr]."""
+    torch = require_gpu()
+    dev = torch.device("cuda", device)
+    if not isinstance(echo, torch.Tensor):
+        echo = torch.from_numpy(np.ascontiguousarray(echo, dtype=np.complex64)).to(dev)
+    if echo.dim() != 2 or echo.dtype != torch.complex64 or not echo.is_contiguous():
+        raise ValueError("echo must be a contiguous [na][nr] complex64 tensor")
+    na, nr = native.wk_padded_size(*echo.shape)
+    if out is None:
+        out = torch.empty((na, nr), dtype=torch.complex64, device=dev)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    ctx.omega_k(echo.data_ptr(), echo.shape[0], echo.shape[1], wp, out.data_ptr(), stream)
+    if sync:
+        torch.cuda.synchronize(dev)
+    return out
+
+
+def stolt_sinc(ctx: native.Context, data, fr_axis, fa_axis, *, f0: float, c: float, vr: float, fs: float,
+               taps: int = 6, window="floor", device: int = 0, out=None, sync: bool = True):
+    """Stolt interpolation of [na][nr] complex data (numpy or a complex64 device
+    tensor; out may be the input tensor) with explicit host axes."""
+    torch = require_gpu()
+    dev = torch.device("cuda", device)
+    if not isinstance(data, torch.Tensor):
+        data = torch.from_numpy(np.ascontiguousarray(data, dtype=np.complex64)).to(dev)
+    if data.dim() != 2 or data.dtype != torch.complex64 or not data.is_contiguous():
+        raise ValueError("data must be a contiguous [na][nr] complex64 tensor")
+    na, nr = data.shape
+    if out is None:
+        out = torch.empty_like(data)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    ctx.stolt_sinc(data.data_ptr(), out.data_ptr(), fr_axis, fa_axis, f0=f0, c=c, vr=vr, na=na, nr=nr,
+                   fs=fs, taps=taps, window=window, stream=stream)
+    if sync:
+        torch.cuda.synchronize(dev)
+    return out

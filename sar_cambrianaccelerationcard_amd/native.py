@@ -132,6 +132,14 @@ class RawRequest(C.Structure):
                 ("grid", Grid), ("params", Params), ("h_image", C.c_void_p)]
 
 
+class WkParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("c", "f0", "fs", "prf", "kr", "vr", "r0", "f_nc")] + \
+        [("taps", C.c_int32), ("window", C.c_int32)]
+
+
+STOLT_FLOOR, STOLT_CENTERED = 0, 1
+STOLT_WINDOWS = {"floor": STOLT_FLOOR, "centered": STOLT_CENTERED}
+
 JOB_QUEUED, JOB_RESIDENT, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = range(6)
 
 
@@ -184,6 +192,12 @@ _SIGS = {
                                    C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "sarbp_range_compress": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(RangeParams),
                                        C.c_void_p, C.c_void_p]),
+    "sarbp_stolt_sinc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32,
+                                   C.c_double, C.c_int32, C.c_int32, C.c_void_p]),
+    "sarbp_wk_padded_size": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "sarbp_omega_k": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.POINTER(WkParams),
+                                C.c_void_p, C.c_void_p]),
     "sarbp_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Grid),
                                  C.POINTER(Params), C.c_void_p, C.c_size_t, C.c_int32, C.c_void_p,
                                  C.POINTER(AutotuneResult)]),
@@ -258,6 +272,25 @@ def range_info(rp: RangeParams) -> tuple:
     return nf.value, r0.value, dr.value
 
 
+def make_wk_params(*, c: float, f0: float, fs: float, prf: float, kr: float, vr: float, r0: float,
+                   f_nc: float, taps: int = 6, window="floor") -> WkParams:
+    """omega-K parameters; kr is the signed chirp rate (negative for a down-chirp)."""
+    wp = WkParams()
+    for k, v in dict(c=c, f0=f0, fs=fs, prf=prf, kr=kr, vr=vr, r0=r0, f_nc=f_nc).items():
+        setattr(wp, k, float(v))
+    wp.taps = int(taps)
+    wp.window = STOLT_WINDOWS[window] if isinstance(window, str) else int(window)
+    return wp
+
+
+def wk_padded_size(na_in: int, nr_in: int) -> tuple:
+    """Padded (na, nr) of an na_in x nr_in echo (host only)."""
+    lib = load()
+    na, nr = C.c_int32(), C.c_int32()
+    check(lib.sarbp_wk_padded_size(int(na_in), int(nr_in), C.byref(na), C.byref(nr)), "sarbp_wk_padded_size")
+    return na.value, nr.value
+
+
 def make_params(*, c: float, fc: float, r0: float, dr: float, n_pulses: int,
                 n_samples: int, interp="linear", precision="ref", dtype="c64",
                 kaiser_beta: float = 6.0, accumulate: bool = False) -> Params:
@@ -303,6 +336,25 @@ class Context:
         check(self.lib.sarbp_range_compress(self.handle, C.c_void_p(d_raw), int(n_pulses),
                                             C.byref(rp), C.c_void_p(d_profiles), C.c_void_p(stream)),
               "sarbp_range_compress")
+
+    def stolt_sinc(self, d_in: int, d_out: int, fr_axis, fa_axis, *, f0: float, c: float, vr: float,
+                   na: int, nr: int, fs: float, taps: int = 6, window="floor", stream: int = 0) -> None:
+        """sarbp_stolt_sinc on device buffers; the axes are host float64 arrays."""
+        import numpy as np
+        fr = np.ascontiguousarray(fr_axis, dtype=np.float64)
+        fa = np.ascontiguousarray(fa_axis, dtype=np.float64)
+        if fr.size != nr or fa.size != na:
+            raise ValueError("axis lengths must be nr and na")
+        w = STOLT_WINDOWS[window] if isinstance(window, str) else int(window)
+        check(self.lib.sarbp_stolt_sinc(self.handle, C.c_void_p(d_in), C.c_void_p(d_out),
+                                        fr.ctypes.data_as(C.c_void_p), fa.ctypes.data_as(C.c_void_p),
+                                        float(f0), float(c), float(vr), int(na), int(nr), float(fs),
+                                        int(taps), w, C.c_void_p(stream)), "sarbp_stolt_sinc")
+
+    def omega_k(self, d_echo: int, na_in: int, nr_in: int, wp: WkParams, d_image: int,
+                stream: int = 0) -> None:
+        check(self.lib.sarbp_omega_k(self.handle, C.c_void_p(d_echo), int(na_in), int(nr_in), C.byref(wp),
+                                     C.c_void_p(d_image), C.c_void_p(stream)), "sarbp_omega_k")
 
     def set_tuning(self, interp: int, precision: int, dtype: int, rows: int, cols: int,
                    pulses: int) -> None:

@@ -1,0 +1,204 @@
+"""omega-K on the device against the FP64 oracle (oracle/wk_oracle.py):
+sarbp_stolt_sinc on random lines (both windows, several tap counts, the
+reference's own 2 x 3 test input, the RADARSAT axes), sarbp_omega_k on
+simulated point scenes, the RADARSAT-1 crop and the full 3072 x 2048 scene
+size.  Tolerances: Stolt alone 3e-6 of the largest output magnitude (FP32 taps
+and weights, FP64 centres); whole pipeline 2e-5 relative L2 (FP32 transforms
+of up to 4096 x 3414 points)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import wk_oracle as W
+from sar_cambrianaccelerationcard_amd import native
+from sar_cambrianaccelerationcard_amd import omegak as O
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _dev(x):
+    torch = _torch()
+    return torch.from_numpy(np.ascontiguousarray(x, dtype=np.complex64)).to("cuda:0")
+
+
+def _stolt(ctx, S, fr, fa, out=None, **kw):
+    from sar_cambrianaccelerationcard_amd import device
+    d = _dev(S)
+    r = device.stolt_sinc(ctx, d, fr, fa, out=(d if out == "inplace" else None), **kw)
+    return r.cpu().numpy()
+
+
+def _rand(shape, seed):
+    r = np.random.default_rng(seed)
+    return (r.standard_normal(shape) + 1j * r.standard_normal(shape)).astype(np.complex64)
+
+
+def _check_stolt(got, ref):
+    err = np.max(np.abs(got - ref))
+    assert err <= 3e-6 * np.max(np.abs(ref)), (err, np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("window", ["floor", "centered"])
+@pytest.mark.parametrize("taps", [2, 6, 16])
+def test_stolt_matches_oracle(ctx, window, taps):
+    na, nr = 37, 300
+    S = _rand((na, nr), 10 + taps)
+    fs, f0, c, vr = 1e6, 1e9, 3e8, 150.0
+    fr = W.fftshift_axis(nr) * fs / nr
+    # Centres move by 0.01 to 21 bins and stay off integers (the floor window's
+    # taps jump where a centre crosses one).
+    fa = np.linspace(-11.3, 11.9, na)
+    kw = dict(f0=f0, c=c, vr=vr, fs=fs, taps=taps)
+    got = _stolt(ctx, S, fr, fa, window=window, **kw)
+    ref = W.stolt_sinc(S, fr, fa, window=W.FLOOR if window == "floor" else W.CENTERED, **kw)
+    _check_stolt(got, ref)
+
+
+def test_stolt_in_place_equals_out_of_place(ctx):
+    S = _rand((16, 1000), 5)
+    fr = W.fftshift_axis(1000) * 2e5
+    fa = np.linspace(-30.0, 40.0, 16)
+    kw = dict(f0=1e9, c=3e8, vr=150.0, fs=2e8, taps=6)
+    a = _stolt(ctx, S, fr, fa, **kw)
+    b = _stolt(ctx, S, fr, fa, out="inplace", **kw)
+    assert np.array_equal(a, b)
+
+
+def test_stolt_reference_test_input(ctx):
+    # wk_v5.1/wk.cc:798-838 calls StoltInterp_sinc on a 2 x 3 all-ones input
+    # with fr_axis {1, 2, 3}, fa_axis {0.1, 0.2}, f0 = 10 GHz, c = 3e8,
+    # Vr = 100, fr = 1 MHz, P = 6.
+    S = np.ones((2, 3), dtype=np.complex64)
+    fr, fa = np.array([1.0, 2.0, 3.0]), np.array([0.1, 0.2])
+    kw = dict(f0=10e9, c=3e8, vr=100.0, fs=1e6, taps=6)
+    got = _stolt(ctx, S, fr, fa, **kw)
+    ref = W.stolt_sinc(S, fr, fa, window=W.FLOOR, **kw)
+    _check_stolt(got, ref)
+
+
+def test_stolt_radarsat_axes(ctx):
+    # The RADARSAT scene's padded size and axes: centres move by up to ~300 bins.
+    na, nr = 4096, 3414
+    rows = np.arange(0, na, 64)
+    fr, fa, _ = W.axes(na, nr, 32.317e6, 1256.98, -6900.0)
+    S = _rand((rows.size, nr), 7)
+    kw = dict(f0=5.3e9, c=299790000.0, vr=7062.0, fs=32.317e6, taps=6)
+    for window in ("floor", "centered"):
+        got = _stolt(ctx, S, fr, fa[rows], window=window, **kw)
+        ref = W.stolt_sinc(S, fr, fa[rows], window=W.FLOOR if window == "floor" else W.CENTERED, **kw)
+        _check_stolt(got, ref)
+
+
+def test_stolt_rejects_bad_arguments(ctx):
+    torch = _torch()
+    d = torch.zeros((2, 8), dtype=torch.complex64, device="cuda:0")
+    fr, fa = np.zeros(8), np.zeros(2)
+    for kw, st in ((dict(taps=5), native.ERR_INVALID_ARGUMENT), (dict(taps=18), native.ERR_INVALID_ARGUMENT),
+                   (dict(window=3), native.ERR_INVALID_ARGUMENT), (dict(fs=0.0), native.ERR_INVALID_ARGUMENT)):
+        args = dict(f0=1e9, c=3e8, vr=100.0, na=2, nr=8, fs=1e6, taps=6, window=0)
+        args.update(kw)
+        with pytest.raises(native.SarbpError) as e:
+            ctx.stolt_sinc(d.data_ptr(), d.data_ptr(), fr, fa, **args)
+        assert e.value.status == st
+    big = np.zeros(9000)
+    with pytest.raises(native.SarbpError) as e:
+        ctx.stolt_sinc(d.data_ptr(), d.data_ptr(), big, fa, f0=1e9, c=3e8, vr=100.0, na=2, nr=9000, fs=1e6)
+    assert e.value.status == native.ERR_UNSUPPORTED
+
+
+def _omega_k(ctx, echo, wp):
+    from sar_cambrianaccelerationcard_amd import device
+    return device.omega_k(ctx, echo, wp).cpu().numpy()
+
+
+def _oracle(echo, wp):
+    return W.omega_k(echo, c=wp.c, f0=wp.f0, fs=wp.fs, prf=wp.prf, kr=wp.kr, vr=wp.vr, r0=wp.r0, f_nc=wp.f_nc,
+                     taps=wp.taps, window=wp.window)
+
+
+def _rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("f_nc", [0.0, 60.0])
+@pytest.mark.parametrize("window", ["floor", "centered"])
+def test_omega_k_point_scene(ctx, f_nc, window):
+    scene = O.WkScene(f_nc=f_nc).resolved()
+    targets = [(194, 170), (294, 235), (394, 300)]
+    echo = O.simulate(scene, targets)
+    wp = scene.params(window=window)
+    got = _omega_k(ctx, echo, wp)
+    ref = _oracle(echo, wp)
+    assert got.shape == ref.shape == scene.padded
+    assert _rel_l2(got, ref) < 2e-5
+    for t in targets:
+        mg, mr = O.peak_metrics(got, *t), O.peak_metrics(ref, *t)
+        assert (mg["d_line"], mg["d_sample"]) == (mr["d_line"], mr["d_sample"]) == (0, 0)
+        assert abs(mg["concentration"] - mr["concentration"]) < 1e-3 and mg["concentration"] > 0.25
+
+
+def test_omega_k_random_echo_odd_sizes(ctx):
+    # Unequal pads, odd input sizes (padded sizes odd too), a 2-tap kernel.
+    echo = _rand((45, 101), 3)
+    scene = O.WkScene(na_in=45, nr_in=101)
+    wp = scene.params(taps=2)
+    got = _omega_k(ctx, echo, wp)
+    ref = _oracle(echo, wp)
+    assert got.shape == W.padded_size(45, 101)[:2]
+    assert _rel_l2(got, ref) < 2e-5
+
+
+def test_omega_k_radarsat_crop(ctx):
+    from sar_cambrianaccelerationcard_amd import radarsat as RS
+    z = np.load(os.path.join(HERE, "golden", "radarsat_cd_crop.npz"))
+    meta = json.loads(str(z["meta"]))
+    rp = RS.RunParams(**meta["run_params"])
+    echo = RS.unpack4(z["codes"])
+    wp = O.radarsat_params(rp)
+    got = _omega_k(ctx, echo, wp)
+    ref = _oracle(echo, wp)
+    assert got.shape == W.padded_size(*echo.shape)[:2]
+    assert _rel_l2(got, ref) < 2e-5
+
+
+def test_omega_k_full_scene_size(ctx):
+    # The reference's scene size (3072 x 2048 -> 4096 x 3414) with its radar
+    # and squint; three point targets, oracle at full size.
+    scene = O.radarsat_scene()
+    targets = [O.aperture_target(scene, a, s) for a, s in ((1500, 1400), (2000, 1550), (2600, 1690))]
+    echo = O.simulate(scene, targets)
+    wp = scene.params()
+    got = _omega_k(ctx, echo, wp)
+    ref = _oracle(echo, wp)
+    assert got.shape == (4096, 3414)
+    assert _rel_l2(got, ref) < 2e-5
+    for t in targets:
+        mg = O.peak_metrics(got, *t)
+        assert (mg["d_line"], mg["d_sample"]) == (0, 0) and mg["concentration"] > 0.4, mg
+
+
+def test_omega_k_rejects_bad_arguments(ctx):
+    torch = _torch()
+    d = torch.zeros((8, 8), dtype=torch.complex64, device="cuda:0")
+    img = torch.zeros((12, 14), dtype=torch.complex64, device="cuda:0")
+    good = O.WkScene(na_in=8, nr_in=8).params()
+    for field, val in (("taps", 7), ("window", 2), ("prf", 0.0), ("kr", 0.0)):
+        wp = O.WkScene(na_in=8, nr_in=8).params()
+        setattr(wp, field, val)
+        with pytest.raises(native.SarbpError) as e:
+            ctx.omega_k(d.data_ptr(), 8, 8, wp, img.data_ptr())
+        assert e.value.status == native.ERR_INVALID_ARGUMENT, field
+    with pytest.raises(native.SarbpError) as e:
+        ctx.omega_k(d.data_ptr(), 8, 6000, good, img.data_ptr())
+    assert e.value.status == native.ERR_UNSUPPORTED
+    with pytest.raises(native.SarbpError):
+        ctx.omega_k(0, 8, 8, good, img.data_ptr())

@@ -1,0 +1,220 @@
+"""Pins the omega-K oracle (oracle/wk_oracle.py) to the reference's own sizes,
+MATLAB's definitions, closed forms of the Stolt step and the focusing of
+simulated point targets; planted mistakes in the oracle must fail the focus
+check.  CPU only."""
+import numpy as np
+import pytest
+
+import wk_oracle as W
+from sar_cambrianaccelerationcard_amd import native
+from sar_cambrianaccelerationcard_amd import omegak as O
+
+
+def test_padded_size_known_answers():
+    # wk.cc:76-78 and preprocess.cc:21-24: the 3072 x 2048 scene pads to 4096 x 3414.
+    assert W.padded_size(3072, 2048) == (4096, 3414, 512, 683)
+    assert W.padded_size(2, 3) == (2, 5, 0, 1)
+    assert W.padded_size(9, 3) == (13, 5, 2, 1)      # 9/6 = 1.5 rounds away from zero
+
+
+def test_padded_size_c_abi_matches():
+    for na, nr in [(0, 0), (1, 1), (3, 5), (9, 3), (384, 384), (3072, 2048), (1000, 777), (5, 4096)]:
+        exp = W.padded_size(na, nr)[:2]
+        assert native.wk_padded_size(na, nr) == exp, (na, nr)
+    with pytest.raises(native.SarbpError):
+        native.wk_padded_size(-1, 4)
+
+
+def test_fftshift_axis_matlab():
+    # MATLAB: fftshift(-2:1) = [0 1 -2 -1]; fftshift(-2.5:1.5) = [0.5 1.5 -2.5 -1.5 -0.5].
+    assert np.array_equal(W.fftshift_axis(4), [0, 1, -2, -1])
+    assert np.allclose(W.fftshift_axis(5), [0.5, 1.5, -2.5, -1.5, -0.5])
+    # For even lengths the C++ loop of wk.cc:199-208 gives the same axis.
+    for n in (2, 6, 3414, 4096):
+        cpp = np.empty(n)
+        for i in range(n):
+            idx = i + n // 2 if i < n // 2 else i - n // 2
+            cpp[i] = idx - n // 2
+        assert np.array_equal(W.fftshift_axis(n), cpp)
+
+
+def _rand(shape, seed):
+    r = np.random.default_rng(seed)
+    return r.standard_normal(shape) + 1j * r.standard_normal(shape)
+
+
+@pytest.mark.parametrize("window", [W.FLOOR, W.CENTERED])
+@pytest.mark.parametrize("taps", [2, 6, 16])
+def test_stolt_identity_at_zero_doppler(window, taps):
+    # fa = 0 makes fr_new = fr: every centre is its own integer index.
+    S = _rand((3, 50), 1)
+    fr = W.fftshift_axis(50) * 1e5
+    out = W.stolt_sinc(S, fr, np.zeros(3), f0=5e9, c=3e8, vr=7000.0, fs=5e6, taps=taps, window=window)
+    # The floor window clamps taps past the row ends onto the end samples, so
+    # only outputs whose taps all lie in the row are plain copies there.
+    inner = slice(taps // 2, 50 - taps // 2) if window == W.FLOOR else slice(None)
+    assert np.max(np.abs(out - S)[:, inner]) < 1e-13
+
+
+@pytest.mark.parametrize("window", [W.FLOOR, W.CENTERED])
+def test_stolt_integer_shift(window):
+    S = _rand((2, 40), 2)
+    n = np.arange(40)
+    for d in (-3, 2, 7):
+        cen = np.broadcast_to(n + d, S.shape).astype(float)
+        out = W.stolt_rows(S, cen, 6, window)
+        exp = np.zeros_like(S)
+        ok = (n + d >= 0) & (n + d < 40)
+        exp[:, ok] = S[:, (n + d)[ok]]
+        if window == W.FLOOR:  # taps clamped past the ends: see above
+            ok = (n + d - 3 >= 0) & (n + d + 2 <= 39)
+            out, exp = out[:, ok], exp[:, ok]
+        assert np.max(np.abs(out - exp)) < 1e-13, d
+
+
+@pytest.mark.parametrize("window", [W.FLOOR, W.CENTERED])
+@pytest.mark.parametrize("f", [0.3, 0.7])
+def test_stolt_interpolates_tone_in_the_right_direction(window, f):
+    # A slowly varying tone is recovered at n + f (truncated-sinc error only),
+    # not at n - f.
+    n = np.arange(200)
+    nu = 0.1
+    S = np.exp(2j * np.pi * nu * n)[None, :]
+    cen = (n + f)[None, :]
+    inner = slice(20, -20)
+    for taps, bound in ((6, 0.06), (16, 0.015)):
+        out = W.stolt_rows(S, cen, taps, window)[0, inner]
+        good = np.max(np.abs(out - np.exp(2j * np.pi * nu * (n[inner] + f))))
+        bad = np.max(np.abs(out - np.exp(2j * np.pi * nu * (n[inner] - f))))
+        assert good < bound and bad > 0.3, (taps, good, bad)
+
+
+def test_centres_match_exact_stolt_inverse_to_first_order():
+    # Exact Stolt: output frequency f' reads input frequency
+    # sqrt((f0 + f')^2 + c^2 fa^2 / (4 vr^2)) - f0.  The reference's centre
+    # n - (fr_new - fr) Nr/fs agrees to first order in c^2 fa^2 / (f0^2 vr^2).
+    f0, c, vr, fs, nr = 5.3e9, 3e8, 7062.0, 32.317e6, 3414
+    fr = W.fftshift_axis(nr) * fs / nr
+    fa = np.array([-7528.0, -6900.0, -6272.0, 0.0, 300.0])
+    cen = W.centres(fr, fa, f0, c, vr, fs)
+    k = c * c / (4 * vr * vr)
+    f_in = np.sqrt((f0 + fr[None, :]) ** 2 + k * fa[:, None] ** 2) - f0
+    exact = np.arange(nr)[None, :] + (f_in - fr[None, :]) * nr / fs
+    shift = exact - np.arange(nr)[None, :]
+    # second-order remainder: (k fa^2)^2 / (f0 + fr)^3 in Hz, in bins
+    bound = (k * fa[:, None] ** 2) ** 2 / (f0 + fr[None, :]) ** 3 * nr / fs + 1e-9
+    assert np.all(np.abs(cen - exact) <= bound)
+    assert np.max(np.abs(shift)) > 200  # the RADARSAT squint moves centres by hundreds of bins
+
+
+def _mlu_dataflow(S, fr_axis, fa_axis, f0, c, vr, fs, P):
+    """stolt.cc:324-727 step by step: delta in bins, position n - delta, P
+    offsets arange(-P/2), floor, clip to [0, Nr-1], sin(pi x)/(pi x)."""
+    na, nr = S.shape
+    out = np.zeros_like(S)
+    offs = -P / 2 + np.arange(P)
+    for m in range(na):
+        for n in range(nr):
+            fr_new = np.sqrt((f0 + fr_axis[n]) ** 2 - c * c * fa_axis[m] ** 2 / (4 * vr * vr)) - f0
+            pos = n - (fr_new * nr / fs - fr_axis[n] * nr / fs)
+            for p in offs:
+                idx = min(max(np.floor(pos + p), 0), nr - 1)
+                x = np.pi * (pos - idx)
+                out[m, n] += S[m, int(idx)] * (np.sin(x) / x if x != 0 else 1.0)
+    return out
+
+
+def _matlab_loop(S, fr_axis, fa_axis, f0, c, vr, fs, P):
+    """mian_wka_sinc.m:91-113 with 1-based n and idx."""
+    na, nr = S.shape
+    out = np.zeros_like(S)
+    gap = fs / nr
+    for m in range(na):
+        fr_new = np.sqrt((f0 + fr_axis) ** 2 - c * c / 4 / vr ** 2 * fa_axis[m] ** 2) - f0
+        delta = (fr_new - fr_axis) / gap
+        for n in range(1, nr + 1):
+            ci = n - delta[n - 1]
+            for idx in range(max(1, int(np.floor(ci - P / 2))), min(nr, int(np.ceil(ci + P / 2))) + 1):
+                x = ci - idx
+                if abs(x) <= P / 2:
+                    out[m, n - 1] += S[m, idx - 1] * np.sinc(x)
+    return out
+
+
+def test_oracle_matches_reference_loops_on_small_input():
+    S = _rand((4, 23), 3)
+    fs, f0, c, vr = 1e6, 1e9, 3e8, 150.0
+    fr = W.fftshift_axis(23) * fs / 23
+    fa = np.array([-17.0, -6.6, 9.5, 21.0])  # centres moved by 0.3 to 6 bins
+    shift = W.centres(fr, fa, f0, c, vr, fs) - np.arange(23)
+    assert 0.2 < shift.min() and shift.max() < 8 and np.ptp(shift) > 3
+    got_f = W.stolt_sinc(S, fr, fa, f0=f0, c=c, vr=vr, fs=fs, taps=6, window=W.FLOOR)
+    assert np.max(np.abs(got_f - _mlu_dataflow(S, fr, fa, f0, c, vr, fs, 6))) < 1e-12
+    got_c = W.stolt_sinc(S, fr, fa, f0=f0, c=c, vr=vr, fs=fs, taps=6, window=W.CENTERED)
+    assert np.max(np.abs(got_c - _matlab_loop(S, fr, fa, f0, c, vr, fs, 6))) < 1e-12
+    assert np.max(np.abs(got_c - got_f)) > 1e-3  # the two windows differ
+
+
+# ---- focusing of simulated point targets -------------------------------------------
+
+TARGETS = [(194, 170), (294, 235), (394, 300)]   # padded (line, sample); samples < Nr/2
+
+
+def _image(scene, targets=TARGETS, **over):
+    s = scene.resolved()
+    echo = O.simulate(s, targets)
+    wp = dict(c=s.c, f0=s.f0, fs=s.fs, prf=s.prf, kr=s.kr, vr=s.vr, r0=s.r0, f_nc=s.f_nc)
+    wp.update(over)
+    return W.omega_k(echo, **wp)
+
+
+def _focused(img, targets=TARGETS, min_conc=0.25):
+    ms = [O.peak_metrics(img, *t) for t in targets]
+    return all(m["d_line"] == 0 and m["d_sample"] == 0 and m["concentration"] >= min_conc for m in ms), ms
+
+
+@pytest.mark.parametrize("f_nc", [0.0, 60.0])
+@pytest.mark.parametrize("window", [W.FLOOR, W.CENTERED])
+def test_point_targets_focus_where_geometry_puts_them(f_nc, window):
+    ok, ms = _focused(_image(O.WkScene(f_nc=f_nc), window=window))
+    assert ok, ms
+
+
+def test_planted_mistakes_fail_the_focus_check(monkeypatch):
+    scene = O.WkScene(f_nc=60.0)
+    s = scene.resolved()
+    c0, p0, r0, st0 = W.centres, W.pad_downconvert, W.reference_phase, W.stolt_sinc
+    planted = {
+        "chirp sign": lambda: _image(scene, kr=-s.kr),
+        "no Stolt": lambda: (monkeypatch.setattr(W, "stolt_sinc", lambda S, *a, **k: S), _image(scene))[1],
+        "Stolt direction": lambda: (monkeypatch.setattr(
+            W, "centres", lambda fr, fa, *a: 2 * np.arange(fr.size)[None, :] - c0(fr, fa, *a)), _image(scene))[1],
+        "azimuth axis without f_nc": lambda: (monkeypatch.setattr(
+            W, "centres", lambda fr, fa, *a: c0(fr, fa - s.f_nc, *a)), _image(scene))[1],
+        "down-conversion sign": lambda: (monkeypatch.setattr(
+            W, "pad_downconvert", lambda e, prf, f_nc: p0(e, prf=prf, f_nc=-f_nc)), _image(scene))[1],
+        "reference range": lambda: (monkeypatch.setattr(
+            W, "reference_phase", lambda fr, fa, **k: r0(fr, fa, **{**k, "r0": k["r0"] * 1.001})), _image(scene))[1],
+        "reference velocity": lambda: (monkeypatch.setattr(
+            W, "reference_phase", lambda fr, fa, **k: r0(fr, fa, **{**k, "vr": k["vr"] * 1.05})), _image(scene))[1],
+    }
+    for name, make in planted.items():
+        img = make()
+        monkeypatch.setattr(W, "centres", c0)
+        monkeypatch.setattr(W, "pad_downconvert", p0)
+        monkeypatch.setattr(W, "reference_phase", r0)
+        monkeypatch.setattr(W, "stolt_sinc", st0)
+        ok, ms = _focused(img)
+        assert not ok, (name, ms)
+
+
+def test_far_half_of_the_range_window_defocuses():
+    # The Stolt sinc runs along range frequency, so it treats range-time content
+    # beyond Nr/2 samples from sample 0 as negative time: a target there gains
+    # a phase that depends on the fractional shift and defocuses.  The oracle
+    # reproduces this property of the reference algorithm.
+    scene = O.WkScene()
+    _, nr = scene.padded
+    far = [(294, nr // 2 + 60)]
+    ok, ms = _focused(_image(scene, far), far)
+    assert not ok and ms[0]["concentration"] < 0.15, ms
