@@ -153,3 +153,39 @@ def peak_metrics(image: np.ndarray, line: float, sample: float, search: int = 3,
     dl, ds = divmod(k, sub.shape[1])
     return {"d_line": dl - search, "d_sample": ds - search, "peak": float(np.sqrt(sub.max())),
             "concentration": float(sub.max() / win.sum())}
+
+
+def bp_image(ctx, scene: WkScene, echo, *, interp: str = "sinc8", upsample: int = 4,
+             lines: tuple | None = None, samples: tuple | None = None) -> np.ndarray:
+    """Back-projection of the same raw echo onto the omega-K image grid, for
+    comparing the two algorithms: pixel (line, sample) lies at along-track
+    vr * line / prf and slant range r0 + c/2 * sample / fs in the slant plane
+    of a straight track (the simulator's geometry).  Range compression and BP
+    run on the device.  Returns |lines| x |samples| complex64, rows = lines."""
+    import torch
+
+    from . import device as D
+    s = scene.resolved()
+    pa, pr = s.pads
+    na, nr = s.padded
+    l0, l1 = lines or (0, na)
+    s0, s1 = samples or (0, nr)
+    # The simulated chirp is centred on its delay; the matched filter puts the
+    # peak at the chirp's start, half a pulse earlier.
+    t_raw0 = 2.0 * s.r0 / s.c + pr / s.fs + 0.5 * s.tr
+    radar = dict(c=s.c, bandwidth=s.kr * s.tr, pulse_width=s.tr, fs=s.fs, t0=t_raw0, n_raw=s.nr_in,
+                 upsample=upsample, crop_offset=0, n_out=upsample * s.nr_in)
+    prof = D.range_compress(ctx, np.ascontiguousarray(echo, dtype=np.complex64), radar)
+    _, r0, dr = native.range_info(D.range_params(radar))
+    pos = np.zeros((s.na_in, 3))
+    pos[:, 0] = s.vr * (np.arange(s.na_in) + pa) / s.prf
+    dx, dy = s.vr / s.prf, 0.5 * s.c / s.fs
+    grid = native.make_grid((l0 * dx, s.r0 + s0 * dy, 0.0), (dx, 0.0, 0.0), (0.0, dy, 0.0), l1 - l0, s1 - s0)
+    params = native.make_params(c=s.c, fc=s.f0, r0=r0, dr=dr, n_pulses=s.na_in, n_samples=upsample * s.nr_in,
+                                interp=interp)
+    dpos = torch.from_numpy(pos).to(prof.device)
+    img = torch.zeros((s1 - s0, l1 - l0), dtype=torch.complex64, device=prof.device)
+    stream = torch.cuda.current_stream(prof.device).cuda_stream
+    ctx.backproject(prof.data_ptr(), dpos.data_ptr(), grid, params, img.data_ptr(), stream)
+    torch.cuda.synchronize(prof.device)
+    return img.cpu().numpy().T.copy()

@@ -202,3 +202,26 @@ def test_omega_k_rejects_bad_arguments(ctx):
     assert e.value.status == native.ERR_UNSUPPORTED
     with pytest.raises(native.SarbpError):
         ctx.omega_k(0, 8, 8, good, img.data_ptr())
+
+
+@pytest.mark.parametrize("f_nc", [0.0, 60.0])
+def test_bp_and_omega_k_image_the_same_scene(ctx, f_nc):
+    # The two imaging algorithms on the same echoes: BP (exact geometry, device
+    # range compression) and omega-K put every target at the same line and
+    # sample; BP also focuses a target in the far half of the range window,
+    # where the reference's omega-K does not.
+    scene = O.WkScene(f_nc=f_nc).resolved()
+    _, nr = scene.padded
+    near = [(194, 170), (294, 235), (394, 300)]
+    far = (244, nr // 2 + 60)
+    echo = O.simulate(scene, near + [far])
+    wk = _omega_k(ctx, echo, scene.params())
+    bp = O.bp_image(ctx, scene, echo)
+    assert bp.shape == wk.shape
+    for t in near:
+        mw, mb = O.peak_metrics(wk, *t), O.peak_metrics(bp, *t)
+        assert (mw["d_line"], mw["d_sample"]) == (mb["d_line"], mb["d_sample"]) == (0, 0), (t, mw, mb)
+        assert mw["concentration"] > 0.25 and mb["concentration"] > 0.25, (t, mw, mb)
+    mw, mb = O.peak_metrics(wk, *far), O.peak_metrics(bp, *far)
+    assert (mb["d_line"], mb["d_sample"]) == (0, 0) and mb["concentration"] > 0.25, mb
+    assert mw["concentration"] < 0.15, mw
