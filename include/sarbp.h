@@ -460,7 +460,10 @@ typedef enum sarbp_stolt_window {
  * declared task/task.h:37-44): h_fr_axis [nr] and h_fa_axis [na] are the
  * range- and azimuth-frequency axes (host, Hz), fs the range sampling rate
  * (its argument `fr`), taps = P (even, 2..16), window a sarbp_stolt_window;
- * nr <= 8192.  The axes are copied to the device before the call returns.
+ * nr <= 8192.  The axes are staged before the call returns (the host arrays
+ * may then be reused); a call whose axes equal the previous call's on this
+ * context only enqueues the kernel, any other first waits for the previous
+ * call's kernel to finish.
  * One workgroup per azimuth line: the line is staged in LDS, the mapping is
  * evaluated in FP64, the taps in FP32. */
 sarbp_status sarbp_stolt_sinc(sarbp_context* ctx, const sarbp_cf32* d_in, sarbp_cf32* d_out,

@@ -8,6 +8,7 @@ Targets (all written to sar_cambrianaccelerationcard_amd/_lib/):
                 kernels + C ABI (include/sarbp.h); links hipFFT
   libsarsim.so  FP64 CPU scene simulator (csrc/sarsim.h)
   sarbp_run     C++ command-line driver (mirror of the reference's main.cpp)
+  sarwk_run     C++ omega-K command-line imager (mirror of wk.cc's process_wk)
 
 Every HIP file is compiled with packed FP32 instructions disabled
 (-target-feature -packed-fp32-ops) for the device.
@@ -111,8 +112,24 @@ def build_run(force: bool = True) -> str:
     return out
 
 
+def build_wk_run(force: bool = True) -> str:
+    """The sarwk_run omega-K command-line imager (host code only)."""
+    out = os.path.join(LIB, "sarwk_run")
+    src = os.path.join(CSRC, "sarwk_run.cpp")
+    deps = [src, os.path.join(INCLUDE, "sarbp.h"), os.path.join(LIB, "libsarbp.so")]
+    if not force and _newer(out, deps):
+        return out
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__",
+          "-I" + os.path.join(ROCM, "include"), "-I" + INCLUDE, src, "-o", tmp,
+          "-L" + LIB, "-lsarbp", "-L" + os.path.join(ROCM, "lib"), "-lamdhip64",
+          "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(ROCM, "lib")])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all(force: bool = True) -> list[str]:
-    return [build_sarbp(force), build_sarsim(force), build_run(force)]
+    return [build_sarbp(force), build_sarsim(force), build_run(force), build_wk_run(force)]
 
 
 if __name__ == "__main__":
@@ -124,5 +141,8 @@ if __name__ == "__main__":
             print(build_sarbp())
         elif w == "sarsim":
             print(build_sarsim())
+        elif w == "run":
+            print(build_run())
+            print(build_wk_run())
         else:
             raise SystemExit(f"unknown target {w}")

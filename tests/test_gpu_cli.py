@@ -74,3 +74,61 @@ def test_sarbp_run_rejects_bad_options(tmp_path):
     r = subprocess.run([EXE, "--interp", "cubic", "--out", str(tmp_path / "x.bin")], capture_output=True, text=True,
                        timeout=60)
     assert r.returncode == 1 and "bad interp" in r.stderr
+
+
+WK_EXE = os.path.join(ROOT, "sar_cambrianaccelerationcard_amd", "_lib", "sarwk_run")
+
+
+def _write_paras(path, s):
+    # the key: value format of the reference's data/paras.txt (read.cc:54-82)
+    with open(path, "w") as f:
+        f.write(f"Fr: {s.fs:.6f}\nPRF: {s.prf:.6f}\nf0: {s.f0:.6f}\nTr: {s.tr:.9f}\nR0: {s.r0:.9f}\n"
+                f"Kr: {s.kr:.6f}\nc: {s.c:.6f}\n")
+
+
+@pytest.mark.parametrize("fmt", ["txt", "bin"])
+def test_sarwk_run_matches_oracle(tmp_path, fmt):
+    import wk_oracle as W
+    from sar_cambrianaccelerationcard_amd import omegak as OK
+    scene = OK.WkScene(f_nc=60.0).resolved()
+    targets = [(194, 170), (294, 235), (394, 300)]
+    echo = OK.simulate(scene, targets)
+    paras, ech = str(tmp_path / "paras.txt"), str(tmp_path / f"echo.{fmt}")
+    _write_paras(paras, scene)
+    if fmt == "txt":
+        np.savetxt(ech, echo.view(np.float32), fmt="%.9e")  # "re im re im ..." per line
+    else:
+        echo.tofile(ech)
+    out = str(tmp_path / f"result.{fmt}")
+    print(f"[cli] sarwk_run {echo.shape} {fmt}", flush=True)
+    r = subprocess.run([WK_EXE, "--params", paras, "--echo", ech, "--na", str(scene.na_in), "--nr",
+                        str(scene.nr_in), "--vr", repr(scene.vr), "--f-nc", repr(scene.f_nc), "-i", "-o", out],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0 and "first 5 x 5 values" in r.stdout
+    na, nr = scene.padded
+    if fmt == "txt":
+        img = np.loadtxt(out, dtype=np.float32).view(np.complex64)
+    else:
+        img = io.load_image(out, verify=False)
+    assert img.shape == (na, nr)
+    # the parameter file carries 6 decimals, as the reference's paras.txt
+    p = {k: float(v) for k, v in (l.split(":") for l in open(paras))}
+    ref = W.omega_k(echo.astype(np.complex128), c=p["c"], f0=p["f0"], fs=p["Fr"], prf=p["PRF"], kr=p["Kr"],
+                    vr=scene.vr, r0=p["R0"], f_nc=scene.f_nc)
+    err = M.rel_l2(img, ref)
+    print(f"  rel_l2 vs omega-K oracle {err:.3e}", flush=True)
+    assert err <= 2e-5
+
+
+def test_sarwk_run_rejects_bad_input(tmp_path):
+    paras = str(tmp_path / "paras.txt")
+    from sar_cambrianaccelerationcard_amd import omegak as OK
+    _write_paras(paras, OK.WkScene().resolved())
+    ech = str(tmp_path / "echo.bin")
+    np.zeros((4, 5), dtype=np.complex64).tofile(ech)
+    r = subprocess.run([WK_EXE, "--params", paras, "--echo", ech, "--na", "4", "--nr", "6"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 1 and "expected 24" in r.stderr
+    r = subprocess.run([WK_EXE, "--window", "hann"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "--window" in r.stderr
