@@ -68,8 +68,69 @@ __device__ __forceinline__ float sinc_at(float x, float sin_pi_x) {
   return x == 0.0f ? 1.0f : sin_pi_x * __builtin_amdgcn_rcpf(3.14159265358979f * x);
 }
 
-// K = samples per thread (nr <= K * kWkThreads).
-template <int K>
+// Sum over the taps of one output at centre c0 + f (f in [0, 1)) for any tap
+// count, with per-tap bounds handling: the floor window clamps, the centred
+// window skips taps outside the line.
+__device__ __forceinline__ float2 stolt_taps_general(const float2* line, int nr, int c0, float f, float sf,
+                                                     int half, int window) {
+  float2 acc = make_float2(0.0f, 0.0f);
+  if (window == SARBP_STOLT_FLOOR) {
+    for (int p = -half; p < half; ++p) {
+      int idx = c0 + p;
+      float w;
+      if ((unsigned)idx < (unsigned)nr) {
+        const float x = f - (float)p;
+        w = sinc_at(x, (p & 1) ? -sf : sf);
+      } else {
+        idx = idx < 0 ? 0 : nr - 1;
+        const float x = (float)(c0 - idx) + f;
+        w = sinc_at(x, sinpif(x));
+      }
+      const float2 v = line[idx];
+      acc.x = fmaf(w, v.x, acc.x);
+      acc.y = fmaf(w, v.y, acc.y);
+    }
+  } else {
+    // Integers q with |f - q| <= half: q = -half .. half when f == 0, else
+    // -half+1 .. half (the q = -half tap then lies beyond half).
+    for (int q = -half; q <= half; ++q) {
+      const int idx = c0 + q;
+      const float x = f - (float)q;
+      if (fabsf(x) > (float)half || (unsigned)idx >= (unsigned)nr) continue;
+      const float w = sinc_at(x, (q & 1) ? -sf : sf);
+      const float2 v = line[idx];
+      acc.x = fmaf(w, v.x, acc.x);
+      acc.y = fmaf(w, v.y, acc.y);
+    }
+  }
+  return acc;
+}
+
+// The same sum for P taps at offsets q0 .. q0+P-1 that all lie inside the
+// line, 0 < f < 1 (no tap at distance 0): weights (-1)^q sin(pi f) / (pi (f - q)),
+// reciprocals taken in pairs (1/a = b/(ab), 1/b = a/(ab)).
+template <int P>
+__device__ __forceinline__ float2 stolt_taps_inner(const float2* line, int c0, float f, float sf, int q0) {
+  float d[P];
+#pragma unroll
+  for (int t = 0; t < P; ++t) d[t] = 3.14159265358979f * (f - (float)(q0 + t));
+  float2 acc = make_float2(0.0f, 0.0f);
+#pragma unroll
+  for (int t = 0; t < P; t += 2) {
+    const float r = __builtin_amdgcn_rcpf(d[t] * d[t + 1]);
+    const float s0 = ((q0 + t) & 1) ? -sf : sf;
+    const float w0 = s0 * d[t + 1] * r, w1 = -s0 * d[t] * r;
+    const float2 v0 = line[c0 + q0 + t], v1 = line[c0 + q0 + t + 1];
+    acc.x = fmaf(w0, v0.x, acc.x);
+    acc.y = fmaf(w0, v0.y, acc.y);
+    acc.x = fmaf(w1, v1.x, acc.x);
+    acc.y = fmaf(w1, v1.y, acc.y);
+  }
+  return acc;
+}
+
+// K = samples per thread (nr <= K * kWkThreads); P = taps (0: any, runtime).
+template <int K, int P>
 __global__ __launch_bounds__(kWkThreads) void wk_stolt(StoltArgs a) {
   extern __shared__ float2 line[];
   const int m = blockIdx.x;
@@ -91,10 +152,11 @@ __global__ __launch_bounds__(kWkThreads) void wk_stolt(StoltArgs a) {
       const double f = a.f0 + fr;
       const double s = sqrt(fmax(fma(f, f, -kfa2), 0.0));
       if (a.rfm) {
+        // Phase in revolutions, reduced to [-1/2, 1/2]: the hardware sine and
+        // cosine take revolutions (absolute error ~1.2e-7 there).
         const double rev = fma(a.rfm_a, s, a.rfm_b * fr * fr);
         const float ph = (float)(rev - rint(rev));
-        float sn, cs;
-        sincospif(2.0f * ph, &sn, &cs);
+        const float sn = __builtin_amdgcn_sinf(ph), cs = __builtin_amdgcn_cosf(ph);
         v = make_float2(v.x * cs - v.y * sn, v.x * sn + v.y * cs);
       }
       line[j] = v;
@@ -106,7 +168,10 @@ __global__ __launch_bounds__(kWkThreads) void wk_stolt(StoltArgs a) {
   }
   __syncthreads();
 
-  const int half = a.taps >> 1;
+  const int half = P > 0 ? P / 2 : a.taps >> 1;
+  // First tap offset and the range of c0 for which every tap is inside.
+  const int q0 = a.window == SARBP_STOLT_FLOOR ? -half : -half + 1;
+  const int c_lo = -q0, c_hi = nr - 1 - (q0 + 2 * half - 1);
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int j = threadIdx.x + k * kWkThreads;
@@ -115,36 +180,11 @@ __global__ __launch_bounds__(kWkThreads) void wk_stolt(StoltArgs a) {
     const float f = fr_off[k];
     // sin(pi (f - p)) = (-1)^p sin(pi f) for integer p.
     const float sf = sinpif(f);
-    float2 acc = make_float2(0.0f, 0.0f);
-    if (a.window == SARBP_STOLT_FLOOR) {
-      for (int p = -half; p < half; ++p) {
-        int idx = c0 + p;
-        float w;
-        if ((unsigned)idx < (unsigned)nr) {
-          const float x = f - (float)p;
-          w = sinc_at(x, (p & 1) ? -sf : sf);
-        } else {
-          idx = idx < 0 ? 0 : nr - 1;
-          const float x = (float)(c0 - idx) + f;
-          w = sinc_at(x, sinpif(x));
-        }
-        const float2 v = line[idx];
-        acc.x = fmaf(w, v.x, acc.x);
-        acc.y = fmaf(w, v.y, acc.y);
-      }
-    } else {
-      // Integers q with |f - q| <= half: q = -half .. half when f == 0, else
-      // -half+1 .. half (the q = -half tap then lies beyond half).
-      for (int q = -half; q <= half; ++q) {
-        const int idx = c0 + q;
-        const float x = f - (float)q;
-        if (fabsf(x) > (float)half || (unsigned)idx >= (unsigned)nr) continue;
-        const float w = sinc_at(x, (q & 1) ? -sf : sf);
-        const float2 v = line[idx];
-        acc.x = fmaf(w, v.x, acc.x);
-        acc.y = fmaf(w, v.y, acc.y);
-      }
-    }
+    float2 acc;
+    if (P > 0 && f > 0.0f && c0 >= c_lo && c0 <= c_hi)
+      acc = stolt_taps_inner<(P > 0 ? P : 2)>(line, c0, f, sf, q0);
+    else
+      acc = stolt_taps_general(line, nr, c0, f, sf, half, a.window);
     a.out[base + j] = make_float2(acc.x * a.scale, acc.y * a.scale);
   }
 }
@@ -180,18 +220,28 @@ __global__ __launch_bounds__(kWkThreads) void wk_prep(PrepArgs a) {
   }
 }
 
-hipError_t launch_stolt(const StoltArgs& a, hipStream_t s) {
+template <int P>
+void launch_stolt_p(const StoltArgs& a, hipStream_t s) {
   const int k = (a.nr + kWkThreads - 1) / kWkThreads;
   const size_t lds = (size_t)a.nr * sizeof(float2);
   const dim3 grid((unsigned)a.na), block(kWkThreads);
   if (k <= 4)
-    hipLaunchKernelGGL(wk_stolt<4>, grid, block, lds, s, a);
+    hipLaunchKernelGGL((wk_stolt<4, P>), grid, block, lds, s, a);
   else if (k <= 8)
-    hipLaunchKernelGGL(wk_stolt<8>, grid, block, lds, s, a);
+    hipLaunchKernelGGL((wk_stolt<8, P>), grid, block, lds, s, a);
   else if (k <= 16)
-    hipLaunchKernelGGL(wk_stolt<16>, grid, block, lds, s, a);
+    hipLaunchKernelGGL((wk_stolt<16, P>), grid, block, lds, s, a);
   else
-    hipLaunchKernelGGL(wk_stolt<32>, grid, block, lds, s, a);
+    hipLaunchKernelGGL((wk_stolt<32, P>), grid, block, lds, s, a);
+}
+
+// The reference's 6 taps get an unrolled kernel; other tap counts the
+// generic one.
+hipError_t launch_stolt(const StoltArgs& a, hipStream_t s) {
+  if (a.taps == 6)
+    launch_stolt_p<6>(a, s);
+  else
+    launch_stolt_p<0>(a, s);
   return hipGetLastError();
 }
 
