@@ -3,9 +3,9 @@
 // :106-111), the step that turns raw LFM echoes into the profiles BP reads.
 //
 // One workgroup per pulse (256 threads, up to 1024 for the largest FFTs);
-// every transform runs in LDS (nf complex64 words,
-// nf <= 8192 -> at most 64 KiB), so a pulse costs one HBM read of its raw
-// echo and one write of its output window.
+// every transform runs in LDS (nf complex64 words plus padding, at most
+// 64 KiB for nf = 8192), so a pulse costs one HBM read of its raw echo and
+// one write of its output window.
 //
 // The L-times upsampled inverse transform is split into L polyphase inverse
 // FFTs of size nf instead of one of size L*nf: with k~ the signed bin of the
@@ -13,8 +13,9 @@
 //   x[L q + r] = sum_k Y[k] e^{j 2 pi k~ (L q + r) / (L nf)}
 //              = IFFT_nf( Y[k] e^{j 2 pi k~ r / (L nf)} )[q],
 // so phase r is an nf-point inverse FFT of the spectrum times a ramp.  The
-// FFTs are radix-4 Stockham passes (a radix-2 pass first when log2 nf is
-// odd), in place in LDS: every thread reads its butterfly inputs into
+// FFTs are radix-8 Stockham passes (a radix-2 or radix-4 pass first when
+// log2 nf is not a multiple of 3), in place in LDS with one pad word per 16
+// (bank-conflict-free strides): every thread reads its butterfly inputs into
 // registers, the workgroup synchronises, then writes the outputs.
 //
 // Twiddles, the matched filter conj(FFT(h))/nf and the ramp are FP32 tables
@@ -49,82 +50,139 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 
-// In-place FFT of NF = 2^LG points held in LDS.  tw[k] = exp(-j 2 pi k / NF);
-// INV uses the conjugate twiddles (unscaled inverse).  Ends with a barrier.
+// LDS slot of element i: one pad word per 16 for transforms up to 4096
+// points (the 8192-point buffer has to fit 64 KiB unpadded).  The padding
+// turns the power-of-two strides of the Stockham writes into odd ones, so
+// the 16 lanes of an LDS access fall into distinct banks.
+template <int LG>
+__device__ __forceinline__ int lp(int i) {
+  if constexpr (LG <= 12)
+    return i + (i >> 4);
+  else
+    return i;
+}
+template <int LG>
+constexpr int lds_words() { return LG <= 12 ? (1 << LG) + (1 << LG) / 16 : 1 << LG; }
+
+// R-point DFT of x in registers, in place, natural output order.  INV: the
+// conjugate transform (unscaled inverse).
+template <int R, bool INV>
+__device__ __forceinline__ void dft_r(float2* x) {
+  // multiply by -j (forward) / +j (inverse)
+  auto mj = [](float2 v) { return INV ? make_float2(-v.y, v.x) : make_float2(v.y, -v.x); };
+  if constexpr (R == 2) {
+    const float2 a = x[0], b = x[1];
+    x[0] = cadd(a, b);
+    x[1] = csub(a, b);
+  } else if constexpr (R == 4) {
+    const float2 t0 = cadd(x[0], x[2]), t1 = csub(x[0], x[2]), t2 = cadd(x[1], x[3]), t3 = mj(csub(x[1], x[3]));
+    x[0] = cadd(t0, t2);
+    x[1] = cadd(t1, t3);
+    x[2] = csub(t0, t2);
+    x[3] = csub(t1, t3);
+  } else {
+    // 8 = 4 x 2: u_k = x_k + x_{k+4}, v_k = W8^k (x_k - x_{k+4}); the even
+    // outputs are DFT4(u), the odd ones DFT4(v).
+    constexpr float s = 0.70710678118654752f;
+    float2 u[4], v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      u[k] = cadd(x[k], x[k + 4]);
+      v[k] = csub(x[k], x[k + 4]);
+    }
+    // W8 = (1 - j)/sqrt2 forward, (1 + j)/sqrt2 inverse; W8^2 = -j / +j.
+    v[1] = INV ? make_float2(s * (v[1].x - v[1].y), s * (v[1].x + v[1].y))
+               : make_float2(s * (v[1].x + v[1].y), s * (v[1].y - v[1].x));
+    v[2] = mj(v[2]);
+    v[3] = INV ? make_float2(-s * (v[3].x + v[3].y), s * (v[3].x - v[3].y))
+               : make_float2(s * (v[3].y - v[3].x), -s * (v[3].x + v[3].y));
+    dft_r<4, INV>(u);
+    dft_r<4, INV>(v);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      x[2 * k] = u[k];
+      x[2 * k + 1] = v[k];
+    }
+  }
+}
+
+// One radix-R Stockham pass over NF = 2^LG points in LDS after sub-transforms
+// of size ns = 2^lns: butterfly b (k = b mod ns) reads b + r NF/R, multiplies
+// input r by W^{r k} of size R ns (tw[t] = exp(-j 2 pi t / NF); powers by
+// repeated products), and writes output r to (b - k) R + k + r ns.  All reads
+// complete (barrier) before any write.
+template <int LG, int R, bool INV>
+__device__ __forceinline__ void stockham_pass(float2* buf, const float2* __restrict__ tw, int lns) {
+  constexpr int NF = 1 << LG;
+  constexpr int kT = rc_threads(LG);
+  constexpr int NB = NF / R;
+  constexpr int PER = (NB + kT - 1) / kT;
+  constexpr int LGR = R == 8 ? 3 : (R == 4 ? 2 : 1);
+  const int tid = threadIdx.x;
+  const int ns = 1 << lns;
+  float2 y[PER][R];
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int b = tid + p * kT;
+    if (NB % kT == 0 || b < NB) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) y[p][r] = buf[lp<LG>(b + r * NB)];
+      if (lns > 0) {
+        const int k = b & (ns - 1);
+        float2 w[R];
+        w[1] = tw[k << (LG - lns - LGR)];
+        if (INV) w[1].y = -w[1].y;
+        // Powers by at most three products: w2 = w1^2, w3 = w2 w1, w4 = w2^2,
+        // w5..w7 = w4 w1..w3.
+        if constexpr (R >= 4) {
+          w[2] = cmul(w[1], w[1]);
+          w[3] = cmul(w[2], w[1]);
+        }
+        if constexpr (R == 8) {
+          w[4] = cmul(w[2], w[2]);
+          w[5] = cmul(w[4], w[1]);
+          w[6] = cmul(w[4], w[2]);
+          w[7] = cmul(w[4], w[3]);
+        }
+#pragma unroll
+        for (int r = 1; r < R; ++r) y[p][r] = cmul(y[p][r], w[r]);
+      }
+      dft_r<R, INV>(y[p]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int b = tid + p * kT;
+    if (NB % kT == 0 || b < NB) {
+      const int k = b & (ns - 1);
+      const int dst = ((b - k) << LGR) + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[lp<LG>(dst + r * ns)] = y[p][r];
+    }
+  }
+  __syncthreads();
+}
+
+// In-place FFT of NF = 2^LG points held in LDS at slots lp<LG>(i): a radix
+// 2 or 4 pass when LG is not a multiple of 3, then radix-8 passes.  INV uses
+// the conjugate twiddles (unscaled inverse).  Ends with a barrier.
 template <int LG, bool INV>
 __device__ __forceinline__ void fft_lds(float2* buf, const float2* __restrict__ tw) {
-  constexpr int NF = 1 << LG;
-  constexpr int kRcThreads = rc_threads(LG);
-  const int tid = threadIdx.x;
-  if constexpr (LG & 1) {
-    constexpr int NB = NF / 2;
-    constexpr int PER = (NB + kRcThreads - 1) / kRcThreads;
-    float2 y0[PER], y1[PER];
-#pragma unroll
-    for (int b = 0; b < PER; ++b) {
-      const int j = tid + b * kRcThreads;
-      if (NB % kRcThreads == 0 || j < NB) {
-        const float2 v0 = buf[j], v1 = buf[j + NB];
-        y0[b] = cadd(v0, v1);
-        y1[b] = csub(v0, v1);
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int b = 0; b < PER; ++b) {
-      const int j = tid + b * kRcThreads;
-      if (NB % kRcThreads == 0 || j < NB) {
-        buf[2 * j] = y0[b];
-        buf[2 * j + 1] = y1[b];
-      }
-    }
-    __syncthreads();
+  int lns = 0;
+  if constexpr (LG % 3 == 1) {
+    stockham_pass<LG, 2, INV>(buf, tw, 0);
+    lns = 1;
+  } else if constexpr (LG % 3 == 2) {
+    stockham_pass<LG, 4, INV>(buf, tw, 0);
+    lns = 2;
   }
-  constexpr int NB = NF / 4;
-  constexpr int PER = NB > 0 ? (NB + kRcThreads - 1) / kRcThreads : 0;
+  if constexpr (LG >= 3) {
 #pragma unroll
-  for (int pass = 0; pass < LG / 2; ++pass) {
-    const int lns = (LG & 1) + 2 * pass;  // log2 of the sub-transform size
-    const int ns = 1 << lns;
-    float2 y[PER > 0 ? PER : 1][4];
-#pragma unroll
-    for (int b = 0; b < PER; ++b) {
-      const int j = tid + b * kRcThreads;
-      if (NB % kRcThreads == 0 || j < NB) {
-        float2 v0 = buf[j], v1 = buf[j + NB], v2 = buf[j + 2 * NB], v3 = buf[j + 3 * NB];
-        if (lns > 0) {
-          const int k = j & (ns - 1);
-          const int st = k << (LG - 2 - lns);  // k * NF / (4 ns)
-          float2 w1 = tw[st];
-          if (INV) w1.y = -w1.y;
-          const float2 w2 = cmul(w1, w1), w3 = cmul(w2, w1);
-          v1 = cmul(v1, w1);
-          v2 = cmul(v2, w2);
-          v3 = cmul(v3, w3);
-        }
-        const float2 a0 = cadd(v0, v2), a1 = csub(v0, v2), a2 = cadd(v1, v3), d = csub(v1, v3);
-        // forward: a3 = -j d; inverse: a3 = +j d
-        const float2 a3 = INV ? make_float2(-d.y, d.x) : make_float2(d.y, -d.x);
-        y[b][0] = cadd(a0, a2);
-        y[b][1] = cadd(a1, a3);
-        y[b][2] = csub(a0, a2);
-        y[b][3] = csub(a1, a3);
-      }
+    for (int pass = 0; pass < LG / 3; ++pass) {
+      stockham_pass<LG, 8, INV>(buf, tw, lns);
+      lns += 3;
     }
-    __syncthreads();
-#pragma unroll
-    for (int b = 0; b < PER; ++b) {
-      const int j = tid + b * kRcThreads;
-      if (NB % kRcThreads == 0 || j < NB) {
-        const int k = j & (ns - 1);
-        const int dst = ((j - k) << 2) + k;
-        buf[dst] = y[b][0];
-        buf[dst + ns] = y[b][1];
-        buf[dst + 2 * ns] = y[b][2];
-        buf[dst + 3 * ns] = y[b][3];
-      }
-    }
-    __syncthreads();
   }
 }
 
@@ -166,7 +224,7 @@ __global__ __launch_bounds__(rc_threads(LG)) void rc_kernel(RcArgs a) {
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int i = tid + e * kRcThreads;
-    if (NF % kRcThreads == 0 || i < NF) buf[i] = i < a.n_raw ? row[i] : make_float2(0.f, 0.f);
+    if (NF % kRcThreads == 0 || i < NF) buf[lp<LG>(i)] = i < a.n_raw ? row[i] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   fft_lds<LG, false>(buf, a.tw);
@@ -174,7 +232,7 @@ __global__ __launch_bounds__(rc_threads(LG)) void rc_kernel(RcArgs a) {
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int i = tid + e * kRcThreads;
-    if (NF % kRcThreads == 0 || i < NF) spec[e] = cmul(buf[i], a.hc[i]);
+    if (NF % kRcThreads == 0 || i < NF) spec[e] = cmul(buf[lp<LG>(i)], a.hc[i]);
   }
   const size_t obase = m * (size_t)a.n_out;
   if constexpr (LT > 0) {
@@ -185,14 +243,14 @@ __global__ __launch_bounds__(rc_threads(LG)) void rc_kernel(RcArgs a) {
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const int i = tid + e * kRcThreads;
-        if (NF % kRcThreads == 0 || i < NF) buf[i] = spec[e];
+        if (NF % kRcThreads == 0 || i < NF) buf[lp<LG>(i)] = spec[e];
       }
       __syncthreads();
       fft_lds<LG, true>(buf, a.tw);
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const int q = tid + e * kRcThreads;
-        if (NF % kRcThreads == 0 || q < NF) keep[e][r] = buf[q];
+        if (NF % kRcThreads == 0 || q < NF) keep[e][r] = buf[lp<LG>(q)];
       }
       if (r + 1 < LT) {
 #pragma unroll
@@ -215,7 +273,7 @@ __global__ __launch_bounds__(rc_threads(LG)) void rc_kernel(RcArgs a) {
         const int q = tid + e * kRcThreads;
         if (q >= c * QC && q < (c + 1) * QC) {
 #pragma unroll
-          for (int r = 0; r < LT; ++r) buf[(q - c * QC) * LT + r] = keep[e][r];
+          for (int r = 0; r < LT; ++r) buf[lp<LG>((q - c * QC) * LT + r)] = keep[e][r];
         }
       }
       __syncthreads();
@@ -224,7 +282,7 @@ __global__ __launch_bounds__(rc_threads(LG)) void rc_kernel(RcArgs a) {
         const int i = tid + e * kRcThreads;
         const int o = u0 + i - a.crop;
         if ((NF % kRcThreads == 0 || i < NF) && o >= 0 && o < a.n_out)
-          store_out(a.out, obase + (size_t)o, buf[i], a.dtype);
+          store_out(a.out, obase + (size_t)o, buf[lp<LG>(i)], a.dtype);
       }
     }
   } else {
@@ -239,7 +297,7 @@ __global__ __launch_bounds__(rc_threads(LG)) void rc_kernel(RcArgs a) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
           const int i = tid + e * kRcThreads;
-          if (NF % kRcThreads == 0 || i < NF) buf[i] = spec[e];
+          if (NF % kRcThreads == 0 || i < NF) buf[lp<LG>(i)] = spec[e];
         }
         __syncthreads();
         fft_lds<LG, true>(buf, a.tw);
@@ -247,7 +305,7 @@ __global__ __launch_bounds__(rc_threads(LG)) void rc_kernel(RcArgs a) {
         for (int e = 0; e < E; ++e) {
           const int q = tid + e * kRcThreads;
           if (q >= q_lo && q <= q_hi)
-            store_out(a.out, obase + (size_t)(a.L * q + r - a.crop), buf[q], a.dtype);
+            store_out(a.out, obase + (size_t)(a.L * q + r - a.crop), buf[lp<LG>(q)], a.dtype);
         }
       }
       if (r + 1 < a.L) {
@@ -264,7 +322,7 @@ __global__ __launch_bounds__(rc_threads(LG)) void rc_kernel(RcArgs a) {
 template <int LG, int LT>
 hipError_t launch_rc(const RcArgs& a, int n_pulses, hipStream_t s) {
   hipLaunchKernelGGL((rc_kernel<LG, LT>), dim3((unsigned)n_pulses), dim3(rc_threads(LG)),
-                     (size_t)(1 << LG) * sizeof(float2), s, a);
+                     (size_t)lds_words<LG>() * sizeof(float2), s, a);
   return hipGetLastError();
 }
 
