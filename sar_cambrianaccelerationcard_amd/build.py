@@ -63,7 +63,7 @@ def build_sarbp(force: bool = True) -> str:
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libsarbp.so")
     srcs = [os.path.join(CSRC, f) for f in SARBP_SOURCES]
-    deps = srcs + [os.path.join(CSRC, h) for h in ("sarbp_internal.h", "sarbp_host.h",
+    deps = srcs + [os.path.join(CSRC, h) for h in ("sarbp_internal.h", "sarbp_host.h", "sarbp_fft.h",
                                                    "sarbp_kernels.inc")] + \
         [os.path.join(INCLUDE, "sarbp.h")]
     if not force and _newer(out, deps):

@@ -1,0 +1,159 @@
+// sarbp_fft.h -- in-LDS power-of-two FFTs for one workgroup: radix-8
+// Stockham passes (a radix-2 or radix-4 pass first when log2 n is not a
+// multiple of 3), used by range compression (sarbp_range.hip) and by the
+// omega-K line transforms (sarbp_wk.hip).  Not part of the public ABI.
+//
+// Element i of the transform lives at LDS slot fft::slot<LG>(i): one pad word
+// per 16 for n <= 4096 (buffer fft::lds_words<LG>() complex64 words), an XOR
+// swizzle of bits 1-3 by bits 4-6 for n = 8192 (exactly 64 KiB).  Either
+// turns the power-of-two strides of the Stockham writes into ones whose 16
+// lanes per LDS access hit distinct banks.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace sarbp {
+namespace fft {
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+
+template <int LG>
+__device__ __forceinline__ int slot(int i) {
+  if constexpr (LG <= 12)
+    return i + (i >> 4);
+  else
+    return i ^ (((i >> 4) & 7) << 1);
+}
+template <int LG>
+constexpr int lds_words() { return LG <= 12 ? (1 << LG) + (1 << LG) / 16 : 1 << LG; }
+
+// R-point DFT of x in registers, in place, natural output order.  INV: the
+// conjugate transform (unscaled inverse).
+template <int R, bool INV>
+__device__ __forceinline__ void dft_r(float2* x) {
+  // multiply by -j (forward) / +j (inverse)
+  auto mj = [](float2 v) { return INV ? make_float2(-v.y, v.x) : make_float2(v.y, -v.x); };
+  if constexpr (R == 2) {
+    const float2 a = x[0], b = x[1];
+    x[0] = cadd(a, b);
+    x[1] = csub(a, b);
+  } else if constexpr (R == 4) {
+    const float2 t0 = cadd(x[0], x[2]), t1 = csub(x[0], x[2]), t2 = cadd(x[1], x[3]), t3 = mj(csub(x[1], x[3]));
+    x[0] = cadd(t0, t2);
+    x[1] = cadd(t1, t3);
+    x[2] = csub(t0, t2);
+    x[3] = csub(t1, t3);
+  } else {
+    // 8 = 4 x 2: u_k = x_k + x_{k+4}, v_k = W8^k (x_k - x_{k+4}); the even
+    // outputs are DFT4(u), the odd ones DFT4(v).
+    constexpr float s = 0.70710678118654752f;
+    float2 u[4], v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      u[k] = cadd(x[k], x[k + 4]);
+      v[k] = csub(x[k], x[k + 4]);
+    }
+    // W8 = (1 - j)/sqrt2 forward, (1 + j)/sqrt2 inverse; W8^2 = -j / +j.
+    v[1] = INV ? make_float2(s * (v[1].x - v[1].y), s * (v[1].x + v[1].y))
+               : make_float2(s * (v[1].x + v[1].y), s * (v[1].y - v[1].x));
+    v[2] = mj(v[2]);
+    v[3] = INV ? make_float2(-s * (v[3].x + v[3].y), s * (v[3].x - v[3].y))
+               : make_float2(s * (v[3].y - v[3].x), -s * (v[3].x + v[3].y));
+    dft_r<4, INV>(u);
+    dft_r<4, INV>(v);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      x[2 * k] = u[k];
+      x[2 * k + 1] = v[k];
+    }
+  }
+}
+
+// One radix-R Stockham pass over NF = 2^LG points in LDS after sub-transforms
+// of size ns = 2^lns: butterfly b (k = b mod ns) reads b + r NF/R, multiplies
+// input r by W^{r k} of size R ns (tw[t] = exp(-j 2 pi t / NF); powers by
+// repeated products), and writes output r to (b - k) R + k + r ns.  All reads
+// complete (barrier) before any write.
+template <int LG, int T, int R, bool INV>
+__device__ __forceinline__ void stockham_pass(float2* buf, const float2* __restrict__ tw, int lns) {
+  constexpr int NF = 1 << LG;
+  constexpr int kT = T;
+  constexpr int NB = NF / R;
+  constexpr int PER = (NB + kT - 1) / kT;
+  constexpr int LGR = R == 8 ? 3 : (R == 4 ? 2 : 1);
+  const int tid = threadIdx.x;
+  const int ns = 1 << lns;
+  float2 y[PER][R];
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int b = tid + p * kT;
+    if (NB % kT == 0 || b < NB) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) y[p][r] = buf[slot<LG>(b + r * NB)];
+      if (lns > 0) {
+        const int k = b & (ns - 1);
+        float2 w[R];
+        w[1] = tw[k << (LG - lns - LGR)];
+        if (INV) w[1].y = -w[1].y;
+        // Powers by at most three products: w2 = w1^2, w3 = w2 w1, w4 = w2^2,
+        // w5..w7 = w4 w1..w3.
+        if constexpr (R >= 4) {
+          w[2] = cmul(w[1], w[1]);
+          w[3] = cmul(w[2], w[1]);
+        }
+        if constexpr (R == 8) {
+          w[4] = cmul(w[2], w[2]);
+          w[5] = cmul(w[4], w[1]);
+          w[6] = cmul(w[4], w[2]);
+          w[7] = cmul(w[4], w[3]);
+        }
+#pragma unroll
+        for (int r = 1; r < R; ++r) y[p][r] = cmul(y[p][r], w[r]);
+      }
+      dft_r<R, INV>(y[p]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int b = tid + p * kT;
+    if (NB % kT == 0 || b < NB) {
+      const int k = b & (ns - 1);
+      const int dst = ((b - k) << LGR) + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[slot<LG>(dst + r * ns)] = y[p][r];
+    }
+  }
+  __syncthreads();
+}
+
+// In-place FFT of NF = 2^LG points held in LDS at slots slot<LG>(i) by a
+// workgroup of T threads: a radix 2 or 4 pass when LG is not a multiple of 3,
+// then radix-8 passes.  tw[t] = exp(-j 2 pi t / NF).  INV uses the conjugate
+// twiddles (unscaled inverse).  Ends with a barrier.
+template <int LG, int T, bool INV>
+__device__ __forceinline__ void fft_lds(float2* buf, const float2* __restrict__ tw) {
+  int lns = 0;
+  if constexpr (LG % 3 == 1) {
+    stockham_pass<LG, T, 2, INV>(buf, tw, 0);
+    lns = 1;
+  } else if constexpr (LG % 3 == 2) {
+    stockham_pass<LG, T, 4, INV>(buf, tw, 0);
+    lns = 2;
+  }
+  if constexpr (LG >= 3) {
+#pragma unroll
+    for (int pass = 0; pass < LG / 3; ++pass) {
+      stockham_pass<LG, T, 8, INV>(buf, tw, lns);
+      lns += 3;
+    }
+  }
+}
+
+}  // namespace fft
+}  // namespace sarbp

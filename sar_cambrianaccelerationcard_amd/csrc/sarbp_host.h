@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <complex>
 #include <map>
 #include <mutex>
 #include <string>
@@ -88,6 +89,9 @@ class BatchExec {
   std::map<std::tuple<int, int, int, int, int, size_t>, hipGraphExec_t> graphs_;
   int captured_ = 0;
 };
+
+// Iterative radix-2 FFT, FP64, forward (exp(-j ...)), size a power of two.
+void fft_host(std::vector<std::complex<double>>& a);
 
 // Device tables of range compression (twiddles, matched filter, phase ramp),
 // one set per (FFT size, upsampling, chirp) seen by a context.

@@ -13,14 +13,12 @@
 //   x[L q + r] = sum_k Y[k] e^{j 2 pi k~ (L q + r) / (L nf)}
 //              = IFFT_nf( Y[k] e^{j 2 pi k~ r / (L nf)} )[q],
 // so phase r is an nf-point inverse FFT of the spectrum times a ramp.  The
-// FFTs are radix-8 Stockham passes (a radix-2 or radix-4 pass first when
-// log2 nf is not a multiple of 3), in place in LDS with one pad word per 16
-// (bank-conflict-free strides): every thread reads its butterfly inputs into
-// registers, the workgroup synchronises, then writes the outputs.
+// FFTs are the in-LDS radix-8 Stockham transforms of sarbp_fft.h.
 //
 // Twiddles, the matched filter conj(FFT(h))/nf and the ramp are FP32 tables
 // computed in FP64 on the host and kept per context.
 #include "sarbp.h"
+#include "sarbp_fft.h"
 #include "sarbp_host.h"
 
 #include <cmath>
@@ -44,147 +42,9 @@ constexpr int kRcMaxLog2 = 13;  // nf <= 8192: 64 KiB of LDS
 constexpr int rc_threads(int lg) { return lg >= 11 ? (1 << lg) / 8 : 256; }
 constexpr double kPiD = 3.14159265358979323846;
 
-__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-
-// LDS slot of element i: one pad word per 16 for transforms up to 4096
-// points (the 8192-point buffer has to fit 64 KiB unpadded).  The padding
-// turns the power-of-two strides of the Stockham writes into odd ones, so
-// the 16 lanes of an LDS access fall into distinct banks.
-template <int LG>
-__device__ __forceinline__ int lp(int i) {
-  if constexpr (LG <= 12)
-    return i + (i >> 4);
-  else
-    return i;
-}
-template <int LG>
-constexpr int lds_words() { return LG <= 12 ? (1 << LG) + (1 << LG) / 16 : 1 << LG; }
-
-// R-point DFT of x in registers, in place, natural output order.  INV: the
-// conjugate transform (unscaled inverse).
-template <int R, bool INV>
-__device__ __forceinline__ void dft_r(float2* x) {
-  // multiply by -j (forward) / +j (inverse)
-  auto mj = [](float2 v) { return INV ? make_float2(-v.y, v.x) : make_float2(v.y, -v.x); };
-  if constexpr (R == 2) {
-    const float2 a = x[0], b = x[1];
-    x[0] = cadd(a, b);
-    x[1] = csub(a, b);
-  } else if constexpr (R == 4) {
-    const float2 t0 = cadd(x[0], x[2]), t1 = csub(x[0], x[2]), t2 = cadd(x[1], x[3]), t3 = mj(csub(x[1], x[3]));
-    x[0] = cadd(t0, t2);
-    x[1] = cadd(t1, t3);
-    x[2] = csub(t0, t2);
-    x[3] = csub(t1, t3);
-  } else {
-    // 8 = 4 x 2: u_k = x_k + x_{k+4}, v_k = W8^k (x_k - x_{k+4}); the even
-    // outputs are DFT4(u), the odd ones DFT4(v).
-    constexpr float s = 0.70710678118654752f;
-    float2 u[4], v[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      u[k] = cadd(x[k], x[k + 4]);
-      v[k] = csub(x[k], x[k + 4]);
-    }
-    // W8 = (1 - j)/sqrt2 forward, (1 + j)/sqrt2 inverse; W8^2 = -j / +j.
-    v[1] = INV ? make_float2(s * (v[1].x - v[1].y), s * (v[1].x + v[1].y))
-               : make_float2(s * (v[1].x + v[1].y), s * (v[1].y - v[1].x));
-    v[2] = mj(v[2]);
-    v[3] = INV ? make_float2(-s * (v[3].x + v[3].y), s * (v[3].x - v[3].y))
-               : make_float2(s * (v[3].y - v[3].x), -s * (v[3].x + v[3].y));
-    dft_r<4, INV>(u);
-    dft_r<4, INV>(v);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      x[2 * k] = u[k];
-      x[2 * k + 1] = v[k];
-    }
-  }
-}
-
-// One radix-R Stockham pass over NF = 2^LG points in LDS after sub-transforms
-// of size ns = 2^lns: butterfly b (k = b mod ns) reads b + r NF/R, multiplies
-// input r by W^{r k} of size R ns (tw[t] = exp(-j 2 pi t / NF); powers by
-// repeated products), and writes output r to (b - k) R + k + r ns.  All reads
-// complete (barrier) before any write.
-template <int LG, int R, bool INV>
-__device__ __forceinline__ void stockham_pass(float2* buf, const float2* __restrict__ tw, int lns) {
-  constexpr int NF = 1 << LG;
-  constexpr int kT = rc_threads(LG);
-  constexpr int NB = NF / R;
-  constexpr int PER = (NB + kT - 1) / kT;
-  constexpr int LGR = R == 8 ? 3 : (R == 4 ? 2 : 1);
-  const int tid = threadIdx.x;
-  const int ns = 1 << lns;
-  float2 y[PER][R];
-#pragma unroll
-  for (int p = 0; p < PER; ++p) {
-    const int b = tid + p * kT;
-    if (NB % kT == 0 || b < NB) {
-#pragma unroll
-      for (int r = 0; r < R; ++r) y[p][r] = buf[lp<LG>(b + r * NB)];
-      if (lns > 0) {
-        const int k = b & (ns - 1);
-        float2 w[R];
-        w[1] = tw[k << (LG - lns - LGR)];
-        if (INV) w[1].y = -w[1].y;
-        // Powers by at most three products: w2 = w1^2, w3 = w2 w1, w4 = w2^2,
-        // w5..w7 = w4 w1..w3.
-        if constexpr (R >= 4) {
-          w[2] = cmul(w[1], w[1]);
-          w[3] = cmul(w[2], w[1]);
-        }
-        if constexpr (R == 8) {
-          w[4] = cmul(w[2], w[2]);
-          w[5] = cmul(w[4], w[1]);
-          w[6] = cmul(w[4], w[2]);
-          w[7] = cmul(w[4], w[3]);
-        }
-#pragma unroll
-        for (int r = 1; r < R; ++r) y[p][r] = cmul(y[p][r], w[r]);
-      }
-      dft_r<R, INV>(y[p]);
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int p = 0; p < PER; ++p) {
-    const int b = tid + p * kT;
-    if (NB % kT == 0 || b < NB) {
-      const int k = b & (ns - 1);
-      const int dst = ((b - k) << LGR) + k;
-#pragma unroll
-      for (int r = 0; r < R; ++r) buf[lp<LG>(dst + r * ns)] = y[p][r];
-    }
-  }
-  __syncthreads();
-}
-
-// In-place FFT of NF = 2^LG points held in LDS at slots lp<LG>(i): a radix
-// 2 or 4 pass when LG is not a multiple of 3, then radix-8 passes.  INV uses
-// the conjugate twiddles (unscaled inverse).  Ends with a barrier.
-template <int LG, bool INV>
-__device__ __forceinline__ void fft_lds(float2* buf, const float2* __restrict__ tw) {
-  int lns = 0;
-  if constexpr (LG % 3 == 1) {
-    stockham_pass<LG, 2, INV>(buf, tw, 0);
-    lns = 1;
-  } else if constexpr (LG % 3 == 2) {
-    stockham_pass<LG, 4, INV>(buf, tw, 0);
-    lns = 2;
-  }
-  if constexpr (LG >= 3) {
-#pragma unroll
-    for (int pass = 0; pass < LG / 3; ++pass) {
-      stockham_pass<LG, 8, INV>(buf, tw, lns);
-      lns += 3;
-    }
-  }
-}
+using fft::cadd;
+using fft::cmul;
+using fft::csub;
 
 __device__ __forceinline__ uint32_t bf16_rne(float x) {
   const uint32_t u = __float_as_uint(x);
@@ -224,15 +84,15 @@ __global__ __launch_bounds__(rc_threads(LG)) void rc_kernel(RcArgs a) {
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int i = tid + e * kRcThreads;
-    if (NF % kRcThreads == 0 || i < NF) buf[lp<LG>(i)] = i < a.n_raw ? row[i] : make_float2(0.f, 0.f);
+    if (NF % kRcThreads == 0 || i < NF) buf[fft::slot<LG>(i)] = i < a.n_raw ? row[i] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  fft_lds<LG, false>(buf, a.tw);
+  fft::fft_lds<LG, rc_threads(LG), false>(buf, a.tw);
   float2 spec[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int i = tid + e * kRcThreads;
-    if (NF % kRcThreads == 0 || i < NF) spec[e] = cmul(buf[lp<LG>(i)], a.hc[i]);
+    if (NF % kRcThreads == 0 || i < NF) spec[e] = cmul(buf[fft::slot<LG>(i)], a.hc[i]);
   }
   const size_t obase = m * (size_t)a.n_out;
   if constexpr (LT > 0) {
@@ -243,14 +103,14 @@ __global__ __launch_bounds__(rc_threads(LG)) void rc_kernel(RcArgs a) {
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const int i = tid + e * kRcThreads;
-        if (NF % kRcThreads == 0 || i < NF) buf[lp<LG>(i)] = spec[e];
+        if (NF % kRcThreads == 0 || i < NF) buf[fft::slot<LG>(i)] = spec[e];
       }
       __syncthreads();
-      fft_lds<LG, true>(buf, a.tw);
+      fft::fft_lds<LG, rc_threads(LG), true>(buf, a.tw);
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const int q = tid + e * kRcThreads;
-        if (NF % kRcThreads == 0 || q < NF) keep[e][r] = buf[lp<LG>(q)];
+        if (NF % kRcThreads == 0 || q < NF) keep[e][r] = buf[fft::slot<LG>(q)];
       }
       if (r + 1 < LT) {
 #pragma unroll
@@ -273,7 +133,7 @@ __global__ __launch_bounds__(rc_threads(LG)) void rc_kernel(RcArgs a) {
         const int q = tid + e * kRcThreads;
         if (q >= c * QC && q < (c + 1) * QC) {
 #pragma unroll
-          for (int r = 0; r < LT; ++r) buf[lp<LG>((q - c * QC) * LT + r)] = keep[e][r];
+          for (int r = 0; r < LT; ++r) buf[fft::slot<LG>((q - c * QC) * LT + r)] = keep[e][r];
         }
       }
       __syncthreads();
@@ -282,7 +142,7 @@ __global__ __launch_bounds__(rc_threads(LG)) void rc_kernel(RcArgs a) {
         const int i = tid + e * kRcThreads;
         const int o = u0 + i - a.crop;
         if ((NF % kRcThreads == 0 || i < NF) && o >= 0 && o < a.n_out)
-          store_out(a.out, obase + (size_t)o, buf[lp<LG>(i)], a.dtype);
+          store_out(a.out, obase + (size_t)o, buf[fft::slot<LG>(i)], a.dtype);
       }
     }
   } else {
@@ -297,15 +157,15 @@ __global__ __launch_bounds__(rc_threads(LG)) void rc_kernel(RcArgs a) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
           const int i = tid + e * kRcThreads;
-          if (NF % kRcThreads == 0 || i < NF) buf[lp<LG>(i)] = spec[e];
+          if (NF % kRcThreads == 0 || i < NF) buf[fft::slot<LG>(i)] = spec[e];
         }
         __syncthreads();
-        fft_lds<LG, true>(buf, a.tw);
+        fft::fft_lds<LG, rc_threads(LG), true>(buf, a.tw);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
           const int q = tid + e * kRcThreads;
           if (q >= q_lo && q <= q_hi)
-            store_out(a.out, obase + (size_t)(a.L * q + r - a.crop), buf[lp<LG>(q)], a.dtype);
+            store_out(a.out, obase + (size_t)(a.L * q + r - a.crop), buf[fft::slot<LG>(q)], a.dtype);
         }
       }
       if (r + 1 < a.L) {
@@ -322,7 +182,7 @@ __global__ __launch_bounds__(rc_threads(LG)) void rc_kernel(RcArgs a) {
 template <int LG, int LT>
 hipError_t launch_rc(const RcArgs& a, int n_pulses, hipStream_t s) {
   hipLaunchKernelGGL((rc_kernel<LG, LT>), dim3((unsigned)n_pulses), dim3(rc_threads(LG)),
-                     (size_t)lds_words<LG>() * sizeof(float2), s, a);
+                     (size_t)fft::lds_words<LG>() * sizeof(float2), s, a);
   return hipGetLastError();
 }
 
@@ -358,27 +218,6 @@ hipError_t launch_rc_lg(int lg, const RcArgs& a, int n_pulses, hipStream_t s) {
 
 using cd = std::complex<double>;
 
-// Iterative radix-2 FFT, FP64, forward (exp(-j ...)), n a power of two.
-void fft_host(std::vector<cd>& a) {
-  const size_t n = a.size();
-  for (size_t i = 1, j = 0; i < n; ++i) {
-    size_t bit = n >> 1;
-    for (; j & bit; bit >>= 1) j ^= bit;
-    j ^= bit;
-    if (i < j) std::swap(a[i], a[j]);
-  }
-  for (size_t len = 2; len <= n; len <<= 1) {
-    const double ang = -2.0 * kPiD / (double)len;
-    for (size_t i = 0; i < n; i += len)
-      for (size_t k = 0; k < len / 2; ++k) {
-        const cd w(std::cos(ang * (double)k), std::sin(ang * (double)k));
-        const cd u = a[i + k], v = a[i + k + len / 2] * w;
-        a[i + k] = u + v;
-        a[i + k + len / 2] = u - v;
-      }
-  }
-}
-
 int chirp_samples(const sarbp_range_params* rp) { return (int)std::ceil(rp->pulse_width * rp->fs); }
 
 sarbp_status check_params(const sarbp_range_params* rp, int* lg_out) {
@@ -409,6 +248,27 @@ sarbp_status check_params(const sarbp_range_params* rp, int* lg_out) {
 }
 
 }  // namespace
+
+// Iterative radix-2 FFT, FP64, forward (exp(-j ...)), n a power of two.
+void fft_host(std::vector<cd>& a) {
+  const size_t n = a.size();
+  for (size_t i = 1, j = 0; i < n; ++i) {
+    size_t bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) std::swap(a[i], a[j]);
+  }
+  for (size_t len = 2; len <= n; len <<= 1) {
+    const double ang = -2.0 * kPiD / (double)len;
+    for (size_t i = 0; i < n; i += len)
+      for (size_t k = 0; k < len / 2; ++k) {
+        const cd w(std::cos(ang * (double)k), std::sin(ang * (double)k));
+        const cd u = a[i + k], v = a[i + k + len / 2] * w;
+        a[i + k] = u + v;
+        a[i + k + len / 2] = u - v;
+      }
+  }
+}
 
 struct RangeTables {
   float2* d = nullptr;  // [tw | hc | ramp], nf entries each

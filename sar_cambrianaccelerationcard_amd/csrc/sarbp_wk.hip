@@ -3,31 +3,43 @@
 // interpolation along range frequency with a P-tap truncated sinc, 2-D inverse
 // FFT (code/SAR_WK/wk_v5.2.3/wk.cc:43-343, code/SAR_WK/matlab/mian_wka_sinc.m).
 //
+// The 2-D transforms are separable, so the image is formed line by line in
+// the order azimuth FFT -> [range DFT, reference function, Stolt, range
+// inverse DFT] -> azimuth inverse FFT, and the bracket is ONE kernel per
+// azimuth-frequency line (wk_rows): the line is read from HBM once, its
+// length-nr range DFT and inverse DFT run as Bluestein chirp-z transforms on
+// 2^LG-point FFTs in LDS (nr need not be a power of two: the reference's
+// scene pads to 3414 = 2 * 3 * 569), and it is written once.  The azimuth
+// transforms are power-of-two hipFFT row transforms between tiled transposes.
 // Kernels written here:
-//   wk_prep   one workgroup per padded azimuth line: zeros or the echo sample
-//             times the line's down-conversion factor (FP64 phase, one
-//             sincos per line).  Reads the echo once, writes the padded array
-//             once.
-//   wk_stolt  one workgroup per azimuth-frequency line: the line is read from
-//             HBM into LDS (optionally times the reference function, whose
-//             phase ~1e8 rad is formed and reduced in FP64), then each output
-//             sample gathers its P taps from LDS.  The Stolt square root is
-//             the reference function's, so both come from one FP64 sqrt per
-//             sample.  One read and one write of the line: in place is safe
-//             (the workgroup owns its line and stages it before writing).
-// The 2-D transforms are hipFFT plans (the padded range length of the
-// reference's scene, 3414 = 2 * 3 * 569, needs a prime-size algorithm).
+//   wk_prep      one workgroup per padded azimuth line: zeros, or the echo
+//                sample times the line's down-conversion factor (FP64 phase).
+//   wk_transpose 64 x 64 tiles through LDS.
+//   wk_rows      per line: stage x[n] conj(b[n]) (b[n] = exp(j pi n^2 / nr)),
+//                FFT, times the chirp filter spectrum, inverse FFT, times
+//                conj(b[k]): the range spectrum X[k]; times the reference
+//                function; Stolt taps into registers; then the same chirp-z
+//                steps on conj(Y[n] b[n]) give the inverse DFT.
+//   wk_stolt     the Stolt step alone on a line staged in LDS (the
+//                sarbp_stolt_sinc entry point, and omega-K for nr > 4096
+//                with hipFFT's 2-D plan).
+// Per sample one FP64 sqrt s = sqrt(max(0, (f0+fr)^2 - c^2 fa^2/(4 vr^2)))
+// gives both the reference phase (revolutions 2 r0/c s + fr^2/(2 kr), reduced
+// in FP64) and the Stolt centre n - (s - f0 - fr) nr/fs.
 #include "sarbp.h"
+#include "sarbp_fft.h"
 #include "sarbp_host.h"
 
 #include <hipfft/hipfft.h>
 
 #include <cmath>
+#include <complex>
 #include <cstring>
 #include <map>
 #include <mutex>
 #include <new>
 #include <utility>
+#include <vector>
 
 namespace sarbp {
 
@@ -36,7 +48,11 @@ namespace {
 constexpr int kWkThreads = 256;
 constexpr int kWkMaxRg = 8192;  // a line of complex64 in 64 KiB of LDS
 constexpr int kWkMaxTaps = 16;
+constexpr int kRowsMaxRg = 4096;  // fused line kernel: 2 nr - 1 <= 8192 points
 constexpr double kPiD = 3.14159265358979323846;
+
+using fft::cmul;
+using fft::conjf2;
 
 struct StoltArgs {
   const float2* in;
@@ -68,9 +84,47 @@ __device__ __forceinline__ float sinc_at(float x, float sin_pi_x) {
   return x == 0.0f ? 1.0f : sin_pi_x * __builtin_amdgcn_rcpf(3.14159265358979f * x);
 }
 
+// LDS slot of line sample i: identity (LG = 0) or the FFT buffer's mapping.
+template <int LG>
+__device__ __forceinline__ int lslot(int i) {
+  if constexpr (LG == 0)
+    return i;
+  else
+    return fft::slot<LG>(i);
+}
+
+// Azimuth frequency of line m and c^2 fa^2 / (4 vr^2).
+__device__ __forceinline__ double line_kfa2(const StoltArgs& a, int m) {
+  const double fa = a.fa_axis ? a.fa_axis[m] : a.f_nc + shifted_bin(m, a.na) * a.fa_gap;
+  return a.kk * fa * fa;
+}
+
+// Range-frequency sample j of a line: applies the reference function to v
+// (if a.rfm) and returns the Stolt centre as (c0, f), c0 + f = centre.
+__device__ __forceinline__ float2 stolt_sample(const StoltArgs& a, int j, double kfa2, float2 v, int* c0,
+                                               float* f) {
+  const double fr = a.fr_axis ? a.fr_axis[j] : shifted_bin(j, a.nr) * a.fr_gap;
+  const double fv = a.f0 + fr;
+  const double s = sqrt(fmax(fma(fv, fv, -kfa2), 0.0));
+  if (a.rfm) {
+    // Phase in revolutions, reduced to [-1/2, 1/2]: the hardware sine and
+    // cosine take revolutions (absolute error ~1.2e-7 there).
+    const double rev = fma(a.rfm_a, s, a.rfm_b * fr * fr);
+    const float ph = (float)(rev - rint(rev));
+    const float sn = __builtin_amdgcn_sinf(ph), cs = __builtin_amdgcn_cosf(ph);
+    v = make_float2(v.x * cs - v.y * sn, v.x * sn + v.y * cs);
+  }
+  const double cen = (double)j - (s - fv) * a.nr_over_fs;
+  const double fl = floor(cen);
+  *c0 = (int)fl;
+  *f = (float)(cen - fl);
+  return v;
+}
+
 // Sum over the taps of one output at centre c0 + f (f in [0, 1)) for any tap
 // count, with per-tap bounds handling: the floor window clamps, the centred
 // window skips taps outside the line.
+template <int LG>
 __device__ __forceinline__ float2 stolt_taps_general(const float2* line, int nr, int c0, float f, float sf,
                                                      int half, int window) {
   float2 acc = make_float2(0.0f, 0.0f);
@@ -86,7 +140,7 @@ __device__ __forceinline__ float2 stolt_taps_general(const float2* line, int nr,
         const float x = (float)(c0 - idx) + f;
         w = sinc_at(x, sinpif(x));
       }
-      const float2 v = line[idx];
+      const float2 v = line[lslot<LG>(idx)];
       acc.x = fmaf(w, v.x, acc.x);
       acc.y = fmaf(w, v.y, acc.y);
     }
@@ -98,7 +152,7 @@ __device__ __forceinline__ float2 stolt_taps_general(const float2* line, int nr,
       const float x = f - (float)q;
       if (fabsf(x) > (float)half || (unsigned)idx >= (unsigned)nr) continue;
       const float w = sinc_at(x, (q & 1) ? -sf : sf);
-      const float2 v = line[idx];
+      const float2 v = line[lslot<LG>(idx)];
       acc.x = fmaf(w, v.x, acc.x);
       acc.y = fmaf(w, v.y, acc.y);
     }
@@ -109,7 +163,7 @@ __device__ __forceinline__ float2 stolt_taps_general(const float2* line, int nr,
 // The same sum for P taps at offsets q0 .. q0+P-1 that all lie inside the
 // line, 0 < f < 1 (no tap at distance 0): weights (-1)^q sin(pi f) / (pi (f - q)),
 // reciprocals taken in pairs (1/a = b/(ab), 1/b = a/(ab)).
-template <int P>
+template <int LG, int P>
 __device__ __forceinline__ float2 stolt_taps_inner(const float2* line, int c0, float f, float sf, int q0) {
   float d[P];
 #pragma unroll
@@ -120,13 +174,27 @@ __device__ __forceinline__ float2 stolt_taps_inner(const float2* line, int c0, f
     const float r = __builtin_amdgcn_rcpf(d[t] * d[t + 1]);
     const float s0 = ((q0 + t) & 1) ? -sf : sf;
     const float w0 = s0 * d[t + 1] * r, w1 = -s0 * d[t] * r;
-    const float2 v0 = line[c0 + q0 + t], v1 = line[c0 + q0 + t + 1];
+    const float2 v0 = line[lslot<LG>(c0 + q0 + t)], v1 = line[lslot<LG>(c0 + q0 + t + 1)];
     acc.x = fmaf(w0, v0.x, acc.x);
     acc.y = fmaf(w0, v0.y, acc.y);
     acc.x = fmaf(w1, v1.x, acc.x);
     acc.y = fmaf(w1, v1.y, acc.y);
   }
   return acc;
+}
+
+// One Stolt output at centre c0 + f from a line held in LDS.
+template <int LG, int P>
+__device__ __forceinline__ float2 stolt_output(const StoltArgs& a, const float2* line, int c0, float f) {
+  const int half = P > 0 ? P / 2 : a.taps >> 1;
+  // First tap offset and the range of c0 for which every tap is inside.
+  const int q0 = a.window == SARBP_STOLT_FLOOR ? -half : -half + 1;
+  const int c_lo = -q0, c_hi = a.nr - 1 - (q0 + 2 * half - 1);
+  // sin(pi (f - p)) = (-1)^p sin(pi f) for integer p.
+  const float sf = sinpif(f);
+  if (P > 0 && f > 0.0f && c0 >= c_lo && c0 <= c_hi)
+    return stolt_taps_inner<LG, (P > 0 ? P : 2)>(line, c0, f, sf, q0);
+  return stolt_taps_general<LG>(line, a.nr, c0, f, sf, half, a.window);
 }
 
 // K = samples per thread (nr <= K * kWkThreads); P = taps (0: any, runtime).
@@ -136,9 +204,7 @@ __global__ __launch_bounds__(kWkThreads) void wk_stolt(StoltArgs a) {
   const int m = blockIdx.x;
   const int nr = a.nr;
   const size_t base = (size_t)m * (size_t)nr;
-  const double fa = a.fa_axis ? a.fa_axis[m] : a.f_nc + shifted_bin(m, a.na) * a.fa_gap;
-  const double kfa2 = a.kk * fa * fa;
-
+  const double kfa2 = line_kfa2(a, m);
   int fl[K];
   float fr_off[K];
 #pragma unroll
@@ -146,46 +212,132 @@ __global__ __launch_bounds__(kWkThreads) void wk_stolt(StoltArgs a) {
     const int j = threadIdx.x + k * kWkThreads;
     fl[k] = 0;
     fr_off[k] = 0.0f;
-    if (j < nr) {
-      float2 v = a.in[base + j];
-      const double fr = a.fr_axis ? a.fr_axis[j] : shifted_bin(j, nr) * a.fr_gap;
-      const double f = a.f0 + fr;
-      const double s = sqrt(fmax(fma(f, f, -kfa2), 0.0));
-      if (a.rfm) {
-        // Phase in revolutions, reduced to [-1/2, 1/2]: the hardware sine and
-        // cosine take revolutions (absolute error ~1.2e-7 there).
-        const double rev = fma(a.rfm_a, s, a.rfm_b * fr * fr);
-        const float ph = (float)(rev - rint(rev));
-        const float sn = __builtin_amdgcn_sinf(ph), cs = __builtin_amdgcn_cosf(ph);
-        v = make_float2(v.x * cs - v.y * sn, v.x * sn + v.y * cs);
-      }
-      line[j] = v;
-      const double cen = (double)j - (s - f) * a.nr_over_fs;
-      const double c0 = floor(cen);
-      fl[k] = (int)c0;
-      fr_off[k] = (float)(cen - c0);
-    }
+    if (j < nr) line[j] = stolt_sample(a, j, kfa2, a.in[base + j], &fl[k], &fr_off[k]);
   }
   __syncthreads();
-
-  const int half = P > 0 ? P / 2 : a.taps >> 1;
-  // First tap offset and the range of c0 for which every tap is inside.
-  const int q0 = a.window == SARBP_STOLT_FLOOR ? -half : -half + 1;
-  const int c_lo = -q0, c_hi = nr - 1 - (q0 + 2 * half - 1);
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int j = threadIdx.x + k * kWkThreads;
     if (j >= nr) continue;
-    const int c0 = fl[k];
-    const float f = fr_off[k];
-    // sin(pi (f - p)) = (-1)^p sin(pi f) for integer p.
-    const float sf = sinpif(f);
-    float2 acc;
-    if (P > 0 && f > 0.0f && c0 >= c_lo && c0 <= c_hi)
-      acc = stolt_taps_inner<(P > 0 ? P : 2)>(line, c0, f, sf, q0);
-    else
-      acc = stolt_taps_general(line, nr, c0, f, sf, half, a.window);
+    const float2 acc = stolt_output<0, P>(a, line, fl[k], fr_off[k]);
     a.out[base + j] = make_float2(acc.x * a.scale, acc.y * a.scale);
+  }
+}
+
+// ---- fused line kernel ---------------------------------------------------------
+
+// Chirp-z tables of one (nr, LG): b[n] = exp(j pi n^2 / nr) for n < nr;
+// bh[k] = FFT_M(bb)[k] / M with bb[n] = b[|n|] at n and M - n (|n| < nr),
+// M = 2^LG >= 2 nr - 1; tw[t] = exp(-j 2 pi t / M).
+struct RowArgs {
+  StoltArgs st;       // in = out = the line array [na][nr]
+  const float2* tw;   // [M]
+  const float2* bh;   // [M]
+  const float2* chirp;  // [nr]
+};
+
+constexpr int rows_threads(int lg) { return (1 << lg) / 8; }
+
+// Length-nr DFT of the line staged as buf[slot(n)] = z[n] conj(b[n]) (n < nr,
+// zeros above): on return buf[slot(k)] holds (z conj(b)) * b circularly
+// convolved, i.e. DFT(z)[k] = conj(b[k]) buf[k] for k < nr.
+template <int LG>
+__device__ __forceinline__ void chirp_convolve(float2* buf, const RowArgs& r) {
+  constexpr int NF = 1 << LG, T = rows_threads(LG), E = NF / T;
+  fft::fft_lds<LG, T, false>(buf, r.tw);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int i = threadIdx.x + e * T;
+    buf[fft::slot<LG>(i)] = cmul(buf[fft::slot<LG>(i)], r.bh[i]);
+  }
+  __syncthreads();
+  fft::fft_lds<LG, T, true>(buf, r.tw);
+}
+
+// One workgroup per azimuth-frequency line m; nr <= K T, 2 nr - 1 <= 2^LG.
+template <int LG, int K, int P>
+__global__ __launch_bounds__(rows_threads(LG)) void wk_rows(RowArgs r) {
+  constexpr int NF = 1 << LG, T = rows_threads(LG), E = NF / T;
+  extern __shared__ float2 buf[];
+  const StoltArgs& a = r.st;
+  const int m = blockIdx.x;
+  const int nr = a.nr;
+  const size_t base = (size_t)m * (size_t)nr;
+  const int tid = threadIdx.x;
+
+  // Forward range DFT.
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int i = tid + e * T;
+    buf[fft::slot<LG>(i)] = i < nr ? cmul(a.in[base + i], conjf2(r.chirp[i])) : make_float2(0.0f, 0.0f);
+  }
+  __syncthreads();
+  chirp_convolve<LG>(buf, r);
+
+  // Range spectrum X[k] = conj(b[k]) c[k], times the reference function;
+  // Stolt centres kept per output.
+  const double kfa2 = line_kfa2(a, m);
+  int fl[K];
+  float fr_off[K];
+#pragma unroll
+  for (int t = 0; t < K; ++t) {
+    const int j = tid + t * T;
+    fl[t] = 0;
+    fr_off[t] = 0.0f;
+    if (j < nr) {
+      const float2 x = cmul(buf[fft::slot<LG>(j)], conjf2(r.chirp[j]));
+      buf[fft::slot<LG>(j)] = stolt_sample(a, j, kfa2, x, &fl[t], &fr_off[t]);
+    }
+  }
+  __syncthreads();
+  float2 y[K];
+#pragma unroll
+  for (int t = 0; t < K; ++t) {
+    const int j = tid + t * T;
+    y[t] = j < nr ? stolt_output<LG, P>(a, buf, fl[t], fr_off[t]) : make_float2(0.0f, 0.0f);
+  }
+  __syncthreads();
+
+  // Inverse range DFT: x[n] = conj(DFT(conj(Y))[n]) = b[n] conj(c[n]) with the
+  // line staged as conj(Y[n] b[n]).
+#pragma unroll
+  for (int t = 0; t < K; ++t) {
+    const int j = tid + t * T;
+    if (j < nr) buf[fft::slot<LG>(j)] = conjf2(cmul(y[t], r.chirp[j]));
+  }
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int i = tid + e * T;
+    if (i >= nr) buf[fft::slot<LG>(i)] = make_float2(0.0f, 0.0f);
+  }
+  __syncthreads();
+  chirp_convolve<LG>(buf, r);
+#pragma unroll
+  for (int t = 0; t < K; ++t) {
+    const int j = tid + t * T;
+    if (j < nr) {
+      const float2 v = cmul(r.chirp[j], conjf2(buf[fft::slot<LG>(j)]));
+      a.out[base + j] = make_float2(v.x * a.scale, v.y * a.scale);
+    }
+  }
+}
+
+// out[c][r] = in[r][c] for in [rows][cols]: 64 x 64 tiles, 256 threads.
+__global__ __launch_bounds__(256) void wk_transpose(const float2* __restrict__ in, float2* __restrict__ out,
+                                                     int rows, int cols) {
+  __shared__ float2 tile[64][65];
+  const int c0 = blockIdx.x * 64, r0 = blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int r = r0 + ty + 4 * k, c = c0 + tx;
+    if (r < rows && c < cols) tile[ty + 4 * k][tx] = in[(size_t)r * cols + c];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int c = c0 + ty + 4 * k, r = r0 + tx;
+    if (r < rows && c < cols) out[(size_t)c * rows + r] = tile[tx][ty + 4 * k];
   }
 }
 
@@ -218,6 +370,36 @@ __global__ __launch_bounds__(kWkThreads) void wk_prep(PrepArgs a) {
     }
     dst[j] = v;
   }
+}
+
+// Smallest LG in [11, 13] with 2 nr - 1 <= 2^LG (nr <= kRowsMaxRg).
+int rows_lg(int nr) {
+  int lg = 11;
+  while ((1 << lg) < 2 * nr - 1) ++lg;
+  return lg;
+}
+
+template <int LG, int P>
+void launch_rows_lp(const RowArgs& r, hipStream_t s) {
+  hipLaunchKernelGGL((wk_rows<LG, 4, P>), dim3((unsigned)r.st.na), dim3(rows_threads(LG)),
+                     (size_t)fft::lds_words<LG>() * sizeof(float2), s, r);
+}
+
+hipError_t launch_rows(const RowArgs& r, int lg, hipStream_t s) {
+  const bool p6 = r.st.taps == 6;
+  switch (lg) {
+    case 11: p6 ? launch_rows_lp<11, 6>(r, s) : launch_rows_lp<11, 0>(r, s); break;
+    case 12: p6 ? launch_rows_lp<12, 6>(r, s) : launch_rows_lp<12, 0>(r, s); break;
+    case 13: p6 ? launch_rows_lp<13, 6>(r, s) : launch_rows_lp<13, 0>(r, s); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_transpose(const float2* in, float2* out, int rows, int cols, hipStream_t s) {
+  hipLaunchKernelGGL(wk_transpose, dim3((unsigned)((cols + 63) / 64), (unsigned)((rows + 63) / 64)), dim3(256), 0,
+                     s, in, out, rows, cols);
+  return hipGetLastError();
 }
 
 template <int P>
@@ -261,7 +443,11 @@ sarbp_status check_stolt(double f0, double c, double vr, int na, int nr, double 
 }  // namespace
 
 struct WkCache {
-  std::map<std::pair<int, int>, hipfftHandle> plans;
+  std::map<std::pair<int, int>, hipfftHandle> plans;      // 2-D (na, nr)
+  std::map<std::pair<int, int>, hipfftHandle> row_plans;  // 1-D (length, batch)
+  std::map<int, float2*> chirp_tables;                    // nr -> [tw | bh | chirp]
+  float2* work = nullptr;  // transpose workspace of sarbp_omega_k
+  size_t work_cap = 0;     // complex64 words
   double* d_axes = nullptr;  // [nr | na] of sarbp_stolt_sinc
   double* h_axes = nullptr;  // pinned staging: the axes last uploaded
   size_t cap = 0;
@@ -272,7 +458,11 @@ struct WkCache {
 void wk_cache_destroy(WkCache* wc) {
   if (!wc) return;
   if (wc->done) (void)hipEventSynchronize(wc->done);
+  (void)hipDeviceSynchronize();  // the work buffer and tables may still be in use
   for (auto& kv : wc->plans) (void)hipfftDestroy(kv.second);
+  for (auto& kv : wc->row_plans) (void)hipfftDestroy(kv.second);
+  for (auto& kv : wc->chirp_tables) (void)hipFree(kv.second);
+  (void)hipFree(wc->work);
   (void)hipFree(wc->d_axes);
   (void)hipHostFree(wc->h_axes);
   if (wc->done) (void)hipEventDestroy(wc->done);
@@ -304,6 +494,78 @@ sarbp_status fft_plan(WkCache* wc, int na, int nr, hipfftHandle* out) {
                 "hipfftPlan2d(%d, %d) failed (%d)", na, nr, (int)r);
   wc->plans[key] = h;
   *out = h;
+  return SARBP_OK;
+}
+
+// Batched 1-D transforms of `batch` contiguous rows of length n.
+sarbp_status row_plan(WkCache* wc, int n, int batch, hipfftHandle* out) {
+  const auto key = std::make_pair(n, batch);
+  auto it = wc->row_plans.find(key);
+  if (it != wc->row_plans.end()) {
+    *out = it->second;
+    return SARBP_OK;
+  }
+  hipfftHandle h = nullptr;
+  int len[1] = {n};
+  const hipfftResult r = hipfftPlanMany(&h, 1, len, len, 1, n, len, 1, n, HIPFFT_C2C, batch);
+  if (r != HIPFFT_SUCCESS)
+    return fail(r == HIPFFT_ALLOC_FAILED ? SARBP_ERR_OUT_OF_MEMORY : SARBP_ERR_DEVICE,
+                "hipfftPlanMany(%d x %d) failed (%d)", n, batch, (int)r);
+  wc->row_plans[key] = h;
+  *out = h;
+  return SARBP_OK;
+}
+
+// Chirp-z tables of length nr on 2^lg points (see RowArgs), built in FP64.
+sarbp_status chirp_tables(WkCache* wc, int nr, int lg, const float2** out) {
+  auto it = wc->chirp_tables.find(nr);
+  if (it != wc->chirp_tables.end()) {
+    *out = it->second;
+    return SARBP_OK;
+  }
+  const int M = 1 << lg;
+  using cd = std::complex<double>;
+  std::vector<float2> h((size_t)2 * M + nr);
+  std::vector<cd> b((size_t)nr), bb((size_t)M, cd(0.0, 0.0));
+  for (int n = 0; n < nr; ++n) {
+    // pi n^2 / nr with n^2 reduced modulo 2 nr (exact in 64-bit integers).
+    const long long q = ((long long)n * n) % (2LL * nr);
+    const double ang = kPiD * (double)q / (double)nr;
+    b[n] = cd(std::cos(ang), std::sin(ang));
+  }
+  for (int n = 0; n < nr; ++n) {
+    bb[n] = b[n];
+    if (n > 0) bb[M - n] = b[n];
+  }
+  fft_host(bb);
+  for (int t = 0; t < M; ++t) {
+    const double ang = -2.0 * kPiD * (double)t / (double)M;
+    h[t] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+    const cd v = bb[t] / (double)M;
+    h[M + t] = make_float2((float)v.real(), (float)v.imag());
+  }
+  for (int n = 0; n < nr; ++n) h[2 * (size_t)M + n] = make_float2((float)b[n].real(), (float)b[n].imag());
+  float2* d = nullptr;
+  hipError_t e = hipMalloc(&d, h.size() * sizeof(float2));
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc(chirp tables)");
+  if ((e = hipMemcpy(d, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice)) != hipSuccess) {
+    (void)hipFree(d);
+    return hip_fail(e, "hipMemcpy(chirp tables)");
+  }
+  wc->chirp_tables[nr] = d;
+  *out = d;
+  return SARBP_OK;
+}
+
+sarbp_status work_buffer(WkCache* wc, size_t words) {
+  if (words <= wc->work_cap) return SARBP_OK;
+  hipError_t e = hipDeviceSynchronize();  // the old buffer may be in use
+  if (e != hipSuccess) return hip_fail(e, "hipDeviceSynchronize");
+  (void)hipFree(wc->work);
+  wc->work = nullptr;
+  wc->work_cap = 0;
+  if ((e = hipMalloc(&wc->work, words * sizeof(float2))) != hipSuccess) return hip_fail(e, "hipMalloc(omega-K work)");
+  wc->work_cap = words;
   return SARBP_OK;
 }
 
@@ -416,10 +678,9 @@ sarbp_status sarbp_omega_k(sarbp_context* ctx, const sarbp_cf32* d_echo, int32_t
   if (dg.err != hipSuccess) return hip_fail(dg.err, "hipSetDevice");
   WkCache* wc = nullptr;
   if ((st = wk_cache(ctx, &wc)) != SARBP_OK) return st;
-  hipfftHandle plan = nullptr;
-  if ((st = fft_plan(wc, na, nr, &plan)) != SARBP_OK) return st;
   const hipStream_t s = static_cast<hipStream_t>(stream);
   float2* img = reinterpret_cast<float2*>(d_image);
+  const int pad_az = (na - na_in) / 2, pad_rg = (nr - nr_in) / 2;
 
   PrepArgs pa{};
   pa.echo = reinterpret_cast<const float2*>(d_echo);
@@ -428,13 +689,9 @@ sarbp_status sarbp_omega_k(sarbp_context* ctx, const sarbp_cf32* d_echo, int32_t
   pa.nr = nr;
   pa.na_in = na_in;
   pa.nr_in = nr_in;
-  pa.pad_az = (na - na_in) / 2;
-  pa.pad_rg = (nr - nr_in) / 2;
+  pa.pad_az = pad_az;
+  pa.pad_rg = pad_rg;
   pa.rev_per_line = -wp->f_nc / wp->prf;
-  hipLaunchKernelGGL(wk_prep, dim3((unsigned)na), dim3(kWkThreads), 0, s, pa);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return hip_fail(e, "wk_prep launch");
-  if ((st = exec_fft(plan, img, HIPFFT_FORWARD, s)) != SARBP_OK) return st;
 
   StoltArgs a{};
   a.in = img;
@@ -453,8 +710,46 @@ sarbp_status sarbp_omega_k(sarbp_context* ctx, const sarbp_cf32* d_echo, int32_t
   a.rfm_a = 2.0 * wp->r0 / wp->c;
   a.rfm_b = 0.5 / wp->kr;
   a.scale = (float)(1.0 / ((double)na * (double)nr));
-  if ((e = launch_stolt(a, s)) != hipSuccess) return hip_fail(e, "wk_stolt launch");
-  return exec_fft(plan, img, HIPFFT_BACKWARD, s);
+  hipError_t e;
+
+  if (nr > kRowsMaxRg) {
+    // Long lines: hipFFT's 2-D plan and the Stolt kernel on its own.
+    hipfftHandle plan = nullptr;
+    if ((st = fft_plan(wc, na, nr, &plan)) != SARBP_OK) return st;
+    hipLaunchKernelGGL(wk_prep, dim3((unsigned)na), dim3(kWkThreads), 0, s, pa);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "wk_prep launch");
+    if ((st = exec_fft(plan, img, HIPFFT_FORWARD, s)) != SARBP_OK) return st;
+    if ((e = launch_stolt(a, s)) != hipSuccess) return hip_fail(e, "wk_stolt launch");
+    return exec_fft(plan, img, HIPFFT_BACKWARD, s);
+  }
+
+  // Azimuth FFT (columns, through the transposed workspace; only the nr_in
+  // columns holding echo are non-zero), one fused kernel per line for the
+  // range DFT / reference function / Stolt / range inverse DFT, azimuth
+  // inverse FFT.
+  const int lg = rows_lg(nr);
+  const float2* tab = nullptr;
+  hipfftHandle az_fwd = nullptr, az_inv = nullptr;
+  if ((st = chirp_tables(wc, nr, lg, &tab)) != SARBP_OK) return st;
+  if ((st = row_plan(wc, na, nr_in, &az_fwd)) != SARBP_OK) return st;
+  if ((st = row_plan(wc, na, nr, &az_inv)) != SARBP_OK) return st;
+  if ((st = work_buffer(wc, (size_t)na * nr)) != SARBP_OK) return st;
+  float2* w = wc->work;
+  hipLaunchKernelGGL(wk_prep, dim3((unsigned)na), dim3(kWkThreads), 0, s, pa);
+  if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "wk_prep launch");
+  if ((e = launch_transpose(img, w, na, nr, s)) != hipSuccess) return hip_fail(e, "wk_transpose launch");
+  if ((st = exec_fft(az_fwd, w + (size_t)pad_rg * na, HIPFFT_FORWARD, s)) != SARBP_OK) return st;
+  if ((e = launch_transpose(w, img, nr, na, s)) != hipSuccess) return hip_fail(e, "wk_transpose launch");
+  RowArgs r{};
+  r.st = a;
+  r.tw = tab;
+  r.bh = tab + ((size_t)1 << lg);
+  r.chirp = tab + ((size_t)2 << lg);
+  if ((e = launch_rows(r, lg, s)) != hipSuccess) return hip_fail(e, "wk_rows launch");
+  if ((e = launch_transpose(img, w, na, nr, s)) != hipSuccess) return hip_fail(e, "wk_transpose launch");
+  if ((st = exec_fft(az_inv, w, HIPFFT_BACKWARD, s)) != SARBP_OK) return st;
+  if ((e = launch_transpose(w, img, nr, na, s)) != hipSuccess) return hip_fail(e, "wk_transpose launch");
+  return SARBP_OK;
 }
 
 }  // extern "C"

@@ -225,3 +225,15 @@ def test_bp_and_omega_k_image_the_same_scene(ctx, f_nc):
     mw, mb = O.peak_metrics(wk, *far), O.peak_metrics(bp, *far)
     assert (mb["d_line"], mb["d_sample"]) == (0, 0) and mb["concentration"] > 0.25, mb
     assert mw["concentration"] < 0.15, mw
+
+
+def test_omega_k_long_lines_fallback(ctx):
+    # Padded lines longer than 4096 samples take the 2-D hipFFT plan and the
+    # Stolt kernel on its own; same oracle, same tolerance.
+    echo = _rand((64, 3100), 11)
+    scene = O.WkScene(na_in=64, nr_in=3100)
+    wp = scene.params()
+    got = _omega_k(ctx, echo, wp)
+    ref = _oracle(echo, wp)
+    assert got.shape == (86, 5166)
+    assert _rel_l2(got, ref) < 2e-5
