@@ -13,7 +13,9 @@
 // transforms are power-of-two hipFFT row transforms between tiled transposes.
 // Kernels written here:
 //   wk_prep      one workgroup per padded azimuth line: zeros, or the echo
-//                sample times the line's down-conversion factor (FP64 phase).
+//                sample times the line's down-conversion factor (FP64 phase);
+//                wk_prep_t the same written transposed ([nr][na]) for the
+//                azimuth transforms, in 64 x 64 tiles through LDS.
 //   wk_transpose 64 x 64 tiles through LDS.
 //   wk_rows      per line: stage x[n] conj(b[n]) (b[n] = exp(j pi n^2 / nr)),
 //                FFT, times the chirp filter spectrum, inverse FFT, times
@@ -369,6 +371,41 @@ __global__ __launch_bounds__(kWkThreads) void wk_prep(PrepArgs a) {
       v = make_float2(e.x * cs - e.y * sn, e.x * sn + e.y * cs);
     }
     dst[j] = v;
+  }
+}
+
+// wk_prep written transposed: out[j][m] (= [nr][na]) for 64 x 64 tiles of
+// (range sample j, padded line m), through LDS.
+__global__ __launch_bounds__(256) void wk_prep_t(PrepArgs a) {
+  __shared__ float2 tile[64][65];
+  __shared__ float2 rot[64];
+  const int j0 = blockIdx.x * 64, m0 = blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  if (threadIdx.x < 64) {
+    // exp(-j 2 pi f_nc ta) of line m0 + t, ta = (m - na/2) / prf.
+    const double rev = a.rev_per_line * ((double)(m0 + (int)threadIdx.x) - 0.5 * (double)a.na);
+    float sn, cs;
+    sincospif(2.0f * (float)(rev - rint(rev)), &sn, &cs);
+    rot[threadIdx.x] = make_float2(cs, sn);
+  }
+  __syncthreads();
+  // Read lines m0 + ty + 4k, samples j0 + tx (coalesced along the echo line).
+#pragma unroll 4
+  for (int k = 0; k < 16; ++k) {
+    const int m = m0 + ty + 4 * k, j = j0 + tx;
+    const int mi = m - a.pad_az, ji = j - a.pad_rg;
+    float2 v = make_float2(0.0f, 0.0f);
+    if (mi >= 0 && mi < a.na_in && ji >= 0 && ji < a.nr_in) {
+      const float2 e = a.echo[(size_t)mi * a.nr_in + ji], r = rot[ty + 4 * k];
+      v = make_float2(e.x * r.x - e.y * r.y, e.x * r.y + e.y * r.x);
+    }
+    tile[ty + 4 * k][tx] = v;
+  }
+  __syncthreads();
+#pragma unroll 4
+  for (int k = 0; k < 16; ++k) {
+    const int j = j0 + ty + 4 * k, m = m0 + tx;
+    if (j < a.nr && m < a.na) a.out[(size_t)j * a.na + m] = tile[tx][ty + 4 * k];
   }
 }
 
@@ -735,9 +772,9 @@ sarbp_status sarbp_omega_k(sarbp_context* ctx, const sarbp_cf32* d_echo, int32_t
   if ((st = row_plan(wc, na, nr, &az_inv)) != SARBP_OK) return st;
   if ((st = work_buffer(wc, (size_t)na * nr)) != SARBP_OK) return st;
   float2* w = wc->work;
-  hipLaunchKernelGGL(wk_prep, dim3((unsigned)na), dim3(kWkThreads), 0, s, pa);
-  if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "wk_prep launch");
-  if ((e = launch_transpose(img, w, na, nr, s)) != hipSuccess) return hip_fail(e, "wk_transpose launch");
+  pa.out = w;
+  hipLaunchKernelGGL(wk_prep_t, dim3((unsigned)((nr + 63) / 64), (unsigned)((na + 63) / 64)), dim3(256), 0, s, pa);
+  if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "wk_prep_t launch");
   if ((st = exec_fft(az_fwd, w + (size_t)pad_rg * na, HIPFFT_FORWARD, s)) != SARBP_OK) return st;
   if ((e = launch_transpose(w, img, nr, na, s)) != hipSuccess) return hip_fail(e, "wk_transpose launch");
   RowArgs r{};
