@@ -490,6 +490,11 @@ struct WkCache {
   size_t cap = 0;
   int axes_nr = -1, axes_na = -1;  // split of the uploaded axes
   hipEvent_t done = nullptr;  // after the last sarbp_stolt_sinc kernel
+  // After the last sarbp_omega_k: the next call's stream waits for it, so
+  // calls on different streams do not overlap on the shared workspace and
+  // transform plans.
+  hipEvent_t wk_done = nullptr;
+  bool wk_pending = false;
 };
 
 void wk_cache_destroy(WkCache* wc) {
@@ -503,6 +508,7 @@ void wk_cache_destroy(WkCache* wc) {
   (void)hipFree(wc->d_axes);
   (void)hipHostFree(wc->h_axes);
   if (wc->done) (void)hipEventDestroy(wc->done);
+  if (wc->wk_done) (void)hipEventDestroy(wc->wk_done);
   delete wc;
 }
 
@@ -614,6 +620,10 @@ sarbp_status exec_fft(hipfftHandle h, float2* d, int dir, hipStream_t s) {
   return SARBP_OK;
 }
 
+// Enqueues the omega-K steps of sarbp_omega_k (arguments validated) on s.
+sarbp_status omega_k_enqueue(WkCache* wc, const sarbp_cf32* d_echo, int na_in, int nr_in, int na, int nr,
+                             const sarbp_wk_params* wp, sarbp_cf32* d_image, hipStream_t s);
+
 }  // namespace
 }  // namespace sarbp
 
@@ -716,6 +726,25 @@ sarbp_status sarbp_omega_k(sarbp_context* ctx, const sarbp_cf32* d_echo, int32_t
   WkCache* wc = nullptr;
   if ((st = wk_cache(ctx, &wc)) != SARBP_OK) return st;
   const hipStream_t s = static_cast<hipStream_t>(stream);
+  hipError_t e;
+  if (!wc->wk_done && (e = hipEventCreateWithFlags(&wc->wk_done, hipEventDisableTiming)) != hipSuccess)
+    return hip_fail(e, "hipEventCreate");
+  if (wc->wk_pending && (e = hipStreamWaitEvent(s, wc->wk_done, 0)) != hipSuccess)
+    return hip_fail(e, "hipStreamWaitEvent");
+  st = omega_k_enqueue(wc, d_echo, na_in, nr_in, na, nr, wp, d_image, s);
+  if ((e = hipEventRecord(wc->wk_done, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+  wc->wk_pending = true;
+  return st;
+}
+
+}  // extern "C"
+
+namespace sarbp {
+namespace {
+
+sarbp_status omega_k_enqueue(WkCache* wc, const sarbp_cf32* d_echo, int na_in, int nr_in, int na, int nr,
+                             const sarbp_wk_params* wp, sarbp_cf32* d_image, hipStream_t s) {
+  sarbp_status st;
   float2* img = reinterpret_cast<float2*>(d_image);
   const int pad_az = (na - na_in) / 2, pad_rg = (nr - nr_in) / 2;
 
@@ -789,4 +818,5 @@ sarbp_status sarbp_omega_k(sarbp_context* ctx, const sarbp_cf32* d_echo, int32_t
   return SARBP_OK;
 }
 
-}  // extern "C"
+}  // namespace
+}  // namespace sarbp

@@ -237,3 +237,28 @@ def test_omega_k_long_lines_fallback(ctx):
     ref = _oracle(echo, wp)
     assert got.shape == (86, 5166)
     assert _rel_l2(got, ref) < 2e-5
+
+
+def test_omega_k_calls_on_two_streams(ctx):
+    # Calls on one context share its workspace; issued back to back on two
+    # streams they are ordered on the device and both images are right.
+    torch = _torch()
+    from sar_cambrianaccelerationcard_amd import device
+    scene = O.WkScene(f_nc=60.0).resolved()
+    e1 = O.simulate(scene, [(194, 170)])
+    e2 = O.simulate(scene, [(294, 235)])
+    wp = scene.params()
+    ref1 = device.omega_k(ctx, e1, wp).cpu().numpy()
+    ref2 = device.omega_k(ctx, e2, wp).cpu().numpy()
+    d1, d2 = _dev(e1), _dev(e2)
+    na, nr = scene.padded
+    o1 = torch.empty((na, nr), dtype=torch.complex64, device="cuda:0")
+    o2 = torch.empty_like(o1)
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    torch.cuda.synchronize()
+    for _ in range(3):
+        ctx.omega_k(d1.data_ptr(), scene.na_in, scene.nr_in, wp, o1.data_ptr(), s1.cuda_stream)
+        ctx.omega_k(d2.data_ptr(), scene.na_in, scene.nr_in, wp, o2.data_ptr(), s2.cuda_stream)
+    torch.cuda.synchronize()
+    assert np.array_equal(o1.cpu().numpy(), ref1)
+    assert np.array_equal(o2.cpu().numpy(), ref2)
