@@ -1,7 +1,8 @@
-// sarbp_fft.h -- in-LDS power-of-two FFTs for one workgroup: radix-8
-// Stockham passes (a radix-2 or radix-4 pass first when log2 n is not a
-// multiple of 3), used by range compression (sarbp_range.hip) and by the
-// omega-K line transforms (sarbp_wk.hip).  Not part of the public ABI.
+// sarbp_fft.h -- in-LDS power-of-two FFTs for one workgroup: Stockham passes
+// of radix 8 (n/8 threads) or 16 (n/16 threads), after one smaller-radix pass
+// when log2 n is not a multiple of 3 or 4; used by range compression
+// (sarbp_range.hip) and by the omega-K line transforms (sarbp_wk.hip).  Not
+// part of the public ABI.
 //
 // Element i of the transform lives at LDS slot fft::slot<LG>(i): one pad word
 // per 16 for n <= 4096 (buffer fft::lds_words<LG>() complex64 words), an XOR
@@ -48,6 +49,39 @@ __device__ __forceinline__ void dft_r(float2* x) {
     x[1] = cadd(t1, t3);
     x[2] = csub(t0, t2);
     x[3] = csub(t1, t3);
+  } else if constexpr (R == 16) {
+    // 16 = 4 x 4: A[n1][k1] = DFT4_n2(x[n1 + 4 n2]) W16^(n1 k1), then
+    // y[k1 + 4 k2] = DFT4_n1(A[n1][k1]).
+    constexpr float c1 = 0.92387953251128676f, s1 = 0.38268343236508977f, h = 0.70710678118654752f;
+    float2 a[4][4];
+#pragma unroll
+    for (int n1 = 0; n1 < 4; ++n1) {
+      a[n1][0] = x[n1];
+      a[n1][1] = x[n1 + 4];
+      a[n1][2] = x[n1 + 8];
+      a[n1][3] = x[n1 + 12];
+      dft_r<4, INV>(a[n1]);
+    }
+    // W16^e = (cos(2 pi e/16), -+sin(2 pi e/16)) for e = n1 k1 in {1,2,3,4,6,9}.
+    const float sg = INV ? 1.0f : -1.0f;
+    const float2 w1 = make_float2(c1, sg * s1), w2 = make_float2(h, sg * h), w3 = make_float2(s1, sg * c1);
+    const float2 w6 = make_float2(-h, sg * h), w9 = make_float2(-c1, -sg * s1);
+    a[1][1] = cmul(a[1][1], w1);
+    a[1][2] = cmul(a[1][2], w2);
+    a[1][3] = cmul(a[1][3], w3);
+    a[2][1] = cmul(a[2][1], w2);
+    a[2][2] = mj(a[2][2]);  // W16^4
+    a[2][3] = cmul(a[2][3], w6);
+    a[3][1] = cmul(a[3][1], w3);
+    a[3][2] = cmul(a[3][2], w6);
+    a[3][3] = cmul(a[3][3], w9);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      float2 col[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+      dft_r<4, INV>(col);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) x[k1 + 4 * k2] = col[k2];
+    }
   } else {
     // 8 = 4 x 2: u_k = x_k + x_{k+4}, v_k = W8^k (x_k - x_{k+4}); the even
     // outputs are DFT4(u), the odd ones DFT4(v).
@@ -85,7 +119,7 @@ __device__ __forceinline__ void stockham_pass(float2* buf, const float2* __restr
   constexpr int kT = T;
   constexpr int NB = NF / R;
   constexpr int PER = (NB + kT - 1) / kT;
-  constexpr int LGR = R == 8 ? 3 : (R == 4 ? 2 : 1);
+  constexpr int LGR = R == 16 ? 4 : (R == 8 ? 3 : (R == 4 ? 2 : 1));
   const int tid = threadIdx.x;
   const int ns = 1 << lns;
   float2 y[PER][R];
@@ -100,17 +134,23 @@ __device__ __forceinline__ void stockham_pass(float2* buf, const float2* __restr
         float2 w[R];
         w[1] = tw[k << (LG - lns - LGR)];
         if (INV) w[1].y = -w[1].y;
-        // Powers by at most three products: w2 = w1^2, w3 = w2 w1, w4 = w2^2,
+        // Powers by few products: w2 = w1^2, w3 = w2 w1, w4 = w2^2,
         // w5..w7 = w4 w1..w3.
         if constexpr (R >= 4) {
           w[2] = cmul(w[1], w[1]);
           w[3] = cmul(w[2], w[1]);
         }
-        if constexpr (R == 8) {
+        if constexpr (R >= 8) {
           w[4] = cmul(w[2], w[2]);
           w[5] = cmul(w[4], w[1]);
           w[6] = cmul(w[4], w[2]);
           w[7] = cmul(w[4], w[3]);
+        }
+        if constexpr (R == 16) {
+          // w8 = w4^2, w9..w15 = w8 w1..w7: at most four products deep.
+          w[8] = cmul(w[4], w[4]);
+#pragma unroll
+          for (int q = 1; q < 8; ++q) w[8 + q] = cmul(w[8], w[q]);
         }
 #pragma unroll
         for (int r = 1; r < R; ++r) y[p][r] = cmul(y[p][r], w[r]);
@@ -139,6 +179,28 @@ __device__ __forceinline__ void stockham_pass(float2* buf, const float2* __restr
 template <int LG, int T, bool INV>
 __device__ __forceinline__ void fft_lds(float2* buf, const float2* __restrict__ tw) {
   int lns = 0;
+  if constexpr (T * 16 == (1 << LG)) {
+    // 16 points per thread: a radix 2, 4 or 8 pass when LG is not a multiple
+    // of 4, then radix-16 passes.
+    if constexpr (LG % 4 == 1) {
+      stockham_pass<LG, T, 2, INV>(buf, tw, 0);
+      lns = 1;
+    } else if constexpr (LG % 4 == 2) {
+      stockham_pass<LG, T, 4, INV>(buf, tw, 0);
+      lns = 2;
+    } else if constexpr (LG % 4 == 3) {
+      stockham_pass<LG, T, 8, INV>(buf, tw, 0);
+      lns = 3;
+    }
+    if constexpr (LG >= 4) {
+#pragma unroll
+      for (int pass = 0; pass < LG / 4; ++pass) {
+        stockham_pass<LG, T, 16, INV>(buf, tw, lns);
+        lns += 4;
+      }
+    }
+    return;
+  }
   if constexpr (LG % 3 == 1) {
     stockham_pass<LG, T, 2, INV>(buf, tw, 0);
     lns = 1;

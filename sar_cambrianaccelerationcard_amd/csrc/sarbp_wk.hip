@@ -238,7 +238,7 @@ struct RowArgs {
   const float2* chirp;  // [nr]
 };
 
-constexpr int rows_threads(int lg) { return (1 << lg) / 8; }
+constexpr int rows_threads(int lg) { return (1 << lg) / 16; }
 
 // Length-nr DFT of the line staged as buf[slot(n)] = z[n] conj(b[n]) (n < nr,
 // zeros above): on return buf[slot(k)] holds (z conj(b)) * b circularly
@@ -247,7 +247,7 @@ template <int LG>
 __device__ __forceinline__ void chirp_convolve(float2* buf, const RowArgs& r) {
   constexpr int NF = 1 << LG, T = rows_threads(LG), E = NF / T;
   fft::fft_lds<LG, T, false>(buf, r.tw);
-#pragma unroll
+#pragma unroll 4
   for (int e = 0; e < E; ++e) {
     const int i = threadIdx.x + e * T;
     buf[fft::slot<LG>(i)] = cmul(buf[fft::slot<LG>(i)], r.bh[i]);
@@ -258,7 +258,7 @@ __device__ __forceinline__ void chirp_convolve(float2* buf, const RowArgs& r) {
 
 // One workgroup per azimuth-frequency line m; nr <= K T, 2 nr - 1 <= 2^LG.
 template <int LG, int K, int P>
-__global__ __launch_bounds__(rows_threads(LG)) void wk_rows(RowArgs r) {
+__global__ __launch_bounds__(rows_threads(LG), 4) void wk_rows(RowArgs r) {
   constexpr int NF = 1 << LG, T = rows_threads(LG), E = NF / T;
   extern __shared__ float2 buf[];
   const StoltArgs& a = r.st;
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(rows_threads(LG)) void wk_rows(RowArgs r) {
   const int tid = threadIdx.x;
 
   // Forward range DFT.
-#pragma unroll
+#pragma unroll 4
   for (int e = 0; e < E; ++e) {
     const int i = tid + e * T;
     buf[fft::slot<LG>(i)] = i < nr ? cmul(a.in[base + i], conjf2(r.chirp[i])) : make_float2(0.0f, 0.0f);
@@ -307,7 +307,7 @@ __global__ __launch_bounds__(rows_threads(LG)) void wk_rows(RowArgs r) {
     const int j = tid + t * T;
     if (j < nr) buf[fft::slot<LG>(j)] = conjf2(cmul(y[t], r.chirp[j]));
   }
-#pragma unroll
+#pragma unroll 4
   for (int e = 0; e < E; ++e) {
     const int i = tid + e * T;
     if (i >= nr) buf[fft::slot<LG>(i)] = make_float2(0.0f, 0.0f);
@@ -418,7 +418,7 @@ int rows_lg(int nr) {
 
 template <int LG, int P>
 void launch_rows_lp(const RowArgs& r, hipStream_t s) {
-  hipLaunchKernelGGL((wk_rows<LG, 4, P>), dim3((unsigned)r.st.na), dim3(rows_threads(LG)),
+  hipLaunchKernelGGL((wk_rows<LG, 8, P>), dim3((unsigned)r.st.na), dim3(rows_threads(LG)),
                      (size_t)fft::lds_words<LG>() * sizeof(float2), s, r);
 }
 
