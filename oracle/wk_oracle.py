@@ -62,11 +62,14 @@ def axes(na: int, nr: int, fs: float, prf: float, f_nc: float) -> tuple:
 
 
 def centres(fr_axis: np.ndarray, fa_axis: np.ndarray, f0: float, c: float, vr: float, fs: float) -> np.ndarray:
-    """c[m, n] = n - (fr_new[m, n] - fr[n]) Nr / Fr."""
+    """c[m, n] = n - (fr_new[m, n] - fr[n]) Nr / Fr, evaluated as n - (s - f) Nr/Fr with
+    f = f0 + fr and s = sqrt(max(f^2 - c^2 fa^2 / (4 Vr^2), 0)) (fr_new - fr = s - f): at fa = 0
+    the centres are then exactly the integers n, where the floor window's taps jump."""
     nr = fr_axis.size
     k = c * c / (4.0 * vr * vr)
-    fr_new = np.sqrt(np.maximum((f0 + fr_axis[None, :]) ** 2 - k * fa_axis[:, None] ** 2, 0.0)) - f0
-    return np.arange(nr)[None, :] - (fr_new - fr_axis[None, :]) * (nr / fs)
+    f = f0 + fr_axis[None, :]
+    s = np.sqrt(np.maximum(f * f - k * fa_axis[:, None] ** 2, 0.0))
+    return np.arange(nr)[None, :] - (s - f) * (nr / fs)
 
 
 def stolt_rows(S: np.ndarray, cen: np.ndarray, taps: int = 6, window: int = FLOOR) -> np.ndarray:

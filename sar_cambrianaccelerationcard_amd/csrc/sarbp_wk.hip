@@ -50,7 +50,7 @@ namespace {
 constexpr int kWkThreads = 256;
 constexpr int kWkMaxRg = 8192;  // a line of complex64 in 64 KiB of LDS
 constexpr int kWkMaxTaps = 16;
-constexpr int kRowsMaxRg = 4096;  // fused line kernel: 2 nr - 1 <= 8192 points
+constexpr int kRowsMaxRg = 8192;  // fused line kernel: 2 nr - 1 <= 16384 points (128 KiB of LDS)
 constexpr double kPiD = 3.14159265358979323846;
 
 using fft::cmul;
@@ -409,7 +409,7 @@ __global__ __launch_bounds__(256) void wk_prep_t(PrepArgs a) {
   }
 }
 
-// Smallest LG in [11, 13] with 2 nr - 1 <= 2^LG (nr <= kRowsMaxRg).
+// Smallest LG in [11, 14] with 2 nr - 1 <= 2^LG (nr <= kRowsMaxRg).
 int rows_lg(int nr) {
   int lg = 11;
   while ((1 << lg) < 2 * nr - 1) ++lg;
@@ -417,20 +417,28 @@ int rows_lg(int nr) {
 }
 
 template <int LG, int P>
-void launch_rows_lp(const RowArgs& r, hipStream_t s) {
-  hipLaunchKernelGGL((wk_rows<LG, 8, P>), dim3((unsigned)r.st.na), dim3(rows_threads(LG)),
-                     (size_t)fft::lds_words<LG>() * sizeof(float2), s, r);
+hipError_t launch_rows_lp(const RowArgs& r, hipStream_t s) {
+  constexpr size_t lds = (size_t)fft::lds_words<LG>() * sizeof(float2);
+  if constexpr (lds > 64 * 1024) {
+    // Above 64 KiB of dynamic LDS the kernel has to ask for it (the device
+    // allows 160 KiB per workgroup).
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&wk_rows<LG, 8, P>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL((wk_rows<LG, 8, P>), dim3((unsigned)r.st.na), dim3(rows_threads(LG)), lds, s, r);
+  return hipGetLastError();
 }
 
 hipError_t launch_rows(const RowArgs& r, int lg, hipStream_t s) {
   const bool p6 = r.st.taps == 6;
   switch (lg) {
-    case 11: p6 ? launch_rows_lp<11, 6>(r, s) : launch_rows_lp<11, 0>(r, s); break;
-    case 12: p6 ? launch_rows_lp<12, 6>(r, s) : launch_rows_lp<12, 0>(r, s); break;
-    case 13: p6 ? launch_rows_lp<13, 6>(r, s) : launch_rows_lp<13, 0>(r, s); break;
+    case 11: return p6 ? launch_rows_lp<11, 6>(r, s) : launch_rows_lp<11, 0>(r, s);
+    case 12: return p6 ? launch_rows_lp<12, 6>(r, s) : launch_rows_lp<12, 0>(r, s);
+    case 13: return p6 ? launch_rows_lp<13, 6>(r, s) : launch_rows_lp<13, 0>(r, s);
+    case 14: return p6 ? launch_rows_lp<14, 6>(r, s) : launch_rows_lp<14, 0>(r, s);
     default: return hipErrorInvalidValue;
   }
-  return hipGetLastError();
 }
 
 hipError_t launch_transpose(const float2* in, float2* out, int rows, int cols, hipStream_t s) {
@@ -778,8 +786,17 @@ sarbp_status omega_k_enqueue(WkCache* wc, const sarbp_cf32* d_echo, int na_in, i
   a.scale = (float)(1.0 / ((double)na * (double)nr));
   hipError_t e;
 
-  if (nr > kRowsMaxRg) {
-    // Long lines: hipFFT's 2-D plan and the Stolt kernel on its own.
+  const int lg = rows_lg(nr);
+  bool fused = nr <= kRowsMaxRg;
+  if (fused && fft::lds_words<14>() * sizeof(float2) > 64 * 1024 && lg == 14) {
+    // 128 KiB of LDS per workgroup has to be granted up front.
+    fused = hipFuncSetAttribute(reinterpret_cast<const void*>(&wk_rows<14, 8, 6>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024) == hipSuccess &&
+            hipFuncSetAttribute(reinterpret_cast<const void*>(&wk_rows<14, 8, 0>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024) == hipSuccess;
+  }
+  if (!fused) {
+    // hipFFT's 2-D plan and the Stolt kernel on its own.
     hipfftHandle plan = nullptr;
     if ((st = fft_plan(wc, na, nr, &plan)) != SARBP_OK) return st;
     hipLaunchKernelGGL(wk_prep, dim3((unsigned)na), dim3(kWkThreads), 0, s, pa);
@@ -793,7 +810,6 @@ sarbp_status omega_k_enqueue(WkCache* wc, const sarbp_cf32* d_echo, int na_in, i
   // columns holding echo are non-zero), one fused kernel per line for the
   // range DFT / reference function / Stolt / range inverse DFT, azimuth
   // inverse FFT.
-  const int lg = rows_lg(nr);
   const float2* tab = nullptr;
   hipfftHandle az_fwd = nullptr, az_inv = nullptr;
   if ((st = chirp_tables(wc, nr, lg, &tab)) != SARBP_OK) return st;

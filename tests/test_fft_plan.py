@@ -2,7 +2,7 @@
 emulated on the CPU: the Stockham pass plans (radix 8 on n/8 threads, radix
 16 on n/16 threads, one smaller pass first) and their twiddle indexing give
 the DFT, and the LDS slot maps (one pad word per 16 up to 4096 points, an XOR
-swizzle at 8192) keep each 16-lane access of every pass within at most two
+swizzle at 8192 and 16384) keep each 16-lane access of every pass within at most two
 lanes per bank pair.  The device kernels themselves are checked on the GPU
 through range compression and omega-K parity."""
 import numpy as np
@@ -45,8 +45,7 @@ def stockham(a, radices, inv=False):
     return np.array([buf[slot(i, lg)] for i in range(n)])
 
 
-@pytest.mark.parametrize("lg", range(1, 14))
-@pytest.mark.parametrize("radix", [8, 16])
+@pytest.mark.parametrize("lg,radix", [(lg, r) for lg in range(1, 14) for r in (8, 16)] + [(14, 16)])
 def test_plans_compute_the_dft(lg, radix):
     r = np.random.default_rng(lg)
     a = r.standard_normal(1 << lg) + 1j * r.standard_normal(1 << lg)
@@ -82,10 +81,10 @@ def _pass_conflicts(lg, radix, mapf):
     return worst
 
 
-@pytest.mark.parametrize("lg,radix", [(12, 8), (13, 16), (13, 8)])
+@pytest.mark.parametrize("lg,radix", [(12, 8), (13, 16), (13, 8), (14, 16)])
 def test_slot_maps_spread_the_stockham_strides(lg, radix):
     mapped = _pass_conflicts(lg, radix, lambda i: slot(i, lg))
     assert all(w <= 2 and rd <= 2 for _, _, w, rd in mapped), mapped
-    # Unmapped, some pass writes 8 lanes or more into one bank pair.
+    # Unmapped, some pass writes 4 lanes or more into one bank pair.
     plain = _pass_conflicts(lg, radix, lambda i: i)
-    assert max(w for _, _, w, _ in plain) >= 8, plain
+    assert max(w for _, _, w, _ in plain) >= 4, plain

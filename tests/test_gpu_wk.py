@@ -262,3 +262,15 @@ def test_omega_k_calls_on_two_streams(ctx):
     torch.cuda.synchronize()
     assert np.array_equal(o1.cpu().numpy(), ref1)
     assert np.array_equal(o2.cpu().numpy(), ref2)
+
+
+@pytest.mark.parametrize("na_in,nr_in", [(1, 1), (2, 3), (3, 2), (8, 2458), (8, 2459)])
+def test_omega_k_edge_sizes(ctx, na_in, nr_in):
+    # Single samples, tiny lines, and padded lines of exactly 4096 samples (the
+    # longest the fused line kernel takes) and 4099 (the 2-D-plan path).
+    echo = _rand((na_in, nr_in), na_in * 7 + nr_in)
+    wp = O.WkScene(na_in=na_in, nr_in=nr_in).params()
+    got = _omega_k(ctx, echo, wp)
+    ref = _oracle(echo, wp)
+    assert got.shape == W.padded_size(na_in, nr_in)[:2]
+    assert _rel_l2(got, ref) < 2e-5
