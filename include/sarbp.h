@@ -496,10 +496,11 @@ sarbp_status sarbp_wk_padded_size(int32_t na_in, int32_t nr_in, int32_t* na, int
  * exp(-j 2 pi f_nc ta), ta[m] = (m - na/2) / prf (preprocess.cc:17-166); 2-D
  * FFT (fft.cc:18-198); reference function exp(j (4 pi r0/c s + pi fr^2 / kr))
  * (compression.cc:17-58); Stolt interpolation (above); 2-D inverse FFT scaled
- * by 1/(na nr) (fft.cc:200-359).  For nr <= 4096 the range DFT, reference
- * function, Stolt step and range inverse DFT run as one kernel per azimuth
- * line (chirp-z transforms in LDS) between hipFFT azimuth transforms; longer
- * lines use a hipFFT 2-D plan around the Stolt kernel.  Plans, chirp tables
+ * by 1/(na nr) (fft.cc:200-359).  The range DFT, reference function, Stolt
+ * step and range inverse DFT run as one kernel per azimuth line (chirp-z
+ * transforms in LDS; lines over 4096 samples need 128 KiB of LDS per
+ * workgroup, else SARBP_ERR_UNSUPPORTED) between hipFFT azimuth
+ * transforms.  Plans, chirp tables
  * and a transpose workspace of na x nr complex64 words are kept by the
  * context (the first call of a size builds them).  Calls on one context share
  * them and are ordered on the device: each call's stream first waits for the

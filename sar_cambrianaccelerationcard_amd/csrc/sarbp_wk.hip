@@ -12,9 +12,8 @@
 // scene pads to 3414 = 2 * 3 * 569), and it is written once.  The azimuth
 // transforms are power-of-two hipFFT row transforms between tiled transposes.
 // Kernels written here:
-//   wk_prep      one workgroup per padded azimuth line: zeros, or the echo
-//                sample times the line's down-conversion factor (FP64 phase);
-//                wk_prep_t the same written transposed ([nr][na]) for the
+//   wk_prep_t    zeros, or the echo sample times its line's down-conversion
+//                factor (FP64 phase), written transposed ([nr][na]) for the
 //                azimuth transforms, in 64 x 64 tiles through LDS.
 //   wk_transpose 64 x 64 tiles through LDS.
 //   wk_rows      per line: stage x[n] conj(b[n]) (b[n] = exp(j pi n^2 / nr)),
@@ -23,8 +22,7 @@
 //                function; Stolt taps into registers; then the same chirp-z
 //                steps on conj(Y[n] b[n]) give the inverse DFT.
 //   wk_stolt     the Stolt step alone on a line staged in LDS (the
-//                sarbp_stolt_sinc entry point, and omega-K for nr > 4096
-//                with hipFFT's 2-D plan).
+//                sarbp_stolt_sinc entry point).
 // Per sample one FP64 sqrt s = sqrt(max(0, (f0+fr)^2 - c^2 fa^2/(4 vr^2)))
 // gives both the reference phase (revolutions 2 r0/c s + fr^2/(2 kr), reduced
 // in FP64) and the Stolt centre n - (s - f0 - fr) nr/fs.
@@ -345,37 +343,13 @@ __global__ __launch_bounds__(256) void wk_transpose(const float2* __restrict__ i
 
 struct PrepArgs {
   const float2* echo;  // [na_in][nr_in]
-  float2* out;         // [na][nr]
+  float2* out;         // [nr][na], transposed
   int na, nr, na_in, nr_in, pad_az, pad_rg;
   double rev_per_line;  // -f_nc / prf: down-conversion phase per line, revolutions
 };
 
-__global__ __launch_bounds__(kWkThreads) void wk_prep(PrepArgs a) {
-  const int m = blockIdx.x;
-  const int mi = m - a.pad_az;
-  float2* dst = a.out + (size_t)m * (size_t)a.nr;
-  if (mi < 0 || mi >= a.na_in) {
-    for (int j = threadIdx.x; j < a.nr; j += kWkThreads) dst[j] = make_float2(0.0f, 0.0f);
-    return;
-  }
-  // exp(-j 2 pi f_nc ta), ta = (m - na/2) / prf.
-  const double rev = a.rev_per_line * ((double)m - 0.5 * (double)a.na);
-  float sn, cs;
-  sincospif(2.0f * (float)(rev - rint(rev)), &sn, &cs);
-  const float2* src = a.echo + (size_t)mi * (size_t)a.nr_in;
-  for (int j = threadIdx.x; j < a.nr; j += kWkThreads) {
-    const int ji = j - a.pad_rg;
-    float2 v = make_float2(0.0f, 0.0f);
-    if (ji >= 0 && ji < a.nr_in) {
-      const float2 e = src[ji];
-      v = make_float2(e.x * cs - e.y * sn, e.x * sn + e.y * cs);
-    }
-    dst[j] = v;
-  }
-}
-
-// wk_prep written transposed: out[j][m] (= [nr][na]) for 64 x 64 tiles of
-// (range sample j, padded line m), through LDS.
+// Padding and azimuth down-conversion, written transposed: out[j][m] (=
+// [nr][na]) for 64 x 64 tiles of (range sample j, padded line m), through LDS.
 __global__ __launch_bounds__(256) void wk_prep_t(PrepArgs a) {
   __shared__ float2 tile[64][65];
   __shared__ float2 rot[64];
@@ -488,7 +462,6 @@ sarbp_status check_stolt(double f0, double c, double vr, int na, int nr, double 
 }  // namespace
 
 struct WkCache {
-  std::map<std::pair<int, int>, hipfftHandle> plans;      // 2-D (na, nr)
   std::map<std::pair<int, int>, hipfftHandle> row_plans;  // 1-D (length, batch)
   std::map<int, float2*> chirp_tables;                    // nr -> [tw | bh | chirp]
   float2* work = nullptr;  // transpose workspace of sarbp_omega_k
@@ -509,7 +482,6 @@ void wk_cache_destroy(WkCache* wc) {
   if (!wc) return;
   if (wc->done) (void)hipEventSynchronize(wc->done);
   (void)hipDeviceSynchronize();  // the work buffer and tables may still be in use
-  for (auto& kv : wc->plans) (void)hipfftDestroy(kv.second);
   for (auto& kv : wc->row_plans) (void)hipfftDestroy(kv.second);
   for (auto& kv : wc->chirp_tables) (void)hipFree(kv.second);
   (void)hipFree(wc->work);
@@ -528,23 +500,6 @@ sarbp_status wk_cache(sarbp_context* ctx, WkCache** out) {
     if (!ctx->wk) return fail(SARBP_ERR_OUT_OF_MEMORY, "omega-K cache");
   }
   *out = ctx->wk;
-  return SARBP_OK;
-}
-
-sarbp_status fft_plan(WkCache* wc, int na, int nr, hipfftHandle* out) {
-  const auto key = std::make_pair(na, nr);
-  auto it = wc->plans.find(key);
-  if (it != wc->plans.end()) {
-    *out = it->second;
-    return SARBP_OK;
-  }
-  hipfftHandle h = nullptr;
-  const hipfftResult r = hipfftPlan2d(&h, na, nr, HIPFFT_C2C);
-  if (r != HIPFFT_SUCCESS)
-    return fail(r == HIPFFT_ALLOC_FAILED ? SARBP_ERR_OUT_OF_MEMORY : SARBP_ERR_DEVICE,
-                "hipfftPlan2d(%d, %d) failed (%d)", na, nr, (int)r);
-  wc->plans[key] = h;
-  *out = h;
   return SARBP_OK;
 }
 
@@ -758,7 +713,6 @@ sarbp_status omega_k_enqueue(WkCache* wc, const sarbp_cf32* d_echo, int na_in, i
 
   PrepArgs pa{};
   pa.echo = reinterpret_cast<const float2*>(d_echo);
-  pa.out = img;
   pa.na = na;
   pa.nr = nr;
   pa.na_in = na_in;
@@ -787,23 +741,14 @@ sarbp_status omega_k_enqueue(WkCache* wc, const sarbp_cf32* d_echo, int na_in, i
   hipError_t e;
 
   const int lg = rows_lg(nr);
-  bool fused = nr <= kRowsMaxRg;
-  if (fused && fft::lds_words<14>() * sizeof(float2) > 64 * 1024 && lg == 14) {
+  if (lg == 14) {
     // 128 KiB of LDS per workgroup has to be granted up front.
-    fused = hipFuncSetAttribute(reinterpret_cast<const void*>(&wk_rows<14, 8, 6>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024) == hipSuccess &&
-            hipFuncSetAttribute(reinterpret_cast<const void*>(&wk_rows<14, 8, 0>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024) == hipSuccess;
-  }
-  if (!fused) {
-    // hipFFT's 2-D plan and the Stolt kernel on its own.
-    hipfftHandle plan = nullptr;
-    if ((st = fft_plan(wc, na, nr, &plan)) != SARBP_OK) return st;
-    hipLaunchKernelGGL(wk_prep, dim3((unsigned)na), dim3(kWkThreads), 0, s, pa);
-    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "wk_prep launch");
-    if ((st = exec_fft(plan, img, HIPFFT_FORWARD, s)) != SARBP_OK) return st;
-    if ((e = launch_stolt(a, s)) != hipSuccess) return hip_fail(e, "wk_stolt launch");
-    return exec_fft(plan, img, HIPFFT_BACKWARD, s);
+    for (const void* k : {reinterpret_cast<const void*>(&wk_rows<14, 8, 6>),
+                          reinterpret_cast<const void*>(&wk_rows<14, 8, 0>)}) {
+      if ((e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024)) != hipSuccess)
+        return fail(SARBP_ERR_UNSUPPORTED, "lines of %d samples need 128 KiB of LDS per workgroup: %s", nr,
+                    hipGetErrorString(e));
+    }
   }
 
   // Azimuth FFT (columns, through the transposed workspace; only the nr_in

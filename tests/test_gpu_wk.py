@@ -227,9 +227,9 @@ def test_bp_and_omega_k_image_the_same_scene(ctx, f_nc):
     assert mw["concentration"] < 0.15, mw
 
 
-def test_omega_k_long_lines_fallback(ctx):
-    # Padded lines longer than 4096 samples take the 2-D hipFFT plan and the
-    # Stolt kernel on its own; same oracle, same tolerance.
+def test_omega_k_long_lines(ctx):
+    # Padded lines longer than 4096 samples run the line kernel on
+    # 16384-point transforms (128 KiB of LDS); same oracle, same tolerance.
     echo = _rand((64, 3100), 11)
     scene = O.WkScene(na_in=64, nr_in=3100)
     wp = scene.params()
