@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define SARBP_ABI_VERSION 3
+#define SARBP_ABI_VERSION 4
 
 typedef enum sarbp_status {
   SARBP_OK = 0,
@@ -471,6 +471,25 @@ sarbp_status sarbp_stolt_sinc(sarbp_context* ctx, const sarbp_cf32* d_in, sarbp_
                               double vr, int32_t na, int32_t nr, double fs, int32_t taps,
                               int32_t window, void* stream);
 
+/* Range reference of sarbp_omega_k's reference function. */
+typedef enum sarbp_wk_range_ref {
+  /* The reference's pipeline (compression.cc:17-58): the range spectrum keeps
+   * the DFT's time origin at padded sample 0 and the reference function uses
+   * r0 as given.  It focuses a target only when the two-way time of padded
+   * sample 0 is a whole number of padded range windows (Nr / fs) after
+   * 2 r0 / c and the target lies within Nr/2 samples after padded sample 0;
+   * a target in the far half of the padded window defocuses. */
+  SARBP_WK_RANGE_REF_R0 = 0,
+  /* No counterpart in the reference: r0 is the slant range of raw echo sample
+   * 0 (padded sample 0 at fast time ts = 2 r0/c - pad/fs, pad the leading
+   * zero samples).  The range spectrum is moved to absolute fast time (times
+   * exp(-j 2 pi fr ts)), the reference function uses the range of padded
+   * sample floor(Nr/2), and after the Stolt step the origin goes back to
+   * padded sample 0 (times exp(-j 2 pi fr floor(Nr/2) / fs)): every target in
+   * the padded window focuses, at its own padded range sample, for any r0. */
+  SARBP_WK_RANGE_REF_CENTRE = 1
+} sarbp_wk_range_ref;
+
 typedef struct sarbp_wk_params {
   double c;       /* propagation speed, m/s                                   */
   double f0;      /* carrier frequency, Hz                                    */
@@ -483,6 +502,8 @@ typedef struct sarbp_wk_params {
   double f_nc;    /* Doppler centroid, Hz                                     */
   int32_t taps;   /* Stolt taps P (6 in the reference, wk.cc:273)             */
   int32_t window; /* sarbp_stolt_window                                       */
+  int32_t range_ref; /* sarbp_wk_range_ref (SARBP_WK_RANGE_REF_R0: reference) */
+  int32_t reserved;  /* must be 0                                             */
 } sarbp_wk_params;
 
 /* Padded size of an na_in x nr_in echo: round(na_in/6) zero lines before and
@@ -495,7 +516,8 @@ sarbp_status sarbp_wk_padded_size(int32_t na_in, int32_t nr_in, int32_t* na, int
  * of process_wk (wk.cc:87-285): zero padding and azimuth down-conversion by
  * exp(-j 2 pi f_nc ta), ta[m] = (m - na/2) / prf (preprocess.cc:17-166); 2-D
  * FFT (fft.cc:18-198); reference function exp(j (4 pi r0/c s + pi fr^2 / kr))
- * (compression.cc:17-58); Stolt interpolation (above); 2-D inverse FFT scaled
+ * (compression.cc:17-58; wp->range_ref selects the range origin, above);
+ * Stolt interpolation (above); 2-D inverse FFT scaled
  * by 1/(na nr) (fft.cc:200-359).  The range DFT, reference function, Stolt
  * step and range inverse DFT run as one kernel per azimuth line (chirp-z
  * transforms in LDS; lines over 4096 samples need 128 KiB of LDS per

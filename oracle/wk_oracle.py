@@ -123,13 +123,42 @@ def pad_downconvert(echo: np.ndarray, *, prf: float, f_nc: float) -> np.ndarray:
     return x * np.exp(-2j * np.pi * f_nc * ta)[:, None]
 
 
+REF_R0, REF_CENTRE = 0, 1
+
+
+def time_shift(fr: np.ndarray, t: float) -> np.ndarray:
+    """exp(-j 2 pi fr t): moves a range spectrum's time origin t later."""
+    return np.exp(-2j * np.pi * fr * t)
+
+
 def omega_k(echo: np.ndarray, *, c: float, f0: float, fs: float, prf: float, kr: float, vr: float,
-            r0: float, f_nc: float, taps: int = 6, window: int = FLOOR) -> np.ndarray:
-    """echo [na_in][nr_in] -> complex128 image [Na][Nr] (padded size, ifft2 output)."""
+            r0: float, f_nc: float, taps: int = 6, window: int = FLOOR, range_ref: int = REF_R0) -> np.ndarray:
+    """echo [na_in][nr_in] -> complex128 image [Na][Nr] (padded size, ifft2 output).
+
+    range_ref REF_R0 is the reference's pipeline: the range spectrum keeps the
+    DFT's time origin at padded sample 0 and the reference function uses r0.
+    It focuses a target only if the fast time of padded sample 0 is a whole
+    number of range windows (Nr/Fr) and the target lies in the first half of
+    the window.  REF_CENTRE (this library's option, not in the reference)
+    takes r0 as the slant range of raw sample 0 (padded sample 0 at
+    ts = 2 r0/c - pr/Fr): the spectrum is moved to absolute fast time (times
+    exp(-j 2 pi fr ts)), the reference range is that of padded sample
+    h = floor(Nr/2), and after the Stolt step the origin is moved to padded
+    sample 0 (times exp(-j 2 pi fr h/Fr)), so every range in the window is
+    within Nr/2 samples of the reference and the image columns are padded
+    range samples."""
     x = pad_downconvert(np.asarray(echo, dtype=np.complex128), prf=prf, f_nc=f_nc)
     na, nr = x.shape
     fr, fa, _ = axes(na, nr, fs, prf, f_nc)
     S = np.fft.fft2(x)
-    S *= np.exp(1j * reference_phase(fr, fa, f0=f0, c=c, vr=vr, kr=kr, r0=r0))
+    r_ref = r0
+    if range_ref == REF_CENTRE:
+        h = nr // 2
+        ts = 2.0 * r0 / c - padded_size(*echo.shape)[3] / fs
+        r_ref = 0.5 * c * (ts + h / fs)
+        S *= time_shift(fr, ts)[None, :]
+    S *= np.exp(1j * reference_phase(fr, fa, f0=f0, c=c, vr=vr, kr=kr, r0=r_ref))
     S = stolt_sinc(S, fr, fa, f0=f0, c=c, vr=vr, fs=fs, taps=taps, window=window)
+    if range_ref == REF_CENTRE:
+        S *= time_shift(fr, (nr // 2) / fs)[None, :]
     return np.fft.ifft2(S)

@@ -24,8 +24,9 @@
 //   wk_stolt     the Stolt step alone on a line staged in LDS (the
 //                sarbp_stolt_sinc entry point).
 // Per sample one FP64 sqrt s = sqrt(max(0, (f0+fr)^2 - c^2 fa^2/(4 vr^2)))
-// gives both the reference phase (revolutions 2 r0/c s + fr^2/(2 kr), reduced
-// in FP64) and the Stolt centre n - (s - f0 - fr) nr/fs.
+// gives both the reference phase (revolutions 2 r0/c s + fr^2/(2 kr), plus a
+// linear fr term for the window-centred range reference; reduced in FP64) and
+// the Stolt centre n - (s - f0 - fr) nr/fs.
 #include "sarbp.h"
 #include "sarbp_fft.h"
 #include "sarbp_host.h"
@@ -69,6 +70,7 @@ struct StoltArgs {
   int rfm;            // 1: multiply by the reference function while staging
   double rfm_a;       // 2 r0 / c: s term of the reference phase, revolutions
   double rfm_b;       // 1 / (2 kr): fr^2 term, revolutions
+  double rfm_c;       // fr term of the reference phase, revolutions per Hz
   float scale;        // applied to every output sample
 };
 
@@ -99,21 +101,27 @@ __device__ __forceinline__ double line_kfa2(const StoltArgs& a, int m) {
   return a.kk * fa * fa;
 }
 
+// Range frequency of sample j, Hz.
+__device__ __forceinline__ double line_fr(const StoltArgs& a, int j) {
+  return a.fr_axis ? a.fr_axis[j] : shifted_bin(j, a.nr) * a.fr_gap;
+}
+
+// v exp(j 2 pi rev): the phase reduced to [-1/2, 1/2] revolutions in FP64;
+// the hardware sine and cosine take revolutions (absolute error ~1.2e-7 there).
+__device__ __forceinline__ float2 rotate_rev(float2 v, double rev) {
+  const float ph = (float)(rev - rint(rev));
+  const float sn = __builtin_amdgcn_sinf(ph), cs = __builtin_amdgcn_cosf(ph);
+  return make_float2(v.x * cs - v.y * sn, v.x * sn + v.y * cs);
+}
+
 // Range-frequency sample j of a line: applies the reference function to v
 // (if a.rfm) and returns the Stolt centre as (c0, f), c0 + f = centre.
 __device__ __forceinline__ float2 stolt_sample(const StoltArgs& a, int j, double kfa2, float2 v, int* c0,
                                                float* f) {
-  const double fr = a.fr_axis ? a.fr_axis[j] : shifted_bin(j, a.nr) * a.fr_gap;
+  const double fr = line_fr(a, j);
   const double fv = a.f0 + fr;
   const double s = sqrt(fmax(fma(fv, fv, -kfa2), 0.0));
-  if (a.rfm) {
-    // Phase in revolutions, reduced to [-1/2, 1/2]: the hardware sine and
-    // cosine take revolutions (absolute error ~1.2e-7 there).
-    const double rev = fma(a.rfm_a, s, a.rfm_b * fr * fr);
-    const float ph = (float)(rev - rint(rev));
-    const float sn = __builtin_amdgcn_sinf(ph), cs = __builtin_amdgcn_cosf(ph);
-    v = make_float2(v.x * cs - v.y * sn, v.x * sn + v.y * cs);
-  }
+  if (a.rfm) v = rotate_rev(v, fma(a.rfm_a, s, (a.rfm_b * fr + a.rfm_c) * fr));
   const double cen = (double)j - (s - fv) * a.nr_over_fs;
   const double fl = floor(cen);
   *c0 = (int)fl;
@@ -229,11 +237,17 @@ __global__ __launch_bounds__(kWkThreads) void wk_stolt(StoltArgs a) {
 // Chirp-z tables of one (nr, LG): b[n] = exp(j pi n^2 / nr) for n < nr;
 // bh[k] = FFT_M(bb)[k] / M with bb[n] = b[|n|] at n and M - n (|n| < nr),
 // M = 2^LG >= 2 nr - 1; tw[t] = exp(-j 2 pi t / M).
+// Output sample n of a line goes to (n + out_shift) mod nr, times out_rot
+// (which includes the 1/(na nr) scale): a circular shift with a constant
+// phase is the range-frequency ramp exp(-j 2 pi fr h / fs) of the centred
+// range reference on the fftshift axis.
 struct RowArgs {
   StoltArgs st;       // in = out = the line array [na][nr]
   const float2* tw;   // [M]
   const float2* bh;   // [M]
   const float2* chirp;  // [nr]
+  int out_shift;      // 0 <= out_shift < nr
+  float2 out_rot;
 };
 
 constexpr int rows_threads(int lg) { return (1 << lg) / 16; }
@@ -317,7 +331,9 @@ __global__ __launch_bounds__(rows_threads(LG), 4) void wk_rows(RowArgs r) {
     const int j = tid + t * T;
     if (j < nr) {
       const float2 v = cmul(r.chirp[j], conjf2(buf[fft::slot<LG>(j)]));
-      a.out[base + j] = make_float2(v.x * a.scale, v.y * a.scale);
+      int o = j + r.out_shift;
+      if (o >= nr) o -= nr;
+      a.out[base + o] = cmul(v, r.out_rot);
     }
   }
 }
@@ -681,6 +697,8 @@ sarbp_status sarbp_omega_k(sarbp_context* ctx, const sarbp_cf32* d_echo, int32_t
   if (!(std::isfinite(wp->prf) && wp->prf > 0.0 && std::isfinite(wp->kr) && wp->kr != 0.0 &&
         std::isfinite(wp->r0) && std::isfinite(wp->f_nc)))
     return fail(SARBP_ERR_INVALID_ARGUMENT, "prf, kr, r0 or f_nc invalid");
+  if ((wp->range_ref != SARBP_WK_RANGE_REF_R0 && wp->range_ref != SARBP_WK_RANGE_REF_CENTRE) || wp->reserved != 0)
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "range_ref %d / reserved %d invalid", wp->range_ref, wp->reserved);
   if (na_in == 0 || nr_in == 0) return fail(SARBP_ERR_INVALID_ARGUMENT, "empty echo");
   if (!d_echo || !d_image) return fail(SARBP_ERR_INVALID_ARGUMENT, "echo or image pointer is NULL");
   std::lock_guard<std::mutex> lk(ctx->mu);
@@ -738,6 +756,24 @@ sarbp_status omega_k_enqueue(WkCache* wc, const sarbp_cf32* d_echo, int na_in, i
   a.rfm_a = 2.0 * wp->r0 / wp->c;
   a.rfm_b = 0.5 / wp->kr;
   a.scale = (float)(1.0 / ((double)na * (double)nr));
+  int out_shift = 0;
+  double out_phase = 0.0;  // radians
+  if (wp->range_ref == SARBP_WK_RANGE_REF_CENTRE) {
+    // Padded sample 0 at fast time ts: the spectrum goes to absolute time
+    // (exp(-j 2 pi fr ts), folded into the reference phase), the reference
+    // range is that of padded sample h, and after the Stolt step the origin
+    // returns to padded sample 0 by exp(-j 2 pi fr h / fs).  On the axis
+    // fr[k] = (((k + ceil(nr/2)) mod nr) - nr/2) fs/nr that ramp is
+    // W^(k h) exp(j pi h) W^(ceil(nr/2) h), W = exp(-j 2 pi / nr): a circular
+    // shift of the range line by h times a constant.
+    const double ts = 2.0 * wp->r0 / wp->c - (double)pad_rg / wp->fs;
+    const int h = nr / 2;
+    a.rfm_a = ts + (double)h / wp->fs;
+    a.rfm_c = -ts;
+    out_shift = h;
+    const long long ch = (long long)((nr + 1) / 2) * h % nr;
+    out_phase = kPiD * (double)(h % 2) - 2.0 * kPiD * (double)ch / (double)nr;
+  }
   hipError_t e;
 
   const int lg = rows_lg(nr);
@@ -772,6 +808,8 @@ sarbp_status omega_k_enqueue(WkCache* wc, const sarbp_cf32* d_echo, int na_in, i
   r.tw = tab;
   r.bh = tab + ((size_t)1 << lg);
   r.chirp = tab + ((size_t)2 << lg);
+  r.out_shift = out_shift;
+  r.out_rot = make_float2((float)(std::cos(out_phase) * a.scale), (float)(std::sin(out_phase) * a.scale));
   if ((e = launch_rows(r, lg, s)) != hipSuccess) return hip_fail(e, "wk_rows launch");
   if ((e = launch_transpose(img, w, na, nr, s)) != hipSuccess) return hip_fail(e, "wk_transpose launch");
   if ((st = exec_fft(az_inv, w, HIPFFT_BACKWARD, s)) != SARBP_OK) return st;

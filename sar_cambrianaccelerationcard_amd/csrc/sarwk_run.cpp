@@ -12,10 +12,13 @@
 //
 //   sarwk_run [--params data/paras.txt] [--echo data/echo.txt] [--na 3072] [--nr 2048]
 //             [--vr 7062] [--f-nc -6900] [--taps 6] [--window floor|centered]
-//             [--device 0] [--info | -i] [--out | -o [FILE]]
+//             [--range-ref r0|centre] [--device 0] [--info | -i] [--out | -o [FILE]]
 //
 // Vr and f_nc default to the driver's constants (wk.cc:62-64); Kr is taken
 // as stored in the parameter file (signed: negative for a down-chirp).
+// --range-ref centre (not in the reference) takes R0 as the slant range of
+// the first echo sample and references the window's centre, so targets in
+// the far half of the range window focus too.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -37,14 +40,15 @@ struct Options {
   std::string params = "data/paras.txt", echo = "data/echo.txt", out;
   int na = 3072, nr = 2048, taps = 6, device = 0;
   double vr = 7062.0, f_nc = -6900.0;
-  std::string window = "floor";
+  std::string window = "floor", range_ref = "r0";
   bool info = false, write = false;
 };
 
 void usage() {
   std::fprintf(stderr,
                "usage: sarwk_run [--params FILE] [--echo FILE] [--na N] [--nr N] [--vr V] [--f-nc F]\n"
-               "                 [--taps P] [--window floor|centered] [--device D] [--info|-i]\n"
+               "                 [--taps P] [--window floor|centered] [--range-ref r0|centre]\n"
+               "                 [--device D] [--info|-i]\n"
                "                 [--out|-o [FILE]]\n");
 }
 
@@ -63,6 +67,7 @@ bool parse(int argc, char** argv, Options* o) {
     else if (a == "--f-nc") o->f_nc = std::atof(next("--f-nc"));
     else if (a == "--taps") o->taps = std::atoi(next("--taps"));
     else if (a == "--window") o->window = next("--window");
+    else if (a == "--range-ref") o->range_ref = next("--range-ref");
     else if (a == "--device") o->device = std::atoi(next("--device"));
     else if (a == "--info" || a == "-i") o->info = true;
     else if (a == "--out" || a == "-o") {
@@ -74,6 +79,7 @@ bool parse(int argc, char** argv, Options* o) {
   if (o->out.empty()) o->out = "result.txt";
   if (o->na < 1 || o->nr < 1) throw std::runtime_error("--na and --nr must be positive");
   if (o->window != "floor" && o->window != "centered") throw std::runtime_error("--window floor|centered");
+  if (o->range_ref != "r0" && o->range_ref != "centre") throw std::runtime_error("--range-ref r0|centre");
   return true;
 }
 
@@ -155,9 +161,10 @@ void write_image(const std::string& path, const std::vector<sarbp_cf32>& img, in
                "\"nx\": %d, \"dtype\": \"complex64\", \"bytes\": %zu, \"meta\": {\"writer\": \"sarwk_run\", "
                "\"rows\": \"azimuth frequency lines after the inverse transform\", \"c\": %.10g, \"f0\": %.10g, "
                "\"fs\": %.10g, \"prf\": %.10g, \"kr\": %.10g, \"vr\": %.10g, \"r0\": %.10g, \"f_nc\": %.10g, "
-               "\"taps\": %d, \"window\": \"%s\"}}\n",
+               "\"taps\": %d, \"window\": \"%s\", \"range_ref\": \"%s\"}}\n",
                na, nr, img.size() * sizeof(sarbp_cf32), wp.c, wp.f0, wp.fs, wp.prf, wp.kr, wp.vr, wp.r0, wp.f_nc,
-               wp.taps, wp.window == SARBP_STOLT_FLOOR ? "floor" : "centered");
+               wp.taps, wp.window == SARBP_STOLT_FLOOR ? "floor" : "centered",
+               wp.range_ref == SARBP_WK_RANGE_REF_CENTRE ? "centre" : "r0");
   std::fclose(j);
 }
 
@@ -185,10 +192,12 @@ int main(int argc, char** argv) {
     wp.f_nc = o.f_nc;
     wp.taps = o.taps;
     wp.window = o.window == "floor" ? SARBP_STOLT_FLOOR : SARBP_STOLT_CENTERED;
+    wp.range_ref = o.range_ref == "centre" ? SARBP_WK_RANGE_REF_CENTRE : SARBP_WK_RANGE_REF_R0;
     if (o.info)
       std::printf("Fr %.6g Hz, PRF %.6g Hz, f0 %.6g Hz, R0 %.6f m, Kr %.6g Hz/s, c %.6g m/s, Vr %.6g m/s, "
-                  "f_nc %.6g Hz, P %d, window %s\n",
-                  wp.fs, wp.prf, wp.f0, wp.r0, wp.kr, wp.c, wp.vr, wp.f_nc, wp.taps, o.window.c_str());
+                  "f_nc %.6g Hz, P %d, window %s, range reference %s\n",
+                  wp.fs, wp.prf, wp.f0, wp.r0, wp.kr, wp.c, wp.vr, wp.f_nc, wp.taps, o.window.c_str(),
+                  o.range_ref.c_str());
     const auto echo = read_echo(o.echo);
     if (echo.size() != (size_t)o.na * o.nr)
       throw std::runtime_error("echo holds " + std::to_string(echo.size()) + " samples, expected " +

@@ -134,11 +134,13 @@ class RawRequest(C.Structure):
 
 class WkParams(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("c", "f0", "fs", "prf", "kr", "vr", "r0", "f_nc")] + \
-        [("taps", C.c_int32), ("window", C.c_int32)]
+        [("taps", C.c_int32), ("window", C.c_int32), ("range_ref", C.c_int32), ("reserved", C.c_int32)]
 
 
 STOLT_FLOOR, STOLT_CENTERED = 0, 1
 STOLT_WINDOWS = {"floor": STOLT_FLOOR, "centered": STOLT_CENTERED}
+WK_RANGE_REF_R0, WK_RANGE_REF_CENTRE = 0, 1
+WK_RANGE_REFS = {"r0": WK_RANGE_REF_R0, "centre": WK_RANGE_REF_CENTRE}
 
 JOB_QUEUED, JOB_RESIDENT, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = range(6)
 
@@ -273,13 +275,16 @@ def range_info(rp: RangeParams) -> tuple:
 
 
 def make_wk_params(*, c: float, f0: float, fs: float, prf: float, kr: float, vr: float, r0: float,
-                   f_nc: float, taps: int = 6, window="floor") -> WkParams:
-    """omega-K parameters; kr is the signed chirp rate (negative for a down-chirp)."""
+                   f_nc: float, taps: int = 6, window="floor", range_ref="r0") -> WkParams:
+    """omega-K parameters; kr is the signed chirp rate (negative for a down-chirp).
+    range_ref "r0" is the reference's range origin (r0 the reference range),
+    "centre" the window-centred one (r0 the slant range of raw sample 0)."""
     wp = WkParams()
     for k, v in dict(c=c, f0=f0, fs=fs, prf=prf, kr=kr, vr=vr, r0=r0, f_nc=f_nc).items():
         setattr(wp, k, float(v))
     wp.taps = int(taps)
     wp.window = STOLT_WINDOWS[window] if isinstance(window, str) else int(window)
+    wp.range_ref = WK_RANGE_REFS[range_ref] if isinstance(range_ref, str) else int(range_ref)
     return wp
 
 

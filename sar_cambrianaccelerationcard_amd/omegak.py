@@ -58,10 +58,19 @@ class WkScene:
         _, nr = self.padded
         return replace(self, r0=0.5 * self.c * nr / self.fs)
 
-    def params(self, taps: int = 6, window="floor") -> native.WkParams:
+    @property
+    def first_sample_range(self) -> float:
+        """Slant range of raw echo sample 0 (padded sample pad_rg)."""
         s = self.resolved()
-        return native.make_wk_params(c=s.c, f0=s.f0, fs=s.fs, prf=s.prf, kr=s.kr, vr=s.vr, r0=s.r0,
-                                     f_nc=s.f_nc, taps=taps, window=window)
+        return s.r0 + 0.5 * s.c * s.pads[1] / s.fs
+
+    def params(self, taps: int = 6, window="floor", range_ref="r0") -> native.WkParams:
+        """sarbp_omega_k parameters: r0 is the scene's reference range for
+        range_ref "r0", the range of raw sample 0 for "centre"."""
+        s = self.resolved()
+        r0 = s.first_sample_range if range_ref == "centre" else s.r0
+        return native.make_wk_params(c=s.c, f0=s.f0, fs=s.fs, prf=s.prf, kr=s.kr, vr=s.vr, r0=r0,
+                                     f_nc=s.f_nc, taps=taps, window=window, range_ref=range_ref)
 
 
 def target_at(scene: WkScene, line: float, sample: float) -> tuple:
@@ -123,12 +132,24 @@ def simulate(scene: WkScene, targets, amplitudes=None) -> np.ndarray:
     return echo.astype(np.complex64)
 
 
-def radarsat_params(rp, *, f_nc: float | None = None, taps: int = 6, window="floor") -> native.WkParams:
+def radarsat_params(rp, *, f_nc: float | None = None, taps: int = 6, window="floor",
+                    range_ref="r0") -> native.WkParams:
     """omega-K parameters of the RADARSAT-1 scene (radarsat.RunParams): the
     down-chirp (kr = -Kr), r0 = R0, the driver's vr and Doppler centroid
-    (wk.cc:62-64 / mian_wka_sinc.m:32-34) unless f_nc is given."""
+    (wk.cc:62-64 / mian_wka_sinc.m:32-34) unless f_nc is given.  R0 is the
+    slant range of the first sample, which is what range_ref "centre" takes."""
     return native.make_wk_params(c=rp.c, f0=rp.f0, fs=rp.Fr, prf=rp.PRF, kr=-rp.Kr, vr=rp.Vr, r0=rp.R0,
-                                 f_nc=rp.fdc_hint if f_nc is None else f_nc, taps=taps, window=window)
+                                 f_nc=rp.fdc_hint if f_nc is None else f_nc, taps=taps, window=window,
+                                 range_ref=range_ref)
+
+
+def centre_display_shift(rp, nr: int, shift_range: int = -1800) -> int:
+    """Range display shift that puts a range_ref "centre" image where the
+    reference's display (shift_range on a range_ref "r0" image) puts the
+    same scene: the "r0" image's columns sit (2 R0/c fs) mod Nr samples
+    before the padded samples of the "centre" image."""
+    lag = math.fmod(2.0 * rp.R0 / rp.c * rp.Fr, nr)
+    return int(round(shift_range - lag))
 
 
 def display(image: np.ndarray, shift_range: int = -1800, shift_az: int = -3365, flip: bool = True) -> np.ndarray:

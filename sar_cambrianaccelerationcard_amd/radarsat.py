@@ -158,7 +158,7 @@ def main(argv=None) -> int:
 
         python -m sar_cambrianaccelerationcard_amd.radarsat (--mat-dir DIR | --crop NPZ) --out PREFIX
             [--lines M0 M1] [--nx 1024 --ny 512 --dx 4 --dy 4.64] [--interp sinc8]
-            [--omega-k [--f-nc HZ | --f-nc-estimate]]
+            [--omega-k [--f-nc HZ | --f-nc-estimate] [--range-ref r0|centre]]
 
     DIR holds CD_run_params.mat and CDdata1.mat / CDdata2.mat (read with
     scipy.io.loadmat); NPZ is a packed crop (load_crop).  The region is centred on the beam footprint of lines
@@ -169,7 +169,10 @@ def main(argv=None) -> int:
     scene (Doppler centroid -6900 Hz as in wk.cc:64 unless given or
     estimated); for the full 3072-line scene the image gets the MATLAB
     prototype's display shifts and flip (mian_wka_sinc.m:124-126), and the
-    PNG shows the magnitude saturated at 50 (:129-131)."""
+    PNG shows the magnitude saturated at 50 (:129-131).  --range-ref centre
+    uses the library's window-centred range reference (R0 is the range of the
+    first sample) instead of the reference's, with the range display shift
+    that lines its image up with the reference's display."""
     import argparse
     import os
     import time
@@ -196,6 +199,7 @@ def main(argv=None) -> int:
     fnc = ap.add_mutually_exclusive_group()
     fnc.add_argument("--f-nc", type=float, default=None, help="omega-K Doppler centroid, Hz")
     fnc.add_argument("--f-nc-estimate", action="store_true", help="omega-K with the estimated centroid")
+    ap.add_argument("--range-ref", choices=("r0", "centre"), default="r0", help="omega-K range reference")
     args = ap.parse_args(argv)
     if args.crop:
         raw, rp, meta = load_crop(args.crop)
@@ -254,7 +258,7 @@ def _omega_k_main(args, raw: np.ndarray, rp: RunParams, base: int, fdc: float) -
     if not (base <= m0 < m1 <= base + raw.shape[0]):
         raise SystemExit(f"--lines must lie in [{base}, {base + raw.shape[0]})")
     f_nc = fdc if args.f_nc_estimate else (rp.fdc_hint if args.f_nc is None else args.f_nc)
-    wp = OK.radarsat_params(rp, f_nc=f_nc)
+    wp = OK.radarsat_params(rp, f_nc=f_nc, range_ref=args.range_ref)
     ctx = native.Context(0)
     echo = np.ascontiguousarray(raw[m0 - base:m1 - base])
     D.omega_k(ctx, echo, wp)  # builds the plans and tables of this size
@@ -263,9 +267,10 @@ def _omega_k_main(args, raw: np.ndarray, rp: RunParams, base: int, fdc: float) -
     t = time.perf_counter() - t0
     full = img.shape == (4096, 3414)
     if full:
-        img = OK.display(img)
+        shift = -1800 if args.range_ref == "r0" else OK.centre_display_shift(rp, img.shape[1])
+        img = OK.display(img, shift_range=shift)
     meta = {"scene": "RADARSAT-1 CD data (reference code/SAR_WK/wk_v5.2.3/data)", "algorithm": "omega-K",
-            "lines": [m0, m1], "f_nc_hz": f_nc, "taps": wp.taps, "window": "floor",
+            "lines": [m0, m1], "f_nc_hz": f_nc, "taps": wp.taps, "window": "floor", "range_ref": args.range_ref,
             "display_shifts": full, "seconds": t}
     io.save_image(args.out + ".bin", img, meta)
     io.save_png(args.out + ".png", np.minimum(np.abs(img), 50.0), -25.0)

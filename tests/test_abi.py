@@ -39,7 +39,7 @@ def test_binding_covers_header():
 
 def test_abi_version_and_strerror():
     lib = native.load()
-    assert lib.sarbp_abi_version() == 3
+    assert lib.sarbp_abi_version() == 4
     for s in range(0, 11):
         t = lib.sarbp_strerror(s).decode()
         assert t and t != "unknown status"

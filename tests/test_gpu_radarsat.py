@@ -86,7 +86,8 @@ def test_imaging_command_on_the_crop(tmp_path):
     assert os.path.getsize(out + ".png") > 0
 
 
-def test_omega_k_imaging_command_on_the_crop(tmp_path):
+@pytest.mark.parametrize("range_ref", ["r0", "centre"])
+def test_omega_k_imaging_command_on_the_crop(tmp_path, range_ref):
     """The same command with --omega-k: the crop's 256 lines imaged by
     sarbp_omega_k (the reference's omega-K driver runs on this scene), equal
     to the library call on the same lines."""
@@ -98,7 +99,8 @@ def test_omega_k_imaging_command_on_the_crop(tmp_path):
     from sar_cambrianaccelerationcard_amd import omegak as OK
     out = str(tmp_path / "cdwk")
     r = subprocess.run([sys.executable, "-m", "sar_cambrianaccelerationcard_amd.radarsat", "--crop", CROP,
-                        "--out", out, "--omega-k"], capture_output=True, text=True, timeout=300,
+                        "--out", out, "--omega-k", "--range-ref", range_ref], capture_output=True, text=True,
+                       timeout=300,
                        cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
     print(r.stdout, r.stderr[-2000:], flush=True)
     assert r.returncode == 0
@@ -106,6 +108,6 @@ def test_omega_k_imaging_command_on_the_crop(tmp_path):
     raw, rp, _ = RS.load_crop(CROP)
     assert img.shape == native.wk_padded_size(*raw.shape) and os.path.getsize(out + ".png") > 0
     c = native.Context(0)
-    ref = D.omega_k(c, raw, OK.radarsat_params(rp)).cpu().numpy()
+    ref = D.omega_k(c, raw, OK.radarsat_params(rp, range_ref=range_ref)).cpu().numpy()
     c.close()
     assert np.array_equal(img, ref)

@@ -122,7 +122,7 @@ def _omega_k(ctx, echo, wp):
 
 def _oracle(echo, wp):
     return W.omega_k(echo, c=wp.c, f0=wp.f0, fs=wp.fs, prf=wp.prf, kr=wp.kr, vr=wp.vr, r0=wp.r0, f_nc=wp.f_nc,
-                     taps=wp.taps, window=wp.window)
+                     taps=wp.taps, window=wp.window, range_ref=wp.range_ref)
 
 
 def _rel_l2(a, b):
@@ -157,13 +157,14 @@ def test_omega_k_random_echo_odd_sizes(ctx):
     assert _rel_l2(got, ref) < 2e-5
 
 
-def test_omega_k_radarsat_crop(ctx):
+@pytest.mark.parametrize("range_ref", ["r0", "centre"])
+def test_omega_k_radarsat_crop(ctx, range_ref):
     from sar_cambrianaccelerationcard_amd import radarsat as RS
     z = np.load(os.path.join(HERE, "golden", "radarsat_cd_crop.npz"))
     meta = json.loads(str(z["meta"]))
     rp = RS.RunParams(**meta["run_params"])
     echo = RS.unpack4(z["codes"])
-    wp = O.radarsat_params(rp)
+    wp = O.radarsat_params(rp, range_ref=range_ref)
     got = _omega_k(ctx, echo, wp)
     ref = _oracle(echo, wp)
     assert got.shape == W.padded_size(*echo.shape)[:2]
@@ -191,7 +192,7 @@ def test_omega_k_rejects_bad_arguments(ctx):
     d = torch.zeros((8, 8), dtype=torch.complex64, device="cuda:0")
     img = torch.zeros((12, 14), dtype=torch.complex64, device="cuda:0")
     good = O.WkScene(na_in=8, nr_in=8).params()
-    for field, val in (("taps", 7), ("window", 2), ("prf", 0.0), ("kr", 0.0)):
+    for field, val in (("taps", 7), ("window", 2), ("prf", 0.0), ("kr", 0.0), ("range_ref", 2), ("reserved", 1)):
         wp = O.WkScene(na_in=8, nr_in=8).params()
         setattr(wp, field, val)
         with pytest.raises(native.SarbpError) as e:
@@ -267,9 +268,42 @@ def test_omega_k_calls_on_two_streams(ctx):
 @pytest.mark.parametrize("na_in,nr_in", [(1, 1), (2, 3), (3, 2), (8, 2458), (8, 2459)])
 def test_omega_k_edge_sizes(ctx, na_in, nr_in):
     # Single samples, tiny lines, and padded lines of exactly 4096 samples (the
-    # longest the fused line kernel takes) and 4099 (the 2-D-plan path).
+    # longest on 8192-point transforms) and 4099 (16384-point transforms).
     echo = _rand((na_in, nr_in), na_in * 7 + nr_in)
     wp = O.WkScene(na_in=na_in, nr_in=nr_in).params()
+    got = _omega_k(ctx, echo, wp)
+    ref = _oracle(echo, wp)
+    assert got.shape == W.padded_size(na_in, nr_in)[:2]
+    assert _rel_l2(got, ref) < 2e-5
+
+
+@pytest.mark.parametrize("f_nc", [0.0, 60.0])
+@pytest.mark.parametrize("window", ["floor", "centered"])
+def test_omega_k_centred_range_reference(ctx, f_nc, window):
+    # Targets in both halves of the padded range window and padded sample 0 at
+    # no whole number of range windows: the window-centred range reference
+    # focuses every target, on the device as in the oracle.
+    scene = O.WkScene(f_nc=f_nc, r0=2300.0)
+    _, nr = scene.padded
+    targets = [(194, 170), (294, 235), (394, 300), (244, nr // 2 + 60), (330, nr // 2 + 150)]
+    echo = O.simulate(scene, targets)
+    wp = scene.params(window=window, range_ref="centre")
+    assert wp.range_ref == native.WK_RANGE_REF_CENTRE
+    got = _omega_k(ctx, echo, wp)
+    ref = _oracle(echo, wp)
+    assert _rel_l2(got, ref) < 2e-5
+    for t in targets:
+        mg, mr = O.peak_metrics(got, *t), O.peak_metrics(ref, *t)
+        assert (mg["d_line"], mg["d_sample"]) == (0, 0) and mg["concentration"] > 0.25, mg
+        assert abs(mg["concentration"] - mr["concentration"]) < 1e-3
+
+
+@pytest.mark.parametrize("na_in,nr_in", [(2, 3), (45, 101), (8, 2458), (8, 2459), (64, 3100)])
+def test_omega_k_centred_random_echo(ctx, na_in, nr_in):
+    # Odd and even padded line lengths (the output shift's constant phase
+    # differs) on 8192- and 16384-point transforms.
+    echo = _rand((na_in, nr_in), 5 * na_in + nr_in)
+    wp = O.WkScene(na_in=na_in, nr_in=nr_in, r0=2300.0).params(range_ref="centre")
     got = _omega_k(ctx, echo, wp)
     ref = _oracle(echo, wp)
     assert got.shape == W.padded_size(na_in, nr_in)[:2]

@@ -220,3 +220,57 @@ def test_far_half_of_the_range_window_defocuses():
     far = [(294, nr // 2 + 60)]
     ok, ms = _focused(_image(scene, far), far)
     assert not ok and ms[0]["concentration"] < 0.15, ms
+
+
+# ---- window-centred range reference (an option of this library) ---------------------
+
+def _image_centred(scene, targets, **over):
+    s = scene.resolved()
+    echo = O.simulate(s, targets)
+    wp = dict(c=s.c, f0=s.f0, fs=s.fs, prf=s.prf, kr=s.kr, vr=s.vr, r0=s.first_sample_range, f_nc=s.f_nc,
+              range_ref=W.REF_CENTRE)
+    wp.update(over)
+    return W.omega_k(echo, **wp)
+
+
+def _both_halves(scene):
+    _, nr = scene.padded
+    return TARGETS + [(244, nr // 2 + 60), (330, nr // 2 + 150)]
+
+
+@pytest.mark.parametrize("f_nc", [0.0, 60.0])
+@pytest.mark.parametrize("r0", [0.0, 2300.0])   # 0: a whole number of range windows (1918.7 m)
+@pytest.mark.parametrize("window", [W.FLOOR, W.CENTERED])
+def test_centred_reference_focuses_the_whole_window(f_nc, r0, window):
+    scene = O.WkScene(f_nc=f_nc, r0=r0)
+    tg = _both_halves(scene)
+    ok, ms = _focused(_image_centred(scene, tg, window=window), tg)
+    assert ok, ms
+
+
+def test_reference_pipeline_needs_a_whole_number_of_range_windows():
+    # With padded sample 0 at 2 r0 / c for an arbitrary r0 the reference's
+    # range origin leaves a phase ramp the Stolt step bends: even near-half
+    # targets defocus.
+    scene = O.WkScene(r0=2300.0)
+    ok, ms = _focused(_image(scene))
+    assert not ok and max(m["concentration"] for m in ms) < 0.15, ms
+
+
+def test_centred_reference_planted_mistakes(monkeypatch):
+    scene = O.WkScene(f_nc=60.0, r0=2300.0)
+    s = scene.resolved()
+    tg = _both_halves(scene)
+    shift0 = W.time_shift
+    planted = {
+        "r0 taken as padded sample 0": lambda: _image_centred(scene, tg, r0=s.r0),
+        "time shift sign": lambda: (monkeypatch.setattr(
+            W, "time_shift", lambda fr, t: shift0(fr, -t)), _image_centred(scene, tg))[1],
+        "no origin shifts": lambda: (monkeypatch.setattr(
+            W, "time_shift", lambda fr, t: np.ones_like(fr, dtype=np.complex128)), _image_centred(scene, tg))[1],
+    }
+    for name, make in planted.items():
+        img = make()
+        monkeypatch.setattr(W, "time_shift", shift0)
+        ok, ms = _focused(img, tg)
+        assert not ok, (name, ms)
