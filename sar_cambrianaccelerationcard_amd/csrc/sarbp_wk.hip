@@ -124,9 +124,26 @@ __device__ __forceinline__ float2 stolt_sample(const StoltArgs& a, int j, double
   if (a.rfm) v = rotate_rev(v, fma(a.rfm_a, s, (a.rfm_b * fr + a.rfm_c) * fr));
   const double cen = (double)j - (s - fv) * a.nr_over_fs;
   const double fl = floor(cen);
+  // Within 2^-25 below an integer the fraction would round to 1 in FP32 (a
+  // zero divisor in the weights); the largest float below 1 keeps the taps of
+  // floor(cen).
   *c0 = (int)fl;
-  *f = (float)(cen - fl);
+  *f = fminf((float)(cen - fl), 0x1.fffffep-1f);
   return v;
+}
+
+// sin(pi f) for f in [0, 1]: sin(pi g), g = min(f, 1 - f) (1 - f is exact for
+// f >= 1/2), as g P(g^2) with a degree-4 P fitted for least maximum relative
+// error on g in [0, 1/2] (5.3e-9 before rounding, about 2 ulp in FP32).
+__device__ __forceinline__ float sinpi01(float f) {
+  const float g = fminf(f, 1.0f - f);
+  const float u = g * g;
+  float p = 0.07756070702744297f;
+  p = fmaf(p, u, -0.5982422909074577f);
+  p = fmaf(p, u, 2.550069753062404f);
+  p = fmaf(p, u, -5.167709686218649f);
+  p = fmaf(p, u, 3.14159263690788f);
+  return g * p;
 }
 
 // Sum over the taps of one output at centre c0 + f (f in [0, 1)) for any tap
@@ -169,19 +186,21 @@ __device__ __forceinline__ float2 stolt_taps_general(const float2* line, int nr,
 }
 
 // The same sum for P taps at offsets q0 .. q0+P-1 that all lie inside the
-// line, 0 < f < 1 (no tap at distance 0): weights (-1)^q sin(pi f) / (pi (f - q)),
-// reciprocals taken in pairs (1/a = b/(ab), 1/b = a/(ab)).
+// line, 0 < f < 1 (no tap at distance 0): weights (-1)^q (sin(pi f)/pi) / (f - q),
+// reciprocals taken in pairs (1/a = b/(ab), 1/b = a/(ab)) with sin(pi f)/pi
+// and the sign folded into the pair's factor.
 template <int LG, int P>
 __device__ __forceinline__ float2 stolt_taps_inner(const float2* line, int c0, float f, float sf, int q0) {
+  const float amp = sf * 0.318309886183790672f;
   float d[P];
 #pragma unroll
-  for (int t = 0; t < P; ++t) d[t] = 3.14159265358979f * (f - (float)(q0 + t));
+  for (int t = 0; t < P; ++t) d[t] = f - (float)(q0 + t);
   float2 acc = make_float2(0.0f, 0.0f);
 #pragma unroll
   for (int t = 0; t < P; t += 2) {
-    const float r = __builtin_amdgcn_rcpf(d[t] * d[t + 1]);
-    const float s0 = ((q0 + t) & 1) ? -sf : sf;
-    const float w0 = s0 * d[t + 1] * r, w1 = -s0 * d[t] * r;
+    float r = amp * __builtin_amdgcn_rcpf(d[t] * d[t + 1]);
+    if ((q0 + t) & 1) r = -r;
+    const float w0 = d[t + 1] * r, w1 = -d[t] * r;
     const float2 v0 = line[lslot<LG>(c0 + q0 + t)], v1 = line[lslot<LG>(c0 + q0 + t + 1)];
     acc.x = fmaf(w0, v0.x, acc.x);
     acc.y = fmaf(w0, v0.y, acc.y);
@@ -199,9 +218,26 @@ __device__ __forceinline__ float2 stolt_output(const StoltArgs& a, const float2*
   const int q0 = a.window == SARBP_STOLT_FLOOR ? -half : -half + 1;
   const int c_lo = -q0, c_hi = a.nr - 1 - (q0 + 2 * half - 1);
   // sin(pi (f - p)) = (-1)^p sin(pi f) for integer p.
-  const float sf = sinpif(f);
+  const float sf = sinpi01(f);
   if (P > 0 && f > 0.0f && c0 >= c_lo && c0 <= c_hi)
     return stolt_taps_inner<LG, (P > 0 ? P : 2)>(line, c0, f, sf, q0);
+  if (a.window == SARBP_STOLT_FLOOR) {
+    // Every tap clamped to the same end sample e: 2 half equal terms
+    // sinc(c - e) line[e], sin(pi (c - e)) = (-1)^(c0 - e) sin(pi f).
+    int e = -1;
+    if (c0 + half - 1 <= 0)
+      e = 0;
+    else if (c0 - half >= a.nr - 1)
+      e = a.nr - 1;
+    if (e >= 0) {
+      const int d = c0 - e;
+      const float w = (float)(2 * half) * sinc_at((float)d + f, (d & 1) ? -sf : sf);
+      const float2 v = line[lslot<LG>(e)];
+      return make_float2(w * v.x, w * v.y);
+    }
+  } else if (c0 + half < 0 || c0 - half > a.nr - 1) {
+    return make_float2(0.0f, 0.0f);  // no tap inside the line
+  }
   return stolt_taps_general<LG>(line, a.nr, c0, f, sf, half, a.window);
 }
 

@@ -98,6 +98,32 @@ def test_stolt_radarsat_axes(ctx):
         _check_stolt(got, ref)
 
 
+def test_stolt_centres_just_below_an_integer(ctx):
+    # A zero range-frequency axis and one azimuth frequency chosen so every
+    # centre of row 1 is n + 1 - 1e-8: the fraction rounds to 1 in FP32.  The
+    # taps stay those of floor(centre) and the weights finite.
+    na, nr = 3, 64
+    f0, c, vr, fs = 1e9, 3e8, 150.0, 1e6
+    k = c * c / (4.0 * vr * vr)
+    s = f0 - (1.0 - 1e-8) * fs / nr
+    fa = np.array([0.0, np.sqrt((f0 - s) * (f0 + s) / k), 0.0371])
+    fr = np.zeros(nr)
+    cen = W.centres(fr, fa, f0, c, vr, fs)
+    frac = cen[1] - np.floor(cen[1])
+    assert np.all(frac < 1.0) and np.all(frac.astype(np.float32) == 1.0)
+    S = _rand((na, nr), 21)
+    for window in ("floor", "centered"):
+        got = _stolt(ctx, S, fr, fa, f0=f0, c=c, vr=vr, fs=fs, taps=6, window=window)
+        ref = W.stolt_sinc(S, fr, fa, f0=f0, c=c, vr=vr, fs=fs, taps=6,
+                           window=W.FLOOR if window == "floor" else W.CENTERED)
+        assert np.all(np.isfinite(got)), window
+        _check_stolt(got, ref)
+        # the centre ~ n + 1: the output is S[n + 1] to 1e-6 (short of the
+        # line's end, where the floor window counts clamped copies of the last
+        # sample)
+        assert np.max(np.abs(ref[1, :-2] - S[1, 1:-1])) < 1e-6 * np.max(np.abs(S[1]))
+
+
 def test_stolt_rejects_bad_arguments(ctx):
     torch = _torch()
     d = torch.zeros((2, 8), dtype=torch.complex64, device="cuda:0")
