@@ -523,7 +523,7 @@ sarbp_status sarbp_wk_padded_size(int32_t na_in, int32_t nr_in, int32_t* na, int
  * transforms in LDS; lines over 4096 samples need 128 KiB of LDS per
  * workgroup, else SARBP_ERR_UNSUPPORTED) between hipFFT azimuth
  * transforms.  Plans, chirp tables
- * and a transpose workspace of na x nr complex64 words are kept by the
+ * and a workspace of na x nr complex64 words are kept by the
  * context (the first call of a size builds them).  Calls on one context share
  * them and are ordered on the device: each call's stream first waits for the
  * previous call's work, whatever streams they use.  nr <= 8192. */

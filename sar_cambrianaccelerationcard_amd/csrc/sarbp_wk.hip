@@ -10,7 +10,10 @@
 // length-nr range DFT and inverse DFT run as Bluestein chirp-z transforms on
 // 2^LG-point FFTs in LDS (nr need not be a power of two: the reference's
 // scene pads to 3414 = 2 * 3 * 569), and it is written once.  The azimuth
-// transforms are power-of-two hipFFT row transforms between tiled transposes.
+// transforms are power-of-two hipFFT row transforms of an [nr][na] workspace;
+// the line kernel reads and writes its line as a column of that workspace in
+// place (lines of neighbouring workgroups share the cache lines in the XCD's
+// L2), and one tiled transpose writes the [na][nr] image.
 // Kernels written here:
 //   wk_prep_t    zeros, or the echo sample times its line's down-conversion
 //                factor (FP64 phase), written transposed ([nr][na]) for the
@@ -273,18 +276,33 @@ __global__ __launch_bounds__(kWkThreads) void wk_stolt(StoltArgs a) {
 // Chirp-z tables of one (nr, LG): b[n] = exp(j pi n^2 / nr) for n < nr;
 // bh[k] = FFT_M(bb)[k] / M with bb[n] = b[|n|] at n and M - n (|n| < nr),
 // M = 2^LG >= 2 nr - 1; tw[t] = exp(-j 2 pi t / M).
-// Output sample n of a line goes to (n + out_shift) mod nr, times out_rot
-// (which includes the 1/(na nr) scale): a circular shift with a constant
-// phase is the range-frequency ramp exp(-j 2 pi fr h / fs) of the centred
-// range reference on the fftshift axis.
+// Sample j of input line m is st.in[m in_line + j in_sample] (rows of an
+// [na][nr] array: in_line = nr, in_sample = 1; columns of an [nr][na] array:
+// in_line = 1, in_sample = na), likewise for st.out; samples outside
+// [nz_lo, nz_hi) are known zeros and not read.  Output sample n of a line
+// goes to (n + out_shift) mod nr, times out_rot (which includes the
+// 1/(na nr) scale): a circular shift with a constant phase is the
+// range-frequency ramp exp(-j 2 pi fr h / fs) of the centred range reference
+// on the fftshift axis.
 struct RowArgs {
-  StoltArgs st;       // in = out = the line array [na][nr]
+  StoltArgs st;
   const float2* tw;   // [M]
   const float2* bh;   // [M]
   const float2* chirp;  // [nr]
   int out_shift;      // 0 <= out_shift < nr
+  int nz_lo, nz_hi;   // 0 <= nz_lo <= nz_hi <= nr
+  int in_line, in_sample, out_line, out_sample;
   float2 out_rot;
 };
+
+// Line of workgroup b: the dispatcher deals workgroups to the 8 XCDs in turn
+// (b -> XCD b % 8), so XCD x gets a contiguous run of lines and the lines in
+// flight on one XCD share the 128-byte cache lines of the [nr][na] array in
+// that XCD's L2.
+__device__ __forceinline__ int xcd_line(int b, int n) {
+  const int q = n >> 3, r = n & 7, x = b & 7, i = b >> 3;
+  return x * q + min(x, r) + i;
+}
 
 constexpr int rows_threads(int lg) { return (1 << lg) / 16; }
 
@@ -310,16 +328,18 @@ __global__ __launch_bounds__(rows_threads(LG), 4) void wk_rows(RowArgs r) {
   constexpr int NF = 1 << LG, T = rows_threads(LG), E = NF / T;
   extern __shared__ float2 buf[];
   const StoltArgs& a = r.st;
-  const int m = blockIdx.x;
+  const int m = xcd_line(blockIdx.x, a.na);
   const int nr = a.nr;
-  const size_t base = (size_t)m * (size_t)nr;
+  const float2* __restrict__ in = a.in + (size_t)m * r.in_line;
+  float2* out = a.out + (size_t)m * r.out_line;
   const int tid = threadIdx.x;
 
   // Forward range DFT.
 #pragma unroll 4
   for (int e = 0; e < E; ++e) {
     const int i = tid + e * T;
-    buf[fft::slot<LG>(i)] = i < nr ? cmul(a.in[base + i], conjf2(r.chirp[i])) : make_float2(0.0f, 0.0f);
+    buf[fft::slot<LG>(i)] = i >= r.nz_lo && i < r.nz_hi ? cmul(in[(size_t)i * r.in_sample], conjf2(r.chirp[i]))
+                                                         : make_float2(0.0f, 0.0f);
   }
   __syncthreads();
   chirp_convolve<LG>(buf, r);
@@ -369,7 +389,7 @@ __global__ __launch_bounds__(rows_threads(LG), 4) void wk_rows(RowArgs r) {
       const float2 v = cmul(r.chirp[j], conjf2(buf[fft::slot<LG>(j)]));
       int o = j + r.out_shift;
       if (o >= nr) o -= nr;
-      a.out[base + o] = cmul(v, r.out_rot);
+      out[(size_t)o * r.out_sample] = cmul(v, r.out_rot);
     }
   }
 }
@@ -516,7 +536,7 @@ sarbp_status check_stolt(double f0, double c, double vr, int na, int nr, double 
 struct WkCache {
   std::map<std::pair<int, int>, hipfftHandle> row_plans;  // 1-D (length, batch)
   std::map<int, float2*> chirp_tables;                    // nr -> [tw | bh | chirp]
-  float2* work = nullptr;  // transpose workspace of sarbp_omega_k
+  float2* work = nullptr;  // [nr][na] workspace of sarbp_omega_k
   size_t work_cap = 0;     // complex64 words
   double* d_axes = nullptr;  // [nr | na] of sarbp_stolt_sinc
   double* h_axes = nullptr;  // pinned staging: the axes last uploaded
@@ -823,10 +843,10 @@ sarbp_status omega_k_enqueue(WkCache* wc, const sarbp_cf32* d_echo, int na_in, i
     }
   }
 
-  // Azimuth FFT (columns, through the transposed workspace; only the nr_in
-  // columns holding echo are non-zero), one fused kernel per line for the
-  // range DFT / reference function / Stolt / range inverse DFT, azimuth
-  // inverse FFT.
+  // Azimuth FFT (rows of the transposed workspace; only the nr_in rows
+  // holding echo are non-zero), one fused kernel per line (a column of the
+  // workspace, in place) for the range DFT / reference function / Stolt /
+  // range inverse DFT, azimuth inverse FFT, transpose into the image.
   const float2* tab = nullptr;
   hipfftHandle az_fwd = nullptr, az_inv = nullptr;
   if ((st = chirp_tables(wc, nr, lg, &tab)) != SARBP_OK) return st;
@@ -838,16 +858,22 @@ sarbp_status omega_k_enqueue(WkCache* wc, const sarbp_cf32* d_echo, int na_in, i
   hipLaunchKernelGGL(wk_prep_t, dim3((unsigned)((nr + 63) / 64), (unsigned)((na + 63) / 64)), dim3(256), 0, s, pa);
   if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "wk_prep_t launch");
   if ((st = exec_fft(az_fwd, w + (size_t)pad_rg * na, HIPFFT_FORWARD, s)) != SARBP_OK) return st;
-  if ((e = launch_transpose(w, img, nr, na, s)) != hipSuccess) return hip_fail(e, "wk_transpose launch");
   RowArgs r{};
   r.st = a;
+  r.st.in = w;
+  r.st.out = w;
+  r.nz_lo = pad_rg;
+  r.nz_hi = pad_rg + nr_in;
+  r.in_line = 1;
+  r.in_sample = na;
+  r.out_line = 1;
+  r.out_sample = na;
   r.tw = tab;
   r.bh = tab + ((size_t)1 << lg);
   r.chirp = tab + ((size_t)2 << lg);
   r.out_shift = out_shift;
   r.out_rot = make_float2((float)(std::cos(out_phase) * a.scale), (float)(std::sin(out_phase) * a.scale));
   if ((e = launch_rows(r, lg, s)) != hipSuccess) return hip_fail(e, "wk_rows launch");
-  if ((e = launch_transpose(img, w, na, nr, s)) != hipSuccess) return hip_fail(e, "wk_transpose launch");
   if ((st = exec_fft(az_inv, w, HIPFFT_BACKWARD, s)) != SARBP_OK) return st;
   if ((e = launch_transpose(w, img, nr, na, s)) != hipSuccess) return hip_fail(e, "wk_transpose launch");
   return SARBP_OK;
