@@ -421,11 +421,13 @@ struct PrepArgs {
 };
 
 // Padding and azimuth down-conversion, written transposed: out[j][m] (=
-// [nr][na]) for 64 x 64 tiles of (range sample j, padded line m), through LDS.
+// [nr][na]) for 64 x 64 tiles of (range sample j, padded line m), through LDS,
+// for the rows j in [pad_rg, pad_rg + nr_in) that hold echo (the zero rows
+// are neither transformed nor read before the line kernel writes them).
 __global__ __launch_bounds__(256) void wk_prep_t(PrepArgs a) {
   __shared__ float2 tile[64][65];
   __shared__ float2 rot[64];
-  const int j0 = blockIdx.x * 64, m0 = blockIdx.y * 64;
+  const int j0 = a.pad_rg + blockIdx.x * 64, m0 = blockIdx.y * 64;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   if (threadIdx.x < 64) {
     // exp(-j 2 pi f_nc ta) of line m0 + t, ta = (m - na/2) / prf.
@@ -451,7 +453,7 @@ __global__ __launch_bounds__(256) void wk_prep_t(PrepArgs a) {
 #pragma unroll 4
   for (int k = 0; k < 16; ++k) {
     const int j = j0 + ty + 4 * k, m = m0 + tx;
-    if (j < a.nr && m < a.na) a.out[(size_t)j * a.na + m] = tile[tx][ty + 4 * k];
+    if (j < a.pad_rg + a.nr_in && m < a.na) a.out[(size_t)j * a.na + m] = tile[tx][ty + 4 * k];
   }
 }
 
@@ -855,7 +857,8 @@ sarbp_status omega_k_enqueue(WkCache* wc, const sarbp_cf32* d_echo, int na_in, i
   if ((st = work_buffer(wc, (size_t)na * nr)) != SARBP_OK) return st;
   float2* w = wc->work;
   pa.out = w;
-  hipLaunchKernelGGL(wk_prep_t, dim3((unsigned)((nr + 63) / 64), (unsigned)((na + 63) / 64)), dim3(256), 0, s, pa);
+  hipLaunchKernelGGL(wk_prep_t, dim3((unsigned)((nr_in + 63) / 64), (unsigned)((na + 63) / 64)), dim3(256), 0, s,
+                     pa);
   if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "wk_prep_t launch");
   if ((st = exec_fft(az_fwd, w + (size_t)pad_rg * na, HIPFFT_FORWARD, s)) != SARBP_OK) return st;
   RowArgs r{};
