@@ -457,6 +457,74 @@ __global__ __launch_bounds__(256) void wk_prep_t(PrepArgs a) {
   }
 }
 
+// Padding, azimuth down-conversion and the azimuth FFT of one echo range
+// column c, for a power-of-two padded azimuth length na = 2^LG: the column is
+// read down the echo (the columns in flight on one XCD share the cache lines
+// of its L2), transformed in LDS and written as row pad_rg + c of the [nr][na]
+// workspace.  Replaces wk_prep_t and the hipFFT row transforms for these na.
+struct AzArgs {
+  const float2* echo;   // [na_in][nr_in]
+  float2* out;          // [nr][na]
+  const float2* tw;     // [na]: exp(-j 2 pi t / na)
+  int na, na_in, nr_in, pad_az, pad_rg;
+  double rev_per_line;  // -f_nc / prf, revolutions
+};
+
+template <int LG>
+__global__ __launch_bounds__((1 << LG) / 16) void wk_az_fwd(AzArgs a) {
+  constexpr int NF = 1 << LG, T = NF / 16;
+  extern __shared__ float2 buf[];
+  const int c = xcd_line(blockIdx.x, a.nr_in);
+  const int tid = threadIdx.x;
+#pragma unroll 4
+  for (int e = 0; e < 16; ++e) {
+    const int m = tid + e * T;
+    const int mi = m - a.pad_az;
+    float2 v = make_float2(0.0f, 0.0f);
+    if (mi >= 0 && mi < a.na_in) {
+      // exp(-j 2 pi f_nc ta), ta = (m - na/2) / prf, as in wk_prep_t.
+      const double rev = a.rev_per_line * ((double)m - 0.5 * (double)NF);
+      float sn, cs;
+      sincospif(2.0f * (float)(rev - rint(rev)), &sn, &cs);
+      const float2 x = a.echo[(size_t)mi * a.nr_in + c];
+      v = make_float2(x.x * cs - x.y * sn, x.x * sn + x.y * cs);
+    }
+    buf[fft::slot<LG>(m)] = v;
+  }
+  __syncthreads();
+  fft::fft_lds<LG, T, false>(buf, a.tw);
+  float2* row = a.out + (size_t)(a.pad_rg + c) * NF;
+#pragma unroll 4
+  for (int e = 0; e < 16; ++e) {
+    const int k = tid + e * T;
+    row[k] = buf[fft::slot<LG>(k)];
+  }
+}
+
+// log2 na when na is a power of two the fused azimuth kernel takes, else 0.
+int az_lg(int na) {
+  for (int lg = 10; lg <= 13; ++lg)
+    if (na == (1 << lg)) return lg;
+  return 0;
+}
+
+template <int LG>
+hipError_t launch_az_lg(const AzArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(wk_az_fwd<LG>, dim3((unsigned)a.nr_in), dim3((1 << LG) / 16),
+                     (size_t)fft::lds_words<LG>() * sizeof(float2), s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_az(const AzArgs& a, int lg, hipStream_t s) {
+  switch (lg) {
+    case 10: return launch_az_lg<10>(a, s);
+    case 11: return launch_az_lg<11>(a, s);
+    case 12: return launch_az_lg<12>(a, s);
+    case 13: return launch_az_lg<13>(a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 // Smallest LG in [11, 14] with 2 nr - 1 <= 2^LG (nr <= kRowsMaxRg).
 int rows_lg(int nr) {
   int lg = 11;
@@ -538,6 +606,7 @@ sarbp_status check_stolt(double f0, double c, double vr, int na, int nr, double 
 struct WkCache {
   std::map<std::pair<int, int>, hipfftHandle> row_plans;  // 1-D (length, batch)
   std::map<int, float2*> chirp_tables;                    // nr -> [tw | bh | chirp]
+  std::map<int, float2*> az_twiddles;                     // na -> exp(-j 2 pi t / na)
   float2* work = nullptr;  // [nr][na] workspace of sarbp_omega_k
   size_t work_cap = 0;     // complex64 words
   double* d_axes = nullptr;  // [nr | na] of sarbp_stolt_sinc
@@ -558,6 +627,7 @@ void wk_cache_destroy(WkCache* wc) {
   (void)hipDeviceSynchronize();  // the work buffer and tables may still be in use
   for (auto& kv : wc->row_plans) (void)hipfftDestroy(kv.second);
   for (auto& kv : wc->chirp_tables) (void)hipFree(kv.second);
+  for (auto& kv : wc->az_twiddles) (void)hipFree(kv.second);
   (void)hipFree(wc->work);
   (void)hipFree(wc->d_axes);
   (void)hipHostFree(wc->h_axes);
@@ -633,6 +703,30 @@ sarbp_status chirp_tables(WkCache* wc, int nr, int lg, const float2** out) {
     return hip_fail(e, "hipMemcpy(chirp tables)");
   }
   wc->chirp_tables[nr] = d;
+  *out = d;
+  return SARBP_OK;
+}
+
+// exp(-j 2 pi t / n), t < n, built in FP64.
+sarbp_status az_twiddles(WkCache* wc, int n, const float2** out) {
+  auto it = wc->az_twiddles.find(n);
+  if (it != wc->az_twiddles.end()) {
+    *out = it->second;
+    return SARBP_OK;
+  }
+  std::vector<float2> h((size_t)n);
+  for (int t = 0; t < n; ++t) {
+    const double ang = -2.0 * kPiD * (double)t / (double)n;
+    h[t] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+  }
+  float2* d = nullptr;
+  hipError_t e = hipMalloc(&d, h.size() * sizeof(float2));
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc(azimuth twiddles)");
+  if ((e = hipMemcpy(d, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice)) != hipSuccess) {
+    (void)hipFree(d);
+    return hip_fail(e, "hipMemcpy(azimuth twiddles)");
+  }
+  wc->az_twiddles[n] = d;
   *out = d;
   return SARBP_OK;
 }
@@ -851,16 +945,36 @@ sarbp_status omega_k_enqueue(WkCache* wc, const sarbp_cf32* d_echo, int na_in, i
   // range inverse DFT, azimuth inverse FFT, transpose into the image.
   const float2* tab = nullptr;
   hipfftHandle az_fwd = nullptr, az_inv = nullptr;
+  const int alg = az_lg(na);
+  const float2* az_tw = nullptr;
   if ((st = chirp_tables(wc, nr, lg, &tab)) != SARBP_OK) return st;
-  if ((st = row_plan(wc, na, nr_in, &az_fwd)) != SARBP_OK) return st;
+  if (alg) {
+    if ((st = az_twiddles(wc, na, &az_tw)) != SARBP_OK) return st;
+  } else if ((st = row_plan(wc, na, nr_in, &az_fwd)) != SARBP_OK) {
+    return st;
+  }
   if ((st = row_plan(wc, na, nr, &az_inv)) != SARBP_OK) return st;
   if ((st = work_buffer(wc, (size_t)na * nr)) != SARBP_OK) return st;
   float2* w = wc->work;
-  pa.out = w;
-  hipLaunchKernelGGL(wk_prep_t, dim3((unsigned)((nr_in + 63) / 64), (unsigned)((na + 63) / 64)), dim3(256), 0, s,
-                     pa);
-  if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "wk_prep_t launch");
-  if ((st = exec_fft(az_fwd, w + (size_t)pad_rg * na, HIPFFT_FORWARD, s)) != SARBP_OK) return st;
+  if (alg) {
+    AzArgs az{};
+    az.echo = pa.echo;
+    az.out = w;
+    az.tw = az_tw;
+    az.na = na;
+    az.na_in = na_in;
+    az.nr_in = nr_in;
+    az.pad_az = pad_az;
+    az.pad_rg = pad_rg;
+    az.rev_per_line = pa.rev_per_line;
+    if ((e = launch_az(az, alg, s)) != hipSuccess) return hip_fail(e, "wk_az_fwd launch");
+  } else {
+    pa.out = w;
+    hipLaunchKernelGGL(wk_prep_t, dim3((unsigned)((nr_in + 63) / 64), (unsigned)((na + 63) / 64)), dim3(256), 0,
+                       s, pa);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "wk_prep_t launch");
+    if ((st = exec_fft(az_fwd, w + (size_t)pad_rg * na, HIPFFT_FORWARD, s)) != SARBP_OK) return st;
+  }
   RowArgs r{};
   r.st = a;
   r.st.in = w;
