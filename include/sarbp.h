@@ -160,6 +160,25 @@ sarbp_status sarbp_launch_backprojection(sarbp_cf32* Srnm, sarbp_cf32* Image,
                                          sarbp_sar_params params, int32_t Na,
                                          int32_t Nr);
 
+/*
+ * Drop-in for launchGenerateEcho(Srnm, Ptarget, ta, tr, params, Mslow, Nfast)
+ * (BPA_CUDA_kernels.cu:79-89, kernel generateEchoKernel :3-45): device Srnm
+ * [Mslow][Nfast] (written), device Ptarget [Ntarget][3] (azimuth, ground
+ * range, sigma), device ta [Mslow] and tr [Nfast] (slow and fast time, s);
+ * params by value (c, lambda, Kr, Tr, vr, H, La and Ntarget are used).
+ * Sample (i, j) = sum over targets k with 0 < t < Tr and |Az| < La/2 of
+ * sigma_k exp(j (pi Kr t^2 - 4 pi R / lambda)), Az = ta[i] vr - x_k,
+ * R = sqrt(Az^2 + y_k^2 + H^2), t = tr[j] - 2 R / c.  Range, gate and phase
+ * are evaluated in FP64 (the reference's FP32 phase, ~1e6 rad, carries
+ * ~0.1 rad of error).  Synchronous like the original (null stream, then
+ * hipDeviceSynchronize), a status code instead of exit(); the current HIP
+ * device.
+ */
+sarbp_status sarbp_launch_generate_echo(sarbp_cf32* Srnm, const float* Ptarget,
+                                        const float* ta, const float* tr,
+                                        sarbp_sar_params params, int32_t Mslow,
+                                        int32_t Nfast);
+
 /* ---- plans ------------------------------------------------------------------ */
 
 /*

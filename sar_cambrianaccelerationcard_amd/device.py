@@ -160,3 +160,26 @@ def stolt_sinc(ctx: native.Context, data, fr_axis, fa_axis, *, f0: float, c: flo
     if sync:
         torch.cuda.synchronize(dev)
     return out
+
+
+def generate_echo(ptarget, ta, tr, params: native.SarParams, *, device: int = 0):
+    """Point-target raw echoes [len(ta)][len(tr)] on the device
+    (sarbp_launch_generate_echo, the reference's launchGenerateEcho): targets
+    [Ntarget][3] (azimuth, ground range, sigma), slow times ta and fast times
+    tr, all float32; returns the complex64 tensor (synchronous)."""
+    torch = require_gpu()
+    dev = torch.device("cuda", device)
+    pt = torch.from_numpy(np.ascontiguousarray(ptarget, dtype=np.float32).reshape(-1)).to(dev)
+    d_ta = torch.from_numpy(np.ascontiguousarray(ta, dtype=np.float32)).to(dev)
+    d_tr = torch.from_numpy(np.ascontiguousarray(tr, dtype=np.float32)).to(dev)
+    if pt.numel() < 3 * params.Ntarget:
+        raise ValueError("ptarget holds fewer than Ntarget targets")
+    out = torch.empty((d_ta.numel(), d_tr.numel()), dtype=torch.complex64, device=dev)
+    torch.cuda.synchronize(dev)  # the inputs are on the device before the null-stream launch
+    lib = native.load()
+    native.check(lib.sarbp_launch_generate_echo(native.C.c_void_p(out.data_ptr()), native.C.c_void_p(pt.data_ptr()),
+                                                native.C.c_void_p(d_ta.data_ptr()),
+                                                native.C.c_void_p(d_tr.data_ptr()), params,
+                                                d_ta.numel(), d_tr.numel()),
+                 "sarbp_launch_generate_echo")
+    return out

@@ -82,7 +82,9 @@ class SarParams(C.Structure):
         p.c, p.fc, p.Kr, p.Tr, p.Br, p.Fr = 3e8, 5.3e9, 4e12, 2.5e-6, 1e8, 1.2e8
         p.vr, p.H, p.La, p.Rg0, p.Az0 = 150.0, 5000.0, 4.0, 10000.0, 0.0
         p.lambda_ = p.c / p.fc
-        p.Nfast = int(p.Tr * p.Fr)
+        # static_cast<int>(Tr * Fr) of two floats: 2.5e-6f * 1.2e8f rounds to 300.0f
+        import numpy as np
+        p.Nfast = int(np.float32(p.Tr) * np.float32(p.Fr))
         p.Mslow = 1024
         p.Ntarget = 5
         return p
@@ -180,6 +182,8 @@ _SIGS = {
     "sarbp_sinc8_coefficients_whole": (C.c_int, [C.c_double, C.POINTER(C.c_float)]),
     "sarbp_launch_backprojection": (C.c_int, [C.c_void_p, C.c_void_p, SarParams, C.c_int32,
                                               C.c_int32]),
+    "sarbp_launch_generate_echo": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, SarParams,
+                                             C.c_int32, C.c_int32]),
     "sarbp_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
     "sarbp_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_uint64)]),
     "sarbp_server_submit_raw": (C.c_int, [C.c_void_p, C.POINTER(RawRequest), C.POINTER(C.c_uint64)]),
