@@ -10,13 +10,25 @@
 //             [--spacing 0.5] [--interp nearest|linear|sinc8]
 //             [--precision ref|f64|f32] [--targets 3] [--target-spacing 20]
 //             [--snr-db 20 | --no-noise] [--device 0] [--out sar_image.bin]
+//   sarbp_run --reference-scene [--rg0 10000] [--h 5000] [--la 4] [--mslow 1024]
+//             [--nfast 300] [--device 0] [--out sar_image.bin]
 //
 // Scene: X band (9.6 GHz), LFM 150 MHz / 2 us, 200 MHz complex sampling, 4x
 // range upsampling; straight track at 150 m/s, 3000 m altitude, 1 kHz PRF,
 // centred on the scene; a K x K grid of unit point targets around 4000 m
 // ground range on image pixels.
+//
+// --reference-scene runs the reference's own chain with its own inputs: the
+// SARParams defaults (BPA_CUDA.h:28-50, overridable), the five targets and
+// the slow/fast-time axes of initializeArrays (BPA_CUDA.cpp:25-48), the echo
+// on the device (sarbp_launch_generate_echo for launchGenerateEcho) and the
+// back-projection onto the [Mslow][Nfast] image (sarbp_launch_backprojection
+// for launchBackProjection, BPA_CUDA.cpp:84).  With the defaults the fast-time
+// axis ends 2.5 us after transmission while the targets' echoes arrive after
+// ~67 us, so the echo and the image are zero, as the reference's are.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -36,6 +48,9 @@ struct Options {
   int pulses = 512, samples = 2048, nx = 512, ny = 512, targets = 3, device = 0;
   double spacing = 0.5, target_spacing = 20.0, snr_db = 20.0;
   bool noise = true;
+  bool reference_scene = false;
+  float rg0 = 10000.0f, h = 5000.0f, la = 4.0f;
+  int mslow = 1024, nfast = -1;  // nfast -1: (int)(Tr * Fr) as SARParams computes it
   std::string interp = "linear", precision = "ref", out = "sar_image.bin";
 };
 
@@ -44,6 +59,8 @@ void usage() {
                "usage: sarbp_run [--pulses N] [--samples N] [--nx N] [--ny N] [--spacing M]\n"
                "                 [--interp nearest|linear|sinc8] [--precision ref|f64|f32]\n"
                "                 [--targets K] [--target-spacing M] [--snr-db DB | --no-noise]\n"
+               "                 [--device D] [--out FILE]\n"
+               "       sarbp_run --reference-scene [--rg0 M] [--h M] [--la M] [--mslow N] [--nfast N]\n"
                "                 [--device D] [--out FILE]\n");
 }
 
@@ -66,12 +83,19 @@ bool parse(int argc, char** argv, Options* o) {
     else if (a == "--snr-db") o->snr_db = std::atof(next("--snr-db"));
     else if (a == "--no-noise") o->noise = false;
     else if (a == "--device") o->device = std::atoi(next("--device"));
+    else if (a == "--reference-scene") o->reference_scene = true;
+    else if (a == "--rg0") o->rg0 = (float)std::atof(next("--rg0"));
+    else if (a == "--h") o->h = (float)std::atof(next("--h"));
+    else if (a == "--la") o->la = (float)std::atof(next("--la"));
+    else if (a == "--mslow") o->mslow = std::atoi(next("--mslow"));
+    else if (a == "--nfast") o->nfast = std::atoi(next("--nfast"));
     else if (a == "--out") o->out = next("--out");
     else if (a == "-h" || a == "--help") return false;
     else throw std::runtime_error("unknown option " + a);
   }
   if (o->pulses < 1 || o->samples < 16 || o->nx < 1 || o->ny < 1 || !(o->spacing > 0) || o->targets < 0)
     throw std::runtime_error("sizes must be positive");
+  if (o->mslow < 2 || o->nfast == 0 || o->nfast < -1) throw std::runtime_error("--mslow >= 2 and --nfast >= 1");
   return true;
 }
 
@@ -245,6 +269,74 @@ class SarProcessor {
   double r0_ = 0, dr_ = 0;
 };
 
+// The reference's SAR_Processor with its own inputs (BPA_CUDA.cpp:11-100): the
+// axes in float exactly as initializeArrays computes them.
+int run_reference_scene(const Options& o) {
+  check_hip(hipSetDevice(o.device), "hipSetDevice");
+  sarbp_sar_params p{};
+  p.c = 3e8f;
+  p.fc = 5.3e9f;
+  p.Kr = 4e12f;
+  p.Tr = 2.5e-6f;
+  p.Br = 1e8f;
+  p.Fr = 1.2e8f;
+  p.vr = 150.0f;
+  p.H = o.h;
+  p.La = o.la;
+  p.Rg0 = o.rg0;
+  p.Az0 = 0.0f;
+  p.lambda = p.c / p.fc;
+  p.Nfast = o.nfast > 0 ? o.nfast : static_cast<int>(p.Tr * p.Fr);
+  p.Mslow = o.mslow;
+  p.Ntarget = 5;
+  const float pt[15] = {p.Az0 - 10, p.Rg0 - 20, 1.0f, p.Az0 + 20, p.Rg0 + 30, 0.8f, p.Az0 - 30, p.Rg0 + 10, 1.2f,
+                        p.Az0 + 40, p.Rg0 - 40, 0.9f, p.Az0,      p.Rg0,      1.5f};
+  const float ta_start = -p.La / (2 * p.vr), ta_end = p.La / (2 * p.vr);
+  const float ta_step = (ta_end - ta_start) / (p.Mslow - 1);
+  std::vector<float> ta((size_t)p.Mslow), tr((size_t)p.Nfast);
+  for (int i = 0; i < p.Mslow; ++i) ta[i] = ta_start + i * ta_step;
+  const float tr_step = 1.0f / p.Fr;
+  for (int i = 0; i < p.Nfast; ++i) tr[i] = i * tr_step;
+  const size_t n = (size_t)p.Mslow * p.Nfast;
+  DeviceBuffer<sarbp_cf32> d_srnm(n), d_img(n);
+  DeviceBuffer<float> d_pt(15), d_ta(ta.size()), d_tr(tr.size());
+  check_hip(hipMemcpy(d_pt.p, pt, sizeof pt, hipMemcpyHostToDevice), "H2D targets");
+  check_hip(hipMemcpy(d_ta.p, ta.data(), ta.size() * sizeof(float), hipMemcpyHostToDevice), "H2D ta");
+  check_hip(hipMemcpy(d_tr.p, tr.data(), tr.size() * sizeof(float), hipMemcpyHostToDevice), "H2D tr");
+  const auto t0 = std::chrono::steady_clock::now();
+  check(sarbp_launch_generate_echo(d_srnm.p, d_pt.p, d_ta.p, d_tr.p, p, p.Mslow, p.Nfast),
+        "sarbp_launch_generate_echo");
+  const auto t1 = std::chrono::steady_clock::now();
+  check(sarbp_launch_backprojection(d_srnm.p, d_img.p, p, p.Mslow, p.Nfast), "sarbp_launch_backprojection");
+  const auto t2 = std::chrono::steady_clock::now();
+  std::vector<sarbp_cf32> echo(n), img(n);
+  check_hip(hipMemcpy(echo.data(), d_srnm.p, n * sizeof(sarbp_cf32), hipMemcpyDeviceToHost), "D2H echo");
+  check_hip(hipMemcpy(img.data(), d_img.p, n * sizeof(sarbp_cf32), hipMemcpyDeviceToHost), "D2H image");
+  size_t nz_echo = 0;
+  double peak = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    nz_echo += (echo[i].re != 0.0f || echo[i].im != 0.0f);
+    peak = std::max(peak, std::hypot((double)img[i].re, (double)img[i].im));
+  }
+  auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+  std::printf("reference scene: %d x %d, echo %.2f ms (%zu non-zero samples), BP %.2f ms, image peak |I| = %.4g\n",
+              p.Mslow, p.Nfast, ms(t0, t1), nz_echo, ms(t1, t2), peak);
+  FILE* f = std::fopen(o.out.c_str(), "wb");
+  if (!f) throw std::runtime_error("cannot open " + o.out);
+  const size_t w = std::fwrite(img.data(), sizeof(sarbp_cf32), n, f);
+  std::fclose(f);
+  if (w != n) throw std::runtime_error("short write to " + o.out);
+  FILE* j = std::fopen((o.out + ".json").c_str(), "w");
+  if (!j) throw std::runtime_error("cannot open " + o.out + ".json");
+  std::fprintf(j,
+               "{\"format\": \"raw complex64 interleaved (re, im), row-major [ny][nx]\", \"ny\": %d, \"nx\": %d, "
+               "\"dtype\": \"complex64\", \"bytes\": %zu, \"meta\": {\"writer\": \"sarbp_run --reference-scene\", "
+               "\"Rg0\": %.6g, \"H\": %.6g, \"La\": %.6g, \"Mslow\": %d, \"Nfast\": %d, \"echo_nonzero\": %zu}}\n",
+               p.Mslow, p.Nfast, n * sizeof(sarbp_cf32), p.Rg0, p.H, p.La, p.Mslow, p.Nfast, nz_echo);
+  std::fclose(j);
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -254,6 +346,7 @@ int main(int argc, char** argv) {
       usage();
       return 2;
     }
+    if (o.reference_scene) return run_reference_scene(o);
     SarProcessor processor(o);
     processor.process();
     processor.save(o.out);

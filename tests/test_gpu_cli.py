@@ -163,3 +163,40 @@ def test_sarwk_run_rejects_bad_input(tmp_path):
     assert r.returncode == 1 and "--window" in r.stderr
     r = subprocess.run([WK_EXE, "--range-ref", "middle"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "--range-ref" in r.stderr
+
+
+def test_sarbp_run_reference_scene(tmp_path):
+    # The reference's own chain and inputs: with its defaults the echo and the
+    # image are zero (fast time ends 2.5 us after transmission, the echoes
+    # arrive after ~67 us); a scene within its 375 m window gives the image of
+    # the same two library calls made here, bit for bit.
+    import ctypes as C
+    import torch
+    import echo_oracle as E
+    from sar_cambrianaccelerationcard_amd import device, native
+    out = str(tmp_path / "ref.bin")
+    print("[cli] sarbp_run --reference-scene", flush=True)
+    r = subprocess.run([EXE, "--reference-scene", "--out", out], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0 and "(0 non-zero samples)" in r.stdout
+    img = io.load_image(out, verify=False)
+    assert img.shape == (1024, 300) and not np.any(img)
+
+    out = str(tmp_path / "near.bin")
+    print("[cli] sarbp_run --reference-scene --rg0 250 --h 200 --la 60 --mslow 256", flush=True)
+    r = subprocess.run([EXE, "--reference-scene", "--rg0", "250", "--h", "200", "--la", "60", "--mslow", "256",
+                        "--out", out], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    img = io.load_image(out, verify=False)
+    p = native.SarParams.reference_defaults()
+    p.Rg0, p.H, p.La, p.Mslow = 250.0, 200.0, 60.0, 256
+    pt, ta, tr = E.reference_scene(p)
+    echo = device.generate_echo(pt, ta, tr, p)
+    assert torch.count_nonzero(echo).item() > 0
+    d_img = torch.zeros_like(echo)
+    native.check(native.load().sarbp_launch_backprojection(C.c_void_p(echo.data_ptr()), C.c_void_p(d_img.data_ptr()),
+                                                          p, p.Mslow, p.Nfast), "sarbp_launch_backprojection")
+    ref = d_img.cpu().numpy()
+    assert img.shape == ref.shape == (256, 300) and np.any(ref)
+    assert np.array_equal(img.view(np.uint64), ref.view(np.uint64))
