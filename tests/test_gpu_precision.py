@@ -25,7 +25,6 @@ import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HALF = 32
 
 
@@ -70,8 +69,8 @@ def test_precision_variants(ctx):
             table.append(r)
             print(f"  {prec:4s} {dtype:7s} rel_l2={r['rel_l2']:.2e} phase max {r['max_phase_err_deg']:.2e} deg "
                   f"pslr/islr diff {r['max_pslr_islr_diff_db']:.2e} dB peaks {peaks}", flush=True)
-    out = os.path.join(ROOT, "gpurun_out")
-    if os.path.isdir(out):
+    out = os.environ.get("SARBP_RESULTS_DIR")  # optional: keep the table as JSON
+    if out and os.path.isdir(out):
         with open(os.path.join(out, "precision_variants.json"), "w") as f:
             json.dump(table, f, indent=1)
     by = {(r["precision"], r["dtype"]): r for r in table}
