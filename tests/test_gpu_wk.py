@@ -336,10 +336,11 @@ def test_omega_k_centred_range_reference(ctx, f_nc, window):
         assert abs(mg["concentration"] - mr["concentration"]) < 1e-3
 
 
-@pytest.mark.parametrize("na_in,nr_in", [(2, 3), (45, 101), (8, 2458), (8, 2459), (64, 3100)])
+@pytest.mark.parametrize("na_in,nr_in", [(2, 3), (45, 101), (8, 2458), (8, 2459), (64, 3100), (768, 20)])
 def test_omega_k_centred_random_echo(ctx, na_in, nr_in):
     # Odd and even padded line lengths (the output shift's constant phase
-    # differs) on 8192- and 16384-point transforms.
+    # differs) on 8192- and 16384-point transforms; 768 lines pad to 1024
+    # (the fused azimuth kernel).
     echo = _rand((na_in, nr_in), 5 * na_in + nr_in)
     wp = O.WkScene(na_in=na_in, nr_in=nr_in, r0=2300.0).params(range_ref="centre")
     got = _omega_k(ctx, echo, wp)
