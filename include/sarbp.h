@@ -540,8 +540,9 @@ sarbp_status sarbp_wk_padded_size(int32_t na_in, int32_t nr_in, int32_t* na, int
  * by 1/(na nr) (fft.cc:200-359).  The range DFT, reference function, Stolt
  * step and range inverse DFT run as one kernel per azimuth line (chirp-z
  * transforms in LDS; lines over 4096 samples need 128 KiB of LDS per
- * workgroup, else SARBP_ERR_UNSUPPORTED) between hipFFT azimuth
- * transforms.  Plans, chirp tables
+ * workgroup, else SARBP_ERR_UNSUPPORTED) between azimuth transforms (LDS
+ * kernels when na is a power of two from 1024 to 8192, else hipFFT).  Plans,
+ * twiddle and chirp tables
  * and a workspace of na x nr complex64 words are kept by the
  * context (the first call of a size builds them).  Calls on one context share
  * them and are ordered on the device: each call's stream first waits for the

@@ -501,6 +501,47 @@ __global__ __launch_bounds__((1 << LG) / 16) void wk_az_fwd(AzArgs a) {
   }
 }
 
+// Azimuth inverse FFT of image column r ([na][nr], na = 2^LG) in LDS, in
+// place: the column is read whole before it is written; columns in flight on
+// one XCD share cache lines in its L2.  Unscaled (the line kernel applies
+// 1/(na nr)).
+template <int LG>
+__global__ __launch_bounds__((1 << LG) / 16) void wk_az_inv_col(float2* img, const float2* __restrict__ tw, int nr) {
+  constexpr int NF = 1 << LG, T = NF / 16;
+  extern __shared__ float2 buf[];
+  const int r = xcd_line(blockIdx.x, nr);
+  const int tid = threadIdx.x;
+#pragma unroll 4
+  for (int e = 0; e < 16; ++e) {
+    const int m = tid + e * T;
+    buf[fft::slot<LG>(m)] = img[(size_t)m * nr + r];
+  }
+  __syncthreads();
+  fft::fft_lds<LG, T, true>(buf, tw);
+#pragma unroll 4
+  for (int e = 0; e < 16; ++e) {
+    const int k = tid + e * T;
+    img[(size_t)k * nr + r] = buf[fft::slot<LG>(k)];
+  }
+}
+
+template <int LG>
+hipError_t launch_az_inv_lg(float2* img, const float2* tw, int nr, hipStream_t s) {
+  hipLaunchKernelGGL(wk_az_inv_col<LG>, dim3((unsigned)nr), dim3((1 << LG) / 16),
+                     (size_t)fft::lds_words<LG>() * sizeof(float2), s, img, tw, nr);
+  return hipGetLastError();
+}
+
+hipError_t launch_az_inv(float2* img, const float2* tw, int nr, int lg, hipStream_t s) {
+  switch (lg) {
+    case 10: return launch_az_inv_lg<10>(img, tw, nr, s);
+    case 11: return launch_az_inv_lg<11>(img, tw, nr, s);
+    case 12: return launch_az_inv_lg<12>(img, tw, nr, s);
+    case 13: return launch_az_inv_lg<13>(img, tw, nr, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 // log2 na when na is a power of two the fused azimuth kernel takes, else 0.
 int az_lg(int na) {
   for (int lg = 10; lg <= 13; ++lg)
@@ -953,7 +994,7 @@ sarbp_status omega_k_enqueue(WkCache* wc, const sarbp_cf32* d_echo, int na_in, i
   } else if ((st = row_plan(wc, na, nr_in, &az_fwd)) != SARBP_OK) {
     return st;
   }
-  if ((st = row_plan(wc, na, nr, &az_inv)) != SARBP_OK) return st;
+  if (!alg && (st = row_plan(wc, na, nr, &az_inv)) != SARBP_OK) return st;
   if ((st = work_buffer(wc, (size_t)na * nr)) != SARBP_OK) return st;
   float2* w = wc->work;
   if (alg) {
@@ -985,12 +1026,22 @@ sarbp_status omega_k_enqueue(WkCache* wc, const sarbp_cf32* d_echo, int na_in, i
   r.in_sample = na;
   r.out_line = 1;
   r.out_sample = na;
+  if (alg) {
+    // lines written as rows of the image; the azimuth inverse runs on its columns
+    r.st.out = img;
+    r.out_line = nr;
+    r.out_sample = 1;
+  }
   r.tw = tab;
   r.bh = tab + ((size_t)1 << lg);
   r.chirp = tab + ((size_t)2 << lg);
   r.out_shift = out_shift;
   r.out_rot = make_float2((float)(std::cos(out_phase) * a.scale), (float)(std::sin(out_phase) * a.scale));
   if ((e = launch_rows(r, lg, s)) != hipSuccess) return hip_fail(e, "wk_rows launch");
+  if (alg) {
+    if ((e = launch_az_inv(img, az_tw, nr, alg, s)) != hipSuccess) return hip_fail(e, "wk_az_inv_col launch");
+    return SARBP_OK;
+  }
   if ((st = exec_fft(az_inv, w, HIPFFT_BACKWARD, s)) != SARBP_OK) return st;
   if ((e = launch_transpose(w, img, nr, na, s)) != hipSuccess) return hip_fail(e, "wk_transpose launch");
   return SARBP_OK;
