@@ -525,8 +525,43 @@ __global__ __launch_bounds__((1 << LG) / 16) void wk_az_inv_col(float2* img, con
   }
 }
 
+// Two adjacent columns (2p, 2p + 1) per workgroup for even nr: 16-byte
+// accesses per image row.
+template <int LG>
+__global__ __launch_bounds__((1 << LG) / 16) void wk_az_inv_col2(float2* img, const float2* __restrict__ tw, int nr) {
+  constexpr int NF = 1 << LG, T = NF / 16, W = fft::lds_words<LG>();
+  extern __shared__ float2 buf[];
+  const int r = 2 * xcd_line(blockIdx.x, nr >> 1);
+  const int tid = threadIdx.x;
+#pragma unroll 4
+  for (int e = 0; e < 16; ++e) {
+    const int m = tid + e * T;
+    const float4 v = *reinterpret_cast<const float4*>(img + (size_t)m * nr + r);
+    buf[fft::slot<LG>(m)] = make_float2(v.x, v.y);
+    buf[W + fft::slot<LG>(m)] = make_float2(v.z, v.w);
+  }
+  __syncthreads();
+  fft::fft_lds<LG, T, true>(buf, tw);
+  fft::fft_lds<LG, T, true>(buf + W, tw);
+#pragma unroll 4
+  for (int e = 0; e < 16; ++e) {
+    const int k = tid + e * T;
+    const float2 a = buf[fft::slot<LG>(k)], b = buf[W + fft::slot<LG>(k)];
+    *reinterpret_cast<float4*>(img + (size_t)k * nr + r) = make_float4(a.x, a.y, b.x, b.y);
+  }
+}
+
 template <int LG>
 hipError_t launch_az_inv_lg(float2* img, const float2* tw, int nr, hipStream_t s) {
+  if ((nr & 1) == 0) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&wk_az_inv_col2<LG>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)(2 * fft::lds_words<LG>() * sizeof(float2)));
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(wk_az_inv_col2<LG>, dim3((unsigned)(nr / 2)), dim3((1 << LG) / 16),
+                       (size_t)2 * fft::lds_words<LG>() * sizeof(float2), s, img, tw, nr);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(wk_az_inv_col<LG>, dim3((unsigned)nr), dim3((1 << LG) / 16),
                      (size_t)fft::lds_words<LG>() * sizeof(float2), s, img, tw, nr);
   return hipGetLastError();
