@@ -509,6 +509,24 @@ typedef enum sarbp_wk_range_ref {
   SARBP_WK_RANGE_REF_CENTRE = 1
 } sarbp_wk_range_ref;
 
+/* The reference's other Stolt step, by grid sampling: performStoltInterpolation
+ * (code/SAR_WK/wk_v5.2.3/task/stolt.cc:38-318, declared task/task.h:27-34;
+ * its call in process_wk, wk.cc:272, is commented out), on device
+ * d_in [na][nr] into d_out (may be d_in).  Each line is rotated by nr/2
+ * (column w holds input (w + nr - nr/2) mod nr) and output n is the bilinear,
+ * zero-padded, align-corners grid sample of it at
+ * x = (sqrt((f0 + fr_axis[n])^2 - c^2 fa_axis[m]^2 / (4 vr^2)) - f0) / (fr nr / 2),
+ * on its own row: linear interpolation along the line at
+ * ((x + 1) / 2) (nr - 1), zeros outside.  The mapping is FP64 rounded to
+ * FP32 and the grid arithmetic FP32, as the reference's.  fr is the
+ * frequency step the reference is called with (Fr / Nr); the output is not
+ * rotated back.  Axes as for sarbp_stolt_sinc; nr <= 8192. */
+sarbp_status sarbp_stolt_gridsample(sarbp_context* ctx, const sarbp_cf32* d_in,
+                                    sarbp_cf32* d_out, const double* h_fr_axis,
+                                    const double* h_fa_axis, double f0, double c,
+                                    double vr, int32_t na, int32_t nr, double fr,
+                                    void* stream);
+
 typedef struct sarbp_wk_params {
   double c;       /* propagation speed, m/s                                   */
   double f0;      /* carrier frequency, Hz                                    */

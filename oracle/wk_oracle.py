@@ -108,6 +108,46 @@ def stolt_sinc(S: np.ndarray, fr_axis, fa_axis, *, f0: float, c: float, vr: floa
     return out
 
 
+def stolt_gridsample(S: np.ndarray, fr_axis, fa_axis, *, f0: float, c: float, vr: float,
+                     fr: float) -> np.ndarray:
+    """The reference's GridSample Stolt step, performStoltInterpolation
+    (task/stolt.cc:38-318, declared task/task.h:27-34; its call in process_wk,
+    wk.cc:272, is commented out), on [Na][Nr] S:
+      fr_new[m, n] = sqrt((f0 + fr_axis[n])^2 - c^2 fa[m]^2 / (4 Vr^2)) - f0,
+                     FP64, stored as float32 (calculateNewFrequencyMapping,
+                     stolt.cc:19-35);
+      row m is rotated by mid = Nr/2: S'[m, w] = S[m, (w + Nr - mid) mod Nr]
+                     (stolt.cc:121-130);
+      x[m, n] = float32(fr_new[m, n] / (fr Nr / 2))  (the grid, :199-206);
+      bilinear grid sampling with align_corners and zero padding on S'
+                     (:73-78, :246-272), the row coordinate being row m itself:
+      ix = ((x + 1) / 2) (Nr - 1) in float32, i0 = floor(ix), t = ix - i0,
+      out[m, n] = (1 - t) S'[m, i0] + t S'[m, i0 + 1], samples outside the
+      row are zero.  The output is not rotated back.  fr is the bin spacing
+      the caller passes (fr_gap = Fr / Nr in wk.cc:272)."""
+    S = np.asarray(S, dtype=np.complex128)
+    na, nr = S.shape
+    fr_axis = np.asarray(fr_axis, dtype=np.float64)
+    fa_axis = np.asarray(fa_axis, dtype=np.float64)
+    k = c * c / (4.0 * vr * vr)
+    f = f0 + fr_axis[None, :]
+    fr_new = (np.sqrt(f * f - k * fa_axis[:, None] ** 2) - f0).astype(np.float32)
+    x = (fr_new.astype(np.float64) / (fr * nr * 0.5)).astype(np.float32)
+    one, two = np.float32(1.0), np.float32(2.0)
+    ix = (((x + one) / two) * np.float32(nr - 1)).astype(np.float32)
+    i0 = np.floor(ix)
+    t = (ix - i0).astype(np.float64)
+    i0 = i0.astype(np.int64)
+    shifted = np.roll(S, nr // 2, axis=1)
+    rows = np.arange(na)[:, None]
+
+    def tap(i):
+        ok = (i >= 0) & (i < nr)
+        return np.where(ok, shifted[rows, np.clip(i, 0, nr - 1)], 0.0)
+
+    return (1.0 - t) * tap(i0) + t * tap(i0 + 1)
+
+
 def reference_phase(fr_axis, fa_axis, *, f0: float, c: float, vr: float, kr: float, r0: float) -> np.ndarray:
     k = c * c / (4.0 * vr * vr)
     s = np.sqrt(np.maximum((f0 + fr_axis[None, :]) ** 2 - k * fa_axis[:, None] ** 2, 0.0))

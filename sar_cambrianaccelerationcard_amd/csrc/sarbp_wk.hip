@@ -271,6 +271,81 @@ __global__ __launch_bounds__(kWkThreads) void wk_stolt(StoltArgs a) {
   }
 }
 
+// ---- GridSample Stolt ------------------------------------------------------------
+
+// The reference's GridSample Stolt step (performStoltInterpolation,
+// task/stolt.cc:38-318): line m rotated by nr/2 (column w holds input
+// (w + nr - nr/2) mod nr), output n sampled from it by linear interpolation
+// with zeros outside at ix = ((x + 1) / 2) (nr - 1),
+// x = float(float(sqrt((f0 + fr[n])^2 - kk fa[m]^2) - f0) / (fr nr / 2)) --
+// the bilinear, zero-padded, align-corners grid sample with the row coordinate
+// on row m.  Every step up to the weights is evaluated in the reference's
+// precision and order (FP64 mapping rounded to FP32, FP32 grid arithmetic,
+// no contraction), so the sample positions are the reference's.
+struct GridArgs {
+  const float2* in;
+  float2* out;  // may equal in
+  const double* fr_axis;
+  const double* fa_axis;
+  int na, nr;
+  double f0, kk;      // kk = c^2 / (4 vr^2)
+  double half_band;   // fr nr / 2
+};
+
+template <int K>
+__global__ __launch_bounds__(kWkThreads) void wk_stolt_grid(GridArgs a) {
+#pragma clang fp contract(off)
+  extern __shared__ float2 line[];
+  const int m = blockIdx.x;
+  const int nr = a.nr, mid = nr / 2;
+  const size_t base = (size_t)m * (size_t)nr;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int w = threadIdx.x + k * kWkThreads;
+    if (w < nr) {
+      int src = w + nr - mid;
+      if (src >= nr) src -= nr;
+      line[w] = a.in[base + src];
+    }
+  }
+  __syncthreads();
+  const double fa = a.fa_axis[m];
+  const double kfa2 = a.kk * (fa * fa);
+  const float span = (float)(nr - 1);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int n = threadIdx.x + k * kWkThreads;
+    if (n >= nr) continue;
+    const double fv = a.f0 + a.fr_axis[n];
+    const float fr_new = (float)(sqrt(fv * fv - kfa2) - a.f0);
+    const float x = (float)((double)fr_new / a.half_band);
+    const float ix = ((x + 1.0f) / 2.0f) * span;
+    const float f0i = floorf(ix);
+    const float t = ix - f0i;
+    const int i0 = (int)f0i;
+    float2 v0 = make_float2(0.0f, 0.0f), v1 = v0;
+    if (i0 >= 0 && i0 < nr) v0 = line[i0];
+    if (i0 + 1 >= 0 && i0 + 1 < nr) v1 = line[i0 + 1];
+    const float u = 1.0f - t;
+    a.out[base + n] = make_float2(u * v0.x + t * v1.x, u * v0.y + t * v1.y);
+  }
+}
+
+hipError_t launch_stolt_grid(const GridArgs& a, hipStream_t s) {
+  const int k = (a.nr + kWkThreads - 1) / kWkThreads;
+  const size_t lds = (size_t)a.nr * sizeof(float2);
+  const dim3 grid((unsigned)a.na), block(kWkThreads);
+  if (k <= 4)
+    hipLaunchKernelGGL(wk_stolt_grid<4>, grid, block, lds, s, a);
+  else if (k <= 8)
+    hipLaunchKernelGGL(wk_stolt_grid<8>, grid, block, lds, s, a);
+  else if (k <= 16)
+    hipLaunchKernelGGL(wk_stolt_grid<16>, grid, block, lds, s, a);
+  else
+    hipLaunchKernelGGL(wk_stolt_grid<32>, grid, block, lds, s, a);
+  return hipGetLastError();
+}
+
 // ---- fused line kernel ---------------------------------------------------------
 
 // Chirp-z tables of one (nr, LG): b[n] = exp(j pi n^2 / nr) for n < nr;
@@ -836,6 +911,52 @@ sarbp_status omega_k_enqueue(WkCache* wc, const sarbp_cf32* d_echo, int na_in, i
 
 using namespace sarbp;
 
+namespace sarbp {
+namespace {
+
+// Stages the Stolt steps' axes (host FP64, fr [nr] then fa [na]) on the
+// device; a call whose axes equal the last upload only reuses them, any other
+// first waits for the previous Stolt kernel (which may still read them).
+sarbp_status stage_axes(WkCache* wc, const double* h_fr_axis, const double* h_fa_axis, int na, int nr,
+                        hipStream_t s) {
+  hipError_t e;
+  const size_t n = (size_t)na + (size_t)nr;
+  // Axes equal to the last upload are already on the device: the call then
+  // only enqueues the kernel.  Otherwise the previous call's kernel, which
+  // may still read the axis buffers, is waited for before they are rewritten.
+  const bool same = wc->axes_nr == nr && wc->axes_na == na &&
+                    std::memcmp(wc->h_axes, h_fr_axis, (size_t)nr * sizeof(double)) == 0 &&
+                    std::memcmp(wc->h_axes + nr, h_fa_axis, (size_t)na * sizeof(double)) == 0;
+  if (!same && wc->done && (e = hipEventSynchronize(wc->done)) != hipSuccess)
+    return hip_fail(e, "hipEventSynchronize");
+  if (!same && n > wc->cap) {
+    (void)hipFree(wc->d_axes);
+    (void)hipHostFree(wc->h_axes);
+    wc->d_axes = nullptr;
+    wc->h_axes = nullptr;
+    wc->cap = 0;
+    if ((e = hipMalloc(&wc->d_axes, n * sizeof(double))) != hipSuccess) return hip_fail(e, "hipMalloc(axes)");
+    if ((e = hipHostMalloc(&wc->h_axes, n * sizeof(double), hipHostMallocDefault)) != hipSuccess)
+      return hip_fail(e, "hipHostMalloc(axes)");
+    wc->cap = n;
+  }
+  if (!wc->done && (e = hipEventCreateWithFlags(&wc->done, hipEventDisableTiming)) != hipSuccess)
+    return hip_fail(e, "hipEventCreate");
+  if (!same) {
+    wc->axes_nr = wc->axes_na = -1;
+    std::memcpy(wc->h_axes, h_fr_axis, (size_t)nr * sizeof(double));
+    std::memcpy(wc->h_axes + nr, h_fa_axis, (size_t)na * sizeof(double));
+    if ((e = hipMemcpyAsync(wc->d_axes, wc->h_axes, n * sizeof(double), hipMemcpyHostToDevice, s)) != hipSuccess)
+      return hip_fail(e, "hipMemcpyAsync(axes)");
+    wc->axes_nr = nr;
+    wc->axes_na = na;
+  }
+  return SARBP_OK;
+}
+
+}  // namespace
+}  // namespace sarbp
+
 extern "C" {
 
 sarbp_status sarbp_wk_padded_size(int32_t na_in, int32_t nr_in, int32_t* na, int32_t* nr) {
@@ -863,39 +984,9 @@ sarbp_status sarbp_stolt_sinc(sarbp_context* ctx, const sarbp_cf32* d_in, sarbp_
   if (dg.err != hipSuccess) return hip_fail(dg.err, "hipSetDevice");
   WkCache* wc = nullptr;
   if ((st = wk_cache(ctx, &wc)) != SARBP_OK) return st;
-  hipError_t e;
-  const size_t n = (size_t)na + (size_t)nr;
-  // Axes equal to the last upload are already on the device: the call then
-  // only enqueues the kernel.  Otherwise the previous call's kernel, which
-  // may still read the axis buffers, is waited for before they are rewritten.
-  const bool same = wc->axes_nr == nr && wc->axes_na == na &&
-                    std::memcmp(wc->h_axes, h_fr_axis, (size_t)nr * sizeof(double)) == 0 &&
-                    std::memcmp(wc->h_axes + nr, h_fa_axis, (size_t)na * sizeof(double)) == 0;
-  if (!same && wc->done && (e = hipEventSynchronize(wc->done)) != hipSuccess)
-    return hip_fail(e, "hipEventSynchronize");
-  if (!same && n > wc->cap) {
-    (void)hipFree(wc->d_axes);
-    (void)hipHostFree(wc->h_axes);
-    wc->d_axes = nullptr;
-    wc->h_axes = nullptr;
-    wc->cap = 0;
-    if ((e = hipMalloc(&wc->d_axes, n * sizeof(double))) != hipSuccess) return hip_fail(e, "hipMalloc(axes)");
-    if ((e = hipHostMalloc(&wc->h_axes, n * sizeof(double), hipHostMallocDefault)) != hipSuccess)
-      return hip_fail(e, "hipHostMalloc(axes)");
-    wc->cap = n;
-  }
-  if (!wc->done && (e = hipEventCreateWithFlags(&wc->done, hipEventDisableTiming)) != hipSuccess)
-    return hip_fail(e, "hipEventCreate");
   const hipStream_t s = static_cast<hipStream_t>(stream);
-  if (!same) {
-    wc->axes_nr = wc->axes_na = -1;
-    std::memcpy(wc->h_axes, h_fr_axis, (size_t)nr * sizeof(double));
-    std::memcpy(wc->h_axes + nr, h_fa_axis, (size_t)na * sizeof(double));
-    if ((e = hipMemcpyAsync(wc->d_axes, wc->h_axes, n * sizeof(double), hipMemcpyHostToDevice, s)) != hipSuccess)
-      return hip_fail(e, "hipMemcpyAsync(axes)");
-    wc->axes_nr = nr;
-    wc->axes_na = na;
-  }
+  if ((st = stage_axes(wc, h_fr_axis, h_fa_axis, na, nr, s)) != SARBP_OK) return st;
+  hipError_t e;
   StoltArgs a{};
   a.in = reinterpret_cast<const float2*>(d_in);
   a.out = reinterpret_cast<float2*>(d_out);
@@ -911,6 +1002,39 @@ sarbp_status sarbp_stolt_sinc(sarbp_context* ctx, const sarbp_cf32* d_in, sarbp_
   a.rfm = 0;
   a.scale = 1.0f;
   if ((e = launch_stolt(a, s)) != hipSuccess) return hip_fail(e, "wk_stolt launch");
+  if ((e = hipEventRecord(wc->done, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+  return SARBP_OK;
+}
+
+sarbp_status sarbp_stolt_gridsample(sarbp_context* ctx, const sarbp_cf32* d_in, sarbp_cf32* d_out,
+                                   const double* h_fr_axis, const double* h_fa_axis, double f0, double c,
+                                   double vr, int32_t na, int32_t nr, double fr, void* stream) {
+  if (!ctx) return fail(SARBP_ERR_INVALID_ARGUMENT, "context is NULL");
+  sarbp_status st = check_stolt(f0, c, vr, na, nr, 1.0, 2, SARBP_STOLT_FLOOR);
+  if (st != SARBP_OK) return st;
+  if (!(std::isfinite(fr) && fr > 0.0)) return fail(SARBP_ERR_INVALID_ARGUMENT, "fr = %g: must be > 0", fr);
+  if (na == 0 || nr == 0) return SARBP_OK;
+  if (!d_in || !d_out || !h_fr_axis || !h_fa_axis)
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "data or axis pointer is NULL");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  DeviceGuard dg(ctx->device);
+  if (dg.err != hipSuccess) return hip_fail(dg.err, "hipSetDevice");
+  WkCache* wc = nullptr;
+  if ((st = wk_cache(ctx, &wc)) != SARBP_OK) return st;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  if ((st = stage_axes(wc, h_fr_axis, h_fa_axis, na, nr, s)) != SARBP_OK) return st;
+  GridArgs a{};
+  a.in = reinterpret_cast<const float2*>(d_in);
+  a.out = reinterpret_cast<float2*>(d_out);
+  a.fr_axis = wc->d_axes;
+  a.fa_axis = wc->d_axes + nr;
+  a.na = na;
+  a.nr = nr;
+  a.f0 = f0;
+  a.kk = c * c / (4.0 * vr * vr);
+  a.half_band = fr * (double)nr * 0.5;
+  hipError_t e;
+  if ((e = launch_stolt_grid(a, s)) != hipSuccess) return hip_fail(e, "wk_stolt_grid launch");
   if ((e = hipEventRecord(wc->done, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
   return SARBP_OK;
 }

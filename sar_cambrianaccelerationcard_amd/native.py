@@ -201,6 +201,9 @@ _SIGS = {
     "sarbp_stolt_sinc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32,
                                    C.c_double, C.c_int32, C.c_int32, C.c_void_p]),
+    "sarbp_stolt_gridsample": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32,
+                                         C.c_double, C.c_void_p]),
     "sarbp_wk_padded_size": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "sarbp_omega_k": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.POINTER(WkParams),
                                 C.c_void_p, C.c_void_p]),
@@ -359,6 +362,20 @@ class Context:
                                         fr.ctypes.data_as(C.c_void_p), fa.ctypes.data_as(C.c_void_p),
                                         float(f0), float(c), float(vr), int(na), int(nr), float(fs),
                                         int(taps), w, C.c_void_p(stream)), "sarbp_stolt_sinc")
+
+    def stolt_gridsample(self, d_in: int, d_out: int, fr_axis, fa_axis, *, f0: float, c: float, vr: float,
+                         na: int, nr: int, fr: float, stream: int = 0) -> None:
+        """sarbp_stolt_gridsample (the reference's GridSample Stolt step) on
+        device buffers; the axes are host float64 arrays, fr the bin step."""
+        import numpy as np
+        fra = np.ascontiguousarray(fr_axis, dtype=np.float64)
+        faa = np.ascontiguousarray(fa_axis, dtype=np.float64)
+        if fra.size != nr or faa.size != na:
+            raise ValueError("axis lengths must be nr and na")
+        check(self.lib.sarbp_stolt_gridsample(self.handle, C.c_void_p(d_in), C.c_void_p(d_out),
+                                              fra.ctypes.data_as(C.c_void_p), faa.ctypes.data_as(C.c_void_p),
+                                              float(f0), float(c), float(vr), int(na), int(nr), float(fr),
+                                              C.c_void_p(stream)), "sarbp_stolt_gridsample")
 
     def omega_k(self, d_echo: int, na_in: int, nr_in: int, wp: WkParams, d_image: int,
                 stream: int = 0) -> None:

@@ -124,6 +124,58 @@ def test_stolt_centres_just_below_an_integer(ctx):
         assert np.max(np.abs(ref[1, :-2] - S[1, 1:-1])) < 1e-6 * np.max(np.abs(S[1]))
 
 
+def _gridsample(ctx, S, fr_axis, fa_axis, out=None, **kw):
+    d = _dev(S)
+    o = d if out == "inplace" else _torch().empty_like(d)
+    ctx.stolt_gridsample(d.data_ptr(), o.data_ptr(), fr_axis, fa_axis, na=S.shape[0], nr=S.shape[1], **kw)
+    _torch().cuda.synchronize()
+    return o.cpu().numpy()
+
+
+@pytest.mark.parametrize("nr", [300, 301])
+def test_stolt_gridsample_matches_oracle(ctx, nr):
+    # The reference's GridSample Stolt step (task/stolt.cc:38-318): rotated
+    # rows, FP32 grid, linear interpolation with zeros outside.
+    na = 37
+    S = _rand((na, nr), 31 + nr)
+    fs = 1e6
+    fr = W.fftshift_axis(nr) * fs / nr
+    fa = np.linspace(-11.3, 11.9, na)
+    kw = dict(f0=1e9, c=3e8, vr=150.0, fr=fs / nr)
+    got = _gridsample(ctx, S, fr, fa, **kw)
+    ref = W.stolt_gridsample(S, fr, fa, **kw)
+    err = np.max(np.abs(got - ref))
+    assert err <= 1e-6 * np.max(np.abs(S)), err
+    assert np.array_equal(_gridsample(ctx, S, fr, fa, out="inplace", **kw), got)
+
+
+def test_stolt_gridsample_radarsat_axes(ctx):
+    na, nr = 4096, 3414
+    rows = np.arange(0, na, 128)
+    fr, fa, _ = W.axes(na, nr, 32.317e6, 1256.98, -6900.0)
+    S = _rand((rows.size, nr), 8)
+    kw = dict(f0=5.3e9, c=299790000.0, vr=7062.0, fr=32.317e6 / nr)
+    got = _gridsample(ctx, S, fr, fa[rows], **kw)
+    ref = W.stolt_gridsample(S, fr, fa[rows], **kw)
+    assert np.max(np.abs(got - ref)) <= 1e-6 * np.max(np.abs(S))
+
+
+def test_stolt_gridsample_rejects_bad_arguments(ctx):
+    torch = _torch()
+    d = torch.zeros((2, 8), dtype=torch.complex64, device="cuda:0")
+    fr, fa = np.zeros(8), np.zeros(2)
+    for kw in (dict(fr=0.0), dict(c=0.0), dict(vr=0.0)):
+        args = dict(f0=1e9, c=3e8, vr=100.0, na=2, nr=8, fr=1e3)
+        args.update(kw)
+        with pytest.raises(native.SarbpError) as e:
+            ctx.stolt_gridsample(d.data_ptr(), d.data_ptr(), fr, fa, **args)
+        assert e.value.status == native.ERR_INVALID_ARGUMENT, kw
+    with pytest.raises(native.SarbpError) as e:
+        ctx.stolt_gridsample(d.data_ptr(), d.data_ptr(), np.zeros(9000), fa, f0=1e9, c=3e8, vr=100.0, na=2,
+                             nr=9000, fr=1e3)
+    assert e.value.status == native.ERR_UNSUPPORTED
+
+
 def test_stolt_rejects_bad_arguments(ctx):
     torch = _torch()
     d = torch.zeros((2, 8), dtype=torch.complex64, device="cuda:0")

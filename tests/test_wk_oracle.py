@@ -274,3 +274,51 @@ def test_centred_reference_planted_mistakes(monkeypatch):
         monkeypatch.setattr(W, "time_shift", shift0)
         ok, ms = _focused(img, tg)
         assert not ok, (name, ms)
+
+
+# ---- GridSample Stolt (performStoltInterpolation, task/stolt.cc:38-318) --------------
+
+def _grid_case(na=9, nr=40, seed=3):
+    rng = np.random.default_rng(seed)
+    S = rng.standard_normal((na, nr)) + 1j * rng.standard_normal((na, nr))
+    fs, f0, c, vr = 1e6, 1e9, 3e8, 150.0
+    fr_axis = W.fftshift_axis(nr) * fs / nr
+    fa_axis = np.linspace(-7.3, 9.1, na)
+    return S, fr_axis, fa_axis, dict(f0=f0, c=c, vr=vr, fr=fs / nr)
+
+
+def test_gridsample_stolt_matches_torch_grid_sample():
+    # The reference hands the rotated rows and the float32 grid to CNNL's
+    # bilinear, zero-padded, align-corners GridSample, real and imaginary parts
+    # separately (stolt.cc:73-78, :199-206, :246-272); torch.grid_sample with
+    # the same options and the same grid is an independent implementation.
+    import torch
+    S, fr_axis, fa_axis, kw = _grid_case()
+    na, nr = S.shape
+    k = kw["c"] ** 2 / (4 * kw["vr"] ** 2)
+    f = kw["f0"] + fr_axis[None, :]
+    fr_new = (np.sqrt(f * f - k * fa_axis[:, None] ** 2) - kw["f0"]).astype(np.float32)
+    grid = np.zeros((1, na, nr, 2), dtype=np.float32)
+    grid[0, :, :, 0] = (fr_new.astype(np.float64) / (kw["fr"] * nr * 0.5)).astype(np.float32)
+    grid[0, :, :, 1] = (2.0 * np.arange(na, dtype=np.float32) / np.float32(na - 1) - 1.0)[:, None]
+    shifted = np.roll(S, nr // 2, axis=1)
+    out = np.zeros_like(S)
+    for part, put in ((shifted.real, 1.0), (shifted.imag, 1j)):
+        t = torch.from_numpy(part.astype(np.float32))[None, None]
+        g = torch.nn.functional.grid_sample(t, torch.from_numpy(grid), mode="bilinear", padding_mode="zeros",
+                                            align_corners=True)
+        out = out + put * g[0, 0].numpy().astype(np.float64)
+    ref = W.stolt_gridsample(S, fr_axis, fa_axis, **kw)
+    assert np.max(np.abs(out - ref)) < 1e-5 * np.max(np.abs(S))
+    assert np.max(np.abs(ref)) > 0.1
+
+
+def test_gridsample_stolt_rotation_and_padding():
+    # Row rotation by Nr/2 (stolt.cc:121-130: source column (w + Nr - mid) mod
+    # Nr) and zeros outside the row: grid points beyond +-1 give 0.
+    S, fr_axis, fa_axis, kw = _grid_case(na=3, nr=11)
+    mid = 11 // 2
+    rot = np.array([[S[m, (w + 11 - mid) % 11] for w in range(11)] for m in range(3)])
+    assert np.array_equal(np.roll(S, mid, axis=1), rot)
+    far = W.stolt_gridsample(S, fr_axis, fa_axis, **{**kw, "fr": kw["fr"] * 1e-3})  # |x| >> 1
+    assert np.count_nonzero(far) <= 3 * 2   # at most the bins whose x rounds into [-1, 1]
