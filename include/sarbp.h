@@ -548,6 +548,36 @@ typedef struct sarbp_wk_params {
  * (code/SAR_WK/wk_v5.2.3/task/preprocess.cc:21-24).  Host-only. */
 sarbp_status sarbp_wk_padded_size(int32_t na_in, int32_t nr_in, int32_t* na, int32_t* nr);
 
+/* The steps of process_wk one by one, for callers that drive them as the
+ * reference's task API does (task/task.h:46-73); sarbp_omega_k runs them all
+ * in one pass.  Device buffers, asynchronous on `stream`.
+ *
+ * preprocess_wk (preprocess.cc:17-166): d_echo [na_in][nr_in] -> d_out
+ * [na][nr] (the padded size): zeros around the echo, line m times
+ * exp(-j 2 pi f_nc ta_m), ta_m = (m - na/2) / prf. */
+sarbp_status sarbp_wk_preprocess(sarbp_context* ctx, const sarbp_cf32* d_echo,
+                                 int32_t na_in, int32_t nr_in, double prf,
+                                 double f_nc, sarbp_cf32* d_out, void* stream);
+
+/* perform2DFFT / perform2DIFFT (fft.cc:18-359): the 2-D DFT of d_data
+ * [na][nr] in place, natural order; inverse != 0: the inverse transform
+ * scaled by 1/(na nr).  Plans are kept by the context; calls on one context
+ * are ordered on the device like sarbp_omega_k's. */
+sarbp_status sarbp_wk_fft2(sarbp_context* ctx, sarbp_cf32* d_data, int32_t na,
+                           int32_t nr, int32_t inverse, void* stream);
+
+/* generateReferenceFunction_CPU (compression.cc:17-58), on the device: d_theta
+ * [na][nr] = exp(j theta), theta = 4 pi r0/c sqrt(max(0, (f0 + fr)^2 -
+ * c^2 fa^2 / (4 vr^2))) - pi fr^2 / kr, with kr as that function takes it
+ * (process_wk passes -Kr of paras.txt, wk.cc:229).  Axes as for
+ * sarbp_stolt_sinc (host FP64, staged before the call returns). */
+sarbp_status sarbp_wk_reference_function(sarbp_context* ctx, sarbp_cf32* d_theta,
+                                         const double* h_fr_axis,
+                                         const double* h_fa_axis, double f0,
+                                         double c, double vr, double kr,
+                                         double r0, int32_t na, int32_t nr,
+                                         void* stream);
+
 /* omega-K image of raw echoes d_echo [na_in][nr_in] into d_image [na][nr]
  * (the padded size; the buffer is also the transform workspace).  The steps
  * of process_wk (wk.cc:87-285): zero padding and azimuth down-conversion by

@@ -532,6 +532,66 @@ __global__ __launch_bounds__(256) void wk_prep_t(PrepArgs a) {
   }
 }
 
+// Padding and azimuth down-conversion into [na][nr] (preprocess_wk,
+// preprocess.cc:17-166): zeros, or the echo sample times its line's factor
+// exp(-j 2 pi f_nc ta), ta = (m - na/2) / prf, as wk_prep_t computes it.
+__global__ __launch_bounds__(256) void wk_prep_rows(PrepArgs a) {
+  const long long total = (long long)a.na * a.nr;
+  const long long step = (long long)gridDim.x * blockDim.x;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += step) {
+    const int m = (int)(idx / a.nr), j = (int)(idx - (long long)m * a.nr);
+    const int mi = m - a.pad_az, ji = j - a.pad_rg;
+    float2 v = make_float2(0.0f, 0.0f);
+    if (mi >= 0 && mi < a.na_in && ji >= 0 && ji < a.nr_in) {
+      const double rev = a.rev_per_line * ((double)m - 0.5 * (double)a.na);
+      float sn, cs;
+      sincospif(2.0f * (float)(rev - rint(rev)), &sn, &cs);
+      const float2 e = a.echo[(size_t)mi * a.nr_in + ji];
+      v = make_float2(e.x * cs - e.y * sn, e.x * sn + e.y * cs);
+    }
+    a.out[idx] = v;
+  }
+}
+
+// exp(j theta) on [na][nr] (generateReferenceFunction_CPU, compression.cc:17-58):
+// theta = 4 pi r0/c sqrt(max(0, (f0 + fr)^2 - c^2 fa^2 / (4 vr^2))) - pi fr^2 / kr,
+// evaluated in revolutions in FP64 and reduced before the hardware sine and
+// cosine.
+struct RefArgs {
+  float2* out;
+  const double* fr_axis;
+  const double* fa_axis;
+  int na, nr;
+  double f0, bb, rev_s, rev_q;  // bb = c^2/(4 vr^2); rev_s = 2 r0 / c; rev_q = -1 / (2 kr)
+};
+
+__global__ __launch_bounds__(256) void wk_ref_table(RefArgs a) {
+  const long long total = (long long)a.na * a.nr;
+  const long long step = (long long)gridDim.x * blockDim.x;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += step) {
+    const int m = (int)(idx / a.nr), j = (int)(idx - (long long)m * a.nr);
+    const double fa = a.fa_axis[m], fr = a.fr_axis[j];
+    const double f = a.f0 + fr;
+    const double s = sqrt(fmax(f * f - a.bb * fa * fa, 0.0));
+    const double rev = a.rev_s * s + a.rev_q * fr * fr;
+    const float ph = (float)(rev - rint(rev));
+    a.out[idx] = make_float2(__builtin_amdgcn_cosf(ph), __builtin_amdgcn_sinf(ph));
+  }
+}
+
+__global__ __launch_bounds__(256) void wk_scale(float2* d, long long n, float sc) {
+  const long long step = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
+    const float2 v = d[i];
+    d[i] = make_float2(v.x * sc, v.y * sc);
+  }
+}
+
+unsigned stride_grid(long long n) {
+  const long long b = (n + 255) / 256;
+  return (unsigned)(b < 65536 ? (b > 0 ? b : 1) : 65536);
+}
+
 // Padding, azimuth down-conversion and the azimuth FFT of one echo range
 // column c, for a power-of-two padded azimuth length na = 2^LG: the column is
 // read down the echo (the columns in flight on one XCD share the cache lines
@@ -758,6 +818,7 @@ struct WkCache {
   std::map<std::pair<int, int>, hipfftHandle> row_plans;  // 1-D (length, batch)
   std::map<int, float2*> chirp_tables;                    // nr -> [tw | bh | chirp]
   std::map<int, float2*> az_twiddles;                     // na -> exp(-j 2 pi t / na)
+  std::map<std::pair<int, int>, hipfftHandle> plans_2d;   // (na, nr) of sarbp_wk_fft2
   float2* work = nullptr;  // [nr][na] workspace of sarbp_omega_k
   size_t work_cap = 0;     // complex64 words
   double* d_axes = nullptr;  // [nr | na] of sarbp_stolt_sinc
@@ -777,6 +838,7 @@ void wk_cache_destroy(WkCache* wc) {
   if (wc->done) (void)hipEventSynchronize(wc->done);
   (void)hipDeviceSynchronize();  // the work buffer and tables may still be in use
   for (auto& kv : wc->row_plans) (void)hipfftDestroy(kv.second);
+  for (auto& kv : wc->plans_2d) (void)hipfftDestroy(kv.second);
   for (auto& kv : wc->chirp_tables) (void)hipFree(kv.second);
   for (auto& kv : wc->az_twiddles) (void)hipFree(kv.second);
   (void)hipFree(wc->work);
@@ -1035,6 +1097,115 @@ sarbp_status sarbp_stolt_gridsample(sarbp_context* ctx, const sarbp_cf32* d_in, 
   a.half_band = fr * (double)nr * 0.5;
   hipError_t e;
   if ((e = launch_stolt_grid(a, s)) != hipSuccess) return hip_fail(e, "wk_stolt_grid launch");
+  if ((e = hipEventRecord(wc->done, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+  return SARBP_OK;
+}
+
+sarbp_status sarbp_wk_preprocess(sarbp_context* ctx, const sarbp_cf32* d_echo, int32_t na_in, int32_t nr_in,
+                                double prf, double f_nc, sarbp_cf32* d_out, void* stream) {
+  if (!ctx) return fail(SARBP_ERR_INVALID_ARGUMENT, "context is NULL");
+  int32_t na = 0, nr = 0;
+  sarbp_status st = sarbp_wk_padded_size(na_in, nr_in, &na, &nr);
+  if (st != SARBP_OK) return st;
+  if (!(std::isfinite(prf) && prf > 0.0 && std::isfinite(f_nc)))
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "prf or f_nc invalid");
+  if (na == 0 || nr == 0) return SARBP_OK;
+  if (!d_out || (na_in > 0 && nr_in > 0 && !d_echo))
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "echo or output pointer is NULL");
+  DeviceGuard dg(ctx->device);
+  if (dg.err != hipSuccess) return hip_fail(dg.err, "hipSetDevice");
+  PrepArgs pa{};
+  pa.echo = reinterpret_cast<const float2*>(d_echo);
+  pa.out = reinterpret_cast<float2*>(d_out);
+  pa.na = na;
+  pa.nr = nr;
+  pa.na_in = na_in;
+  pa.nr_in = nr_in;
+  pa.pad_az = (na - na_in) / 2;
+  pa.pad_rg = (nr - nr_in) / 2;
+  pa.rev_per_line = -f_nc / prf;
+  hipLaunchKernelGGL(wk_prep_rows, dim3(stride_grid((long long)na * nr)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), pa);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? SARBP_OK : hip_fail(e, "wk_prep_rows launch");
+}
+
+sarbp_status sarbp_wk_fft2(sarbp_context* ctx, sarbp_cf32* d_data, int32_t na, int32_t nr, int32_t inverse,
+                           void* stream) {
+  if (!ctx) return fail(SARBP_ERR_INVALID_ARGUMENT, "context is NULL");
+  if (na < 0 || nr < 0) return fail(SARBP_ERR_INVALID_ARGUMENT, "negative size %d x %d", na, nr);
+  if (na == 0 || nr == 0) return SARBP_OK;
+  if (!d_data) return fail(SARBP_ERR_INVALID_ARGUMENT, "data pointer is NULL");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  DeviceGuard dg(ctx->device);
+  if (dg.err != hipSuccess) return hip_fail(dg.err, "hipSetDevice");
+  WkCache* wc = nullptr;
+  sarbp_status st = wk_cache(ctx, &wc);
+  if (st != SARBP_OK) return st;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  hipError_t e;
+  // The plan's work area is shared by calls on this context: order them.
+  if (!wc->wk_done && (e = hipEventCreateWithFlags(&wc->wk_done, hipEventDisableTiming)) != hipSuccess)
+    return hip_fail(e, "hipEventCreate");
+  if (wc->wk_pending && (e = hipStreamWaitEvent(s, wc->wk_done, 0)) != hipSuccess)
+    return hip_fail(e, "hipStreamWaitEvent");
+  const auto key = std::make_pair((int)na, (int)nr);
+  hipfftHandle h = nullptr;
+  auto it = wc->plans_2d.find(key);
+  if (it != wc->plans_2d.end()) {
+    h = it->second;
+  } else {
+    const hipfftResult r = hipfftPlan2d(&h, na, nr, HIPFFT_C2C);
+    if (r != HIPFFT_SUCCESS)
+      return fail(r == HIPFFT_ALLOC_FAILED ? SARBP_ERR_OUT_OF_MEMORY : SARBP_ERR_DEVICE,
+                  "hipfftPlan2d(%d x %d) failed (%d)", na, nr, (int)r);
+    wc->plans_2d[key] = h;
+  }
+  if ((st = exec_fft(h, reinterpret_cast<float2*>(d_data), inverse ? HIPFFT_BACKWARD : HIPFFT_FORWARD, s)) !=
+      SARBP_OK)
+    return st;
+  if (inverse) {
+    const long long n = (long long)na * nr;
+    hipLaunchKernelGGL(wk_scale, dim3(stride_grid(n)), dim3(256), 0, s, reinterpret_cast<float2*>(d_data), n,
+                       (float)(1.0 / ((double)na * (double)nr)));
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "wk_scale launch");
+  }
+  if ((e = hipEventRecord(wc->wk_done, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+  wc->wk_pending = true;
+  return SARBP_OK;
+}
+
+sarbp_status sarbp_wk_reference_function(sarbp_context* ctx, sarbp_cf32* d_theta, const double* h_fr_axis,
+                                         const double* h_fa_axis, double f0, double c, double vr, double kr,
+                                         double r0, int32_t na, int32_t nr, void* stream) {
+  if (!ctx) return fail(SARBP_ERR_INVALID_ARGUMENT, "context is NULL");
+  if (na < 0 || nr < 0) return fail(SARBP_ERR_INVALID_ARGUMENT, "negative size %d x %d", na, nr);
+  if (!(std::isfinite(f0) && std::isfinite(c) && c > 0.0 && std::isfinite(vr) && vr != 0.0 && std::isfinite(kr) &&
+        kr != 0.0 && std::isfinite(r0)))
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "f0, c, vr, kr or r0 invalid");
+  if (na == 0 || nr == 0) return SARBP_OK;
+  if (!d_theta || !h_fr_axis || !h_fa_axis) return fail(SARBP_ERR_INVALID_ARGUMENT, "pointer is NULL");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  DeviceGuard dg(ctx->device);
+  if (dg.err != hipSuccess) return hip_fail(dg.err, "hipSetDevice");
+  WkCache* wc = nullptr;
+  sarbp_status st = wk_cache(ctx, &wc);
+  if (st != SARBP_OK) return st;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  if ((st = stage_axes(wc, h_fr_axis, h_fa_axis, na, nr, s)) != SARBP_OK) return st;
+  RefArgs a{};
+  a.out = reinterpret_cast<float2*>(d_theta);
+  a.fr_axis = wc->d_axes;
+  a.fa_axis = wc->d_axes + nr;
+  a.na = na;
+  a.nr = nr;
+  a.f0 = f0;
+  a.bb = c * c / (4.0 * vr * vr);
+  a.rev_s = 2.0 * r0 / c;
+  a.rev_q = -0.5 / kr;
+  hipLaunchKernelGGL(wk_ref_table, dim3(stride_grid((long long)na * nr)), dim3(256), 0, s, a);
+  hipError_t e;
+  if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "wk_ref_table launch");
   if ((e = hipEventRecord(wc->done, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
   return SARBP_OK;
 }

@@ -204,6 +204,12 @@ _SIGS = {
     "sarbp_stolt_gridsample": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32,
                                          C.c_double, C.c_void_p]),
+    "sarbp_wk_preprocess": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                      C.c_void_p, C.c_void_p]),
+    "sarbp_wk_fft2": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]),
+    "sarbp_wk_reference_function": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                              C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32,
+                                              C.c_int32, C.c_void_p]),
     "sarbp_wk_padded_size": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "sarbp_omega_k": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.POINTER(WkParams),
                                 C.c_void_p, C.c_void_p]),
@@ -376,6 +382,28 @@ class Context:
                                               fra.ctypes.data_as(C.c_void_p), faa.ctypes.data_as(C.c_void_p),
                                               float(f0), float(c), float(vr), int(na), int(nr), float(fr),
                                               C.c_void_p(stream)), "sarbp_stolt_gridsample")
+
+    def wk_preprocess(self, d_echo: int, na_in: int, nr_in: int, prf: float, f_nc: float, d_out: int,
+                      stream: int = 0) -> None:
+        check(self.lib.sarbp_wk_preprocess(self.handle, C.c_void_p(d_echo), int(na_in), int(nr_in), float(prf),
+                                           float(f_nc), C.c_void_p(d_out), C.c_void_p(stream)),
+              "sarbp_wk_preprocess")
+
+    def wk_fft2(self, d_data: int, na: int, nr: int, inverse: bool = False, stream: int = 0) -> None:
+        check(self.lib.sarbp_wk_fft2(self.handle, C.c_void_p(d_data), int(na), int(nr), int(bool(inverse)),
+                                     C.c_void_p(stream)), "sarbp_wk_fft2")
+
+    def wk_reference_function(self, d_theta: int, fr_axis, fa_axis, *, f0: float, c: float, vr: float,
+                              kr: float, r0: float, na: int, nr: int, stream: int = 0) -> None:
+        import numpy as np
+        fra = np.ascontiguousarray(fr_axis, dtype=np.float64)
+        faa = np.ascontiguousarray(fa_axis, dtype=np.float64)
+        if fra.size != nr or faa.size != na:
+            raise ValueError("axis lengths must be nr and na")
+        check(self.lib.sarbp_wk_reference_function(self.handle, C.c_void_p(d_theta),
+                                                   fra.ctypes.data_as(C.c_void_p), faa.ctypes.data_as(C.c_void_p),
+                                                   float(f0), float(c), float(vr), float(kr), float(r0), int(na),
+                                                   int(nr), C.c_void_p(stream)), "sarbp_wk_reference_function")
 
     def omega_k(self, d_echo: int, na_in: int, nr_in: int, wp: WkParams, d_image: int,
                 stream: int = 0) -> None:

@@ -399,3 +399,79 @@ def test_omega_k_centred_random_echo(ctx, na_in, nr_in):
     ref = _oracle(echo, wp)
     assert got.shape == W.padded_size(na_in, nr_in)[:2]
     assert _rel_l2(got, ref) < 2e-5
+
+
+# ---- the steps of process_wk one by one (task/task.h:46-73) --------------------------
+
+def _steps_case():
+    scene = O.WkScene(na_in=45, nr_in=101, f_nc=60.0)
+    echo = _rand((45, 101), 41)
+    return scene.resolved(), echo
+
+
+def test_wk_steps_match_oracle(ctx):
+    torch = _torch()
+    sc, echo = _steps_case()
+    na, nr = sc.padded
+    d_echo = _dev(echo)
+    pad = torch.empty((na, nr), dtype=torch.complex64, device="cuda:0")
+    ctx.wk_preprocess(d_echo.data_ptr(), 45, 101, sc.prf, sc.f_nc, pad.data_ptr())
+    torch.cuda.synchronize()
+    ref_pad = W.pad_downconvert(echo.astype(np.complex128), prf=sc.prf, f_nc=sc.f_nc)
+    assert _rel_l2(pad.cpu().numpy(), ref_pad) < 1e-6
+    ctx.wk_fft2(pad.data_ptr(), na, nr)
+    torch.cuda.synchronize()
+    spec = np.fft.fft2(ref_pad)
+    assert _rel_l2(pad.cpu().numpy(), spec) < 2e-6
+    ctx.wk_fft2(pad.data_ptr(), na, nr, inverse=True)
+    torch.cuda.synchronize()
+    assert _rel_l2(pad.cpu().numpy(), ref_pad) < 2e-6
+    fr, fa, _ = W.axes(na, nr, sc.fs, sc.prf, sc.f_nc)
+    theta = torch.empty((na, nr), dtype=torch.complex64, device="cuda:0")
+    kr_arg = -sc.kr  # process_wk passes -Kr (wk.cc:229)
+    ctx.wk_reference_function(theta.data_ptr(), fr, fa, f0=sc.f0, c=sc.c, vr=sc.vr, kr=kr_arg, r0=sc.r0,
+                              na=na, nr=nr)
+    torch.cuda.synchronize()
+    want = np.exp(1j * W.reference_phase(fr, fa, f0=sc.f0, c=sc.c, vr=sc.vr, kr=sc.kr, r0=sc.r0))
+    assert np.max(np.abs(theta.cpu().numpy() - want)) < 1e-6
+
+
+def test_wk_steps_compose_to_omega_k(ctx):
+    # preprocess -> 2-D FFT -> times the reference function -> sinc Stolt ->
+    # 2-D inverse FFT, as process_wk calls them (wk.cc:87-285), gives the
+    # image of sarbp_omega_k.
+    torch = _torch()
+    from sar_cambrianaccelerationcard_amd import device
+    sc, echo = _steps_case()
+    na, nr = sc.padded
+    wp = sc.params()
+    d_echo = _dev(echo)
+    x = torch.empty((na, nr), dtype=torch.complex64, device="cuda:0")
+    ctx.wk_preprocess(d_echo.data_ptr(), 45, 101, sc.prf, sc.f_nc, x.data_ptr())
+    ctx.wk_fft2(x.data_ptr(), na, nr)
+    fr, fa, _ = W.axes(na, nr, sc.fs, sc.prf, sc.f_nc)
+    theta = torch.empty_like(x)
+    ctx.wk_reference_function(theta.data_ptr(), fr, fa, f0=sc.f0, c=sc.c, vr=sc.vr, kr=-sc.kr, r0=sc.r0,
+                              na=na, nr=nr)
+    torch.cuda.synchronize()
+    x *= theta
+    torch.cuda.synchronize()
+    ctx.stolt_sinc(x.data_ptr(), x.data_ptr(), fr, fa, f0=sc.f0, c=sc.c, vr=sc.vr, na=na, nr=nr, fs=sc.fs)
+    ctx.wk_fft2(x.data_ptr(), na, nr, inverse=True)
+    torch.cuda.synchronize()
+    img = device.omega_k(ctx, echo, wp).cpu().numpy()
+    assert _rel_l2(x.cpu().numpy(), img) < 2e-5
+    assert _rel_l2(x.cpu().numpy(), _oracle(echo, wp)) < 2e-5
+
+
+def test_wk_steps_reject_bad_arguments(ctx):
+    torch = _torch()
+    d = torch.zeros((8, 8), dtype=torch.complex64, device="cuda:0")
+    for call in (lambda: ctx.wk_preprocess(d.data_ptr(), 4, 4, 0.0, 0.0, d.data_ptr()),
+                 lambda: ctx.wk_fft2(0, 8, 8),
+                 lambda: ctx.wk_fft2(d.data_ptr(), -1, 8),
+                 lambda: ctx.wk_reference_function(d.data_ptr(), np.zeros(8), np.zeros(8), f0=1e9, c=3e8,
+                                                   vr=100.0, kr=0.0, r0=1e3, na=8, nr=8)):
+        with pytest.raises(native.SarbpError) as e:
+            call()
+        assert e.value.status == native.ERR_INVALID_ARGUMENT
