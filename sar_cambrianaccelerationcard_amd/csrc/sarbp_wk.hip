@@ -9,16 +9,28 @@
 // azimuth-frequency line (wk_rows): the line is read from HBM once, its
 // length-nr range DFT and inverse DFT run as Bluestein chirp-z transforms on
 // 2^LG-point FFTs in LDS (nr need not be a power of two: the reference's
-// scene pads to 3414 = 2 * 3 * 569), and it is written once.  The azimuth
-// transforms are power-of-two hipFFT row transforms of an [nr][na] workspace;
-// the line kernel reads and writes its line as a column of that workspace in
-// place (lines of neighbouring workgroups share the cache lines in the XCD's
-// L2), and one tiled transpose writes the [na][nr] image.
+// scene pads to 3414 = 2 * 3 * 569), and it is written once.  The line
+// kernel reads its line as a column of an [nr][na] workspace (lines of
+// neighbouring workgroups share the cache lines in the XCD's L2).  For
+// power-of-two na the azimuth transforms are LDS kernels: the forward one
+// writes workspace rows, the line kernel writes image rows and the inverse
+// transforms image columns in place; for other na they are hipFFT row
+// transforms of the workspace, the line kernel writes workspace columns and
+// one tiled transpose writes the [na][nr] image.
 // Kernels written here:
-//   wk_prep_t    zeros, or the echo sample times its line's down-conversion
-//                factor (FP64 phase), written transposed ([nr][na]) for the
-//                azimuth transforms, in 64 x 64 tiles through LDS.
-//   wk_transpose 64 x 64 tiles through LDS.
+//   wk_az_fwd    (power-of-two na) per echo range column: padding, azimuth
+//                down-conversion (FP64 phase) and the azimuth FFT in LDS,
+//                written as a row of the [nr][na] workspace.
+//   wk_az_inv_col / wk_az_inv_col2  (power-of-two na) the azimuth inverse
+//                FFT in LDS on one or two image columns, in place.
+//   wk_prep_t    (other na) zeros, or the echo sample times its line's
+//                down-conversion factor, written transposed ([nr][na]) for
+//                hipFFT row transforms, in 64 x 64 tiles through LDS.
+//   wk_transpose (other na) 64 x 64 tiles through LDS.
+//   wk_prep_rows, wk_ref_table, wk_scale  the task-level steps
+//                (sarbp_wk_preprocess, sarbp_wk_reference_function, the
+//                scale of sarbp_wk_fft2's inverse).
+//   wk_stolt_grid  the GridSample Stolt step (sarbp_stolt_gridsample).
 //   wk_rows      per line: stage x[n] conj(b[n]) (b[n] = exp(j pi n^2 / nr)),
 //                FFT, times the chirp filter spectrum, inverse FFT, times
 //                conj(b[k]): the range spectrum X[k]; times the reference
