@@ -265,6 +265,28 @@ def test_omega_k_full_scene_size(ctx):
         assert (mg["d_line"], mg["d_sample"]) == (0, 0) and mg["concentration"] > 0.4, mg
 
 
+def test_omega_k_full_scene_size_centred_reference(ctx):
+    # The reference's scene size and radar, targets in both halves of the
+    # padded range window, padded sample 0 at no whole number of range
+    # windows: the window-centred reference focuses all of them.
+    from dataclasses import replace
+    scene = O.radarsat_scene()
+    scene = replace(scene, r0=scene.r0 + 1234.5)
+    _, nr = scene.padded
+    # whole chirps (1349 samples) inside the echo's range samples [683, 2731),
+    # two targets on each side of nr/2 = 1707
+    targets = [O.aperture_target(scene, a, s) for a, s in ((1500, 1400), (2000, 1550), (2600, 1900),
+                                                           (1800, 2030))]
+    echo = O.simulate(scene, targets)
+    wp = scene.params(range_ref="centre")
+    got = _omega_k(ctx, echo, wp)
+    ref = _oracle(echo, wp)
+    assert _rel_l2(got, ref) < 2e-5
+    for t in targets:
+        mg = O.peak_metrics(got, *t)
+        assert (mg["d_line"], mg["d_sample"]) == (0, 0) and mg["concentration"] > 0.4, (t, mg)
+
+
 def test_omega_k_rejects_bad_arguments(ctx):
     torch = _torch()
     d = torch.zeros((8, 8), dtype=torch.complex64, device="cuda:0")
