@@ -700,7 +700,8 @@ __global__ __launch_bounds__((1 << LG) / 16) void wk_az_inv_col2(float2* img, co
 
 template <int LG>
 hipError_t launch_az_inv_lg(float2* img, const float2* tw, int nr, hipStream_t s) {
-  if ((nr & 1) == 0) {
+  // 16-byte accesses need an even row length and a 16-byte aligned image.
+  if ((nr & 1) == 0 && (reinterpret_cast<uintptr_t>(img) & 15) == 0) {
     const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&wk_az_inv_col2<LG>),
                                              hipFuncAttributeMaxDynamicSharedMemorySize,
                                              (int)(2 * fft::lds_words<LG>() * sizeof(float2)));

@@ -340,6 +340,22 @@ def test_omega_k_long_lines(ctx):
     assert _rel_l2(got, ref) < 2e-5
 
 
+def test_omega_k_image_at_an_odd_offset(ctx):
+    # An image that starts 8 bytes into its buffer (not 16-byte aligned) on the
+    # power-of-two azimuth path: the inverse takes one column per workgroup.
+    torch = _torch()
+    echo = _rand((768, 20), 77)
+    wp = O.WkScene(na_in=768, nr_in=20).params()
+    na, nr = W.padded_size(768, 20)[:2]
+    buf = torch.zeros(na * nr + 1, dtype=torch.complex64, device="cuda:0")
+    d_echo = _dev(echo)
+    ctx.omega_k(d_echo.data_ptr(), 768, 20, wp, buf.data_ptr() + 8)
+    torch.cuda.synchronize()
+    got = buf[1:].reshape(na, nr).cpu().numpy()
+    assert _rel_l2(got, _oracle(echo, wp)) < 2e-5
+    assert buf[0].item() == 0
+
+
 def test_omega_k_calls_on_two_streams(ctx):
     # Calls on one context share its workspace; issued back to back on two
     # streams they are ordered on the device and both images are right.
