@@ -589,7 +589,8 @@ sarbp_status sarbp_wk_reference_function(sarbp_context* ctx, sarbp_cf32* d_theta
  * step and range inverse DFT run as one kernel per azimuth line (chirp-z
  * transforms in LDS; lines over 4096 samples need 128 KiB of LDS per
  * workgroup, else SARBP_ERR_UNSUPPORTED) between azimuth transforms (LDS
- * kernels when na is a power of two from 1024 to 8192, else hipFFT).  Plans,
+ * FFTs when na is a power of two from 1024 to 8192, LDS chirp-z transforms for
+ * other na up to 4096, hipFFT beyond).  Plans,
  * twiddle and chirp tables
  * and a workspace of na x nr complex64 words are kept by the
  * context (the first call of a size builds them).  Calls on one context share

@@ -23,6 +23,8 @@
 //                written as a row of the [nr][na] workspace.
 //   wk_az_inv_col / wk_az_inv_col2  (power-of-two na) the azimuth inverse
 //                FFT in LDS on one or two image columns, in place.
+//   wk_az_fwd_bs / wk_az_inv_bs  (other na up to 4096) the same with the
+//                azimuth DFT as a chirp-z transform in LDS.
 //   wk_prep_t    (other na) zeros, or the echo sample times its line's
 //                down-conversion factor, written transposed ([nr][na]) for
 //                hipFFT row transforms, in 64 x 64 tiles through LDS.
@@ -725,6 +727,99 @@ hipError_t launch_az_inv(float2* img, const float2* tw, int nr, int lg, hipStrea
   }
 }
 
+// The azimuth transforms for na that is not a power of two (na <= 4096): the
+// length-na DFT of a column as a chirp-z transform on 2^LG >= 2 na - 1
+// points in LDS (t: the chirp tables of length na, as for the range lines).
+// Forward: pad, down-convert and transform echo column c, written as row
+// pad_rg + c of the [nr][na] workspace.
+template <int LG>
+__global__ __launch_bounds__(rows_threads(LG)) void wk_az_fwd_bs(AzArgs a, RowArgs t) {
+  constexpr int NF = 1 << LG, T = rows_threads(LG), E = NF / T;
+  extern __shared__ float2 buf[];
+  const int c = xcd_line(blockIdx.x, a.nr_in);
+  const int tid = threadIdx.x;
+  const int na = a.na;
+#pragma unroll 4
+  for (int e = 0; e < E; ++e) {
+    const int m = tid + e * T;
+    float2 v = make_float2(0.0f, 0.0f);
+    if (m < na) {
+      const int mi = m - a.pad_az;
+      if (mi >= 0 && mi < a.na_in) {
+        const double rev = a.rev_per_line * ((double)m - 0.5 * (double)na);
+        float sn, cs;
+        sincospif(2.0f * (float)(rev - rint(rev)), &sn, &cs);
+        const float2 x = a.echo[(size_t)mi * a.nr_in + c];
+        v = make_float2(x.x * cs - x.y * sn, x.x * sn + x.y * cs);
+      }
+      v = cmul(v, conjf2(t.chirp[m]));
+    }
+    buf[fft::slot<LG>(m)] = v;
+  }
+  __syncthreads();
+  chirp_convolve<LG>(buf, t);
+  float2* row = a.out + (size_t)(a.pad_rg + c) * na;
+#pragma unroll 4
+  for (int e = 0; e < E; ++e) {
+    const int k = tid + e * T;
+    if (k < na) row[k] = cmul(buf[fft::slot<LG>(k)], conjf2(t.chirp[k]));
+  }
+}
+
+// Inverse (unscaled) of image column r in place: conj(DFT(conj(x))).
+template <int LG>
+__global__ __launch_bounds__(rows_threads(LG)) void wk_az_inv_bs(float2* img, RowArgs t, int na, int nr) {
+  constexpr int NF = 1 << LG, T = rows_threads(LG), E = NF / T;
+  extern __shared__ float2 buf[];
+  const int r = xcd_line(blockIdx.x, nr);
+  const int tid = threadIdx.x;
+#pragma unroll 4
+  for (int e = 0; e < E; ++e) {
+    const int m = tid + e * T;
+    buf[fft::slot<LG>(m)] =
+        m < na ? cmul(conjf2(img[(size_t)m * nr + r]), conjf2(t.chirp[m])) : make_float2(0.0f, 0.0f);
+  }
+  __syncthreads();
+  chirp_convolve<LG>(buf, t);
+#pragma unroll 4
+  for (int e = 0; e < E; ++e) {
+    const int k = tid + e * T;
+    if (k < na) img[(size_t)k * nr + r] = conjf2(cmul(buf[fft::slot<LG>(k)], conjf2(t.chirp[k])));
+  }
+}
+
+template <int LG>
+hipError_t launch_az_bs_lg(const AzArgs& a, const RowArgs& t, hipStream_t s) {
+  hipLaunchKernelGGL(wk_az_fwd_bs<LG>, dim3((unsigned)a.nr_in), dim3(rows_threads(LG)),
+                     (size_t)fft::lds_words<LG>() * sizeof(float2), s, a, t);
+  return hipGetLastError();
+}
+
+template <int LG>
+hipError_t launch_az_inv_bs_lg(float2* img, const RowArgs& t, int na, int nr, hipStream_t s) {
+  hipLaunchKernelGGL(wk_az_inv_bs<LG>, dim3((unsigned)nr), dim3(rows_threads(LG)),
+                     (size_t)fft::lds_words<LG>() * sizeof(float2), s, img, t, na, nr);
+  return hipGetLastError();
+}
+
+hipError_t launch_az_bs(const AzArgs& a, const RowArgs& t, int lg, hipStream_t s) {
+  switch (lg) {
+    case 11: return launch_az_bs_lg<11>(a, t, s);
+    case 12: return launch_az_bs_lg<12>(a, t, s);
+    case 13: return launch_az_bs_lg<13>(a, t, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_az_inv_bs(float2* img, const RowArgs& t, int na, int nr, int lg, hipStream_t s) {
+  switch (lg) {
+    case 11: return launch_az_inv_bs_lg<11>(img, t, na, nr, s);
+    case 12: return launch_az_inv_bs_lg<12>(img, t, na, nr, s);
+    case 13: return launch_az_inv_bs_lg<13>(img, t, na, nr, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 // log2 na when na is a power of two the fused azimuth kernel takes, else 0.
 int az_lg(int na) {
   for (int lg = 10; lg <= 13; ++lg)
@@ -1329,18 +1424,31 @@ sarbp_status omega_k_enqueue(WkCache* wc, const sarbp_cf32* d_echo, int na_in, i
   // range inverse DFT, azimuth inverse FFT, transpose into the image.
   const float2* tab = nullptr;
   hipfftHandle az_fwd = nullptr, az_inv = nullptr;
+  // Azimuth transforms: LDS FFTs for power-of-two na (1024..8192), LDS
+  // chirp-z transforms for other na up to 4096, hipFFT beyond.
   const int alg = az_lg(na);
+  const int blg = !alg && na <= 4096 ? rows_lg(na) : 0;
+  const bool fast = alg || blg;
   const float2* az_tw = nullptr;
+  const float2* az_tab = nullptr;
   if ((st = chirp_tables(wc, nr, lg, &tab)) != SARBP_OK) return st;
   if (alg) {
     if ((st = az_twiddles(wc, na, &az_tw)) != SARBP_OK) return st;
+  } else if (blg) {
+    if ((st = chirp_tables(wc, na, blg, &az_tab)) != SARBP_OK) return st;
   } else if ((st = row_plan(wc, na, nr_in, &az_fwd)) != SARBP_OK) {
     return st;
   }
-  if (!alg && (st = row_plan(wc, na, nr, &az_inv)) != SARBP_OK) return st;
+  if (!fast && (st = row_plan(wc, na, nr, &az_inv)) != SARBP_OK) return st;
+  RowArgs azt{};
+  if (blg) {
+    azt.tw = az_tab;
+    azt.bh = az_tab + ((size_t)1 << blg);
+    azt.chirp = az_tab + ((size_t)2 << blg);
+  }
   if ((st = work_buffer(wc, (size_t)na * nr)) != SARBP_OK) return st;
   float2* w = wc->work;
-  if (alg) {
+  if (fast) {
     AzArgs az{};
     az.echo = pa.echo;
     az.out = w;
@@ -1351,7 +1459,8 @@ sarbp_status omega_k_enqueue(WkCache* wc, const sarbp_cf32* d_echo, int na_in, i
     az.pad_az = pad_az;
     az.pad_rg = pad_rg;
     az.rev_per_line = pa.rev_per_line;
-    if ((e = launch_az(az, alg, s)) != hipSuccess) return hip_fail(e, "wk_az_fwd launch");
+    e = alg ? launch_az(az, alg, s) : launch_az_bs(az, azt, blg, s);
+    if (e != hipSuccess) return hip_fail(e, "azimuth forward launch");
   } else {
     pa.out = w;
     hipLaunchKernelGGL(wk_prep_t, dim3((unsigned)((nr_in + 63) / 64), (unsigned)((na + 63) / 64)), dim3(256), 0,
@@ -1369,7 +1478,7 @@ sarbp_status omega_k_enqueue(WkCache* wc, const sarbp_cf32* d_echo, int na_in, i
   r.in_sample = na;
   r.out_line = 1;
   r.out_sample = na;
-  if (alg) {
+  if (fast) {
     // lines written as rows of the image; the azimuth inverse runs on its columns
     r.st.out = img;
     r.out_line = nr;
@@ -1381,8 +1490,9 @@ sarbp_status omega_k_enqueue(WkCache* wc, const sarbp_cf32* d_echo, int na_in, i
   r.out_shift = out_shift;
   r.out_rot = make_float2((float)(std::cos(out_phase) * a.scale), (float)(std::sin(out_phase) * a.scale));
   if ((e = launch_rows(r, lg, s)) != hipSuccess) return hip_fail(e, "wk_rows launch");
-  if (alg) {
-    if ((e = launch_az_inv(img, az_tw, nr, alg, s)) != hipSuccess) return hip_fail(e, "wk_az_inv_col launch");
+  if (fast) {
+    e = alg ? launch_az_inv(img, az_tw, nr, alg, s) : launch_az_inv_bs(img, azt, na, nr, blg, s);
+    if (e != hipSuccess) return hip_fail(e, "azimuth inverse launch");
     return SARBP_OK;
   }
   if ((st = exec_fft(az_inv, w, HIPFFT_BACKWARD, s)) != SARBP_OK) return st;

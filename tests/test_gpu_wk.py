@@ -393,10 +393,12 @@ def test_omega_k_edge_sizes(ctx, na_in, nr_in):
     assert _rel_l2(got, ref) < 2e-5
 
 
-@pytest.mark.parametrize("na_in,nr_in", [(768, 20), (1536, 9), (6144, 6)])
+@pytest.mark.parametrize("na_in,nr_in", [(768, 20), (1536, 9), (6144, 6), (3100, 6)])
 def test_omega_k_power_of_two_azimuth(ctx, na_in, nr_in):
     # Padded azimuth lengths 1024, 2048 and 8192 take the fused pad /
-    # down-conversion / LDS azimuth FFT kernel (4096: the full-size test).
+    # down-conversion / LDS azimuth FFT kernel (4096: the full-size test);
+    # 4134 (not a power of two, above the chirp-z kernels' 4096) the hipFFT
+    # path; the other tests' lengths the chirp-z azimuth kernels.
     echo = _rand((na_in, nr_in), na_in + nr_in)
     wp = O.WkScene(na_in=na_in, nr_in=nr_in, f_nc=60.0).params()
     got = _omega_k(ctx, echo, wp)
