@@ -1,8 +1,9 @@
 // sarbp_fft.h -- in-LDS power-of-two FFTs for one workgroup: Stockham passes
 // of radix 8 (n/8 threads) or 16 (n/16 threads), after one smaller-radix pass
 // when log2 n is not a multiple of 3 or 4; used by range compression
-// (sarbp_range.hip) and by the omega-K line transforms (sarbp_wk.hip).  Not
-// part of the public ABI.
+// (sarbp_range.hip) and by the omega-K transforms (sarbp_wk.hip).  fft_regs
+// keeps the points in registers (16 per thread) and sends only the inner
+// passes through LDS.  Not part of the public ABI.
 //
 // Element i of the transform lives at LDS slot fft::slot<LG>(i): one pad word
 // per 16 for n <= 4096 (buffer fft::lds_words<LG>() complex64 words), an XOR
@@ -108,19 +109,54 @@ __device__ __forceinline__ void dft_r(float2* x) {
   }
 }
 
+// Twiddles and R-point DFT of butterfly b of a Stockham pass over NF = 2^LG
+// points after sub-transforms of size ns = 2^lns: input r is multiplied by
+// W^{r k}, k = b mod ns, of size R ns (tw[t] = exp(-j 2 pi t / NF); powers by
+// repeated products).
+template <int LG, int R, bool INV>
+__device__ __forceinline__ void twiddle_dft(float2* y, const float2* __restrict__ tw, int b, int lns) {
+  constexpr int LGR = R == 16 ? 4 : (R == 8 ? 3 : (R == 4 ? 2 : 1));
+  if (lns > 0) {
+    const int k = b & ((1 << lns) - 1);
+    float2 w[R];
+    w[1] = tw[k << (LG - lns - LGR)];
+    if (INV) w[1].y = -w[1].y;
+    // Powers by few products: w2 = w1^2, w3 = w2 w1, w4 = w2^2,
+    // w5..w7 = w4 w1..w3.
+    if constexpr (R >= 4) {
+      w[2] = cmul(w[1], w[1]);
+      w[3] = cmul(w[2], w[1]);
+    }
+    if constexpr (R >= 8) {
+      w[4] = cmul(w[2], w[2]);
+      w[5] = cmul(w[4], w[1]);
+      w[6] = cmul(w[4], w[2]);
+      w[7] = cmul(w[4], w[3]);
+    }
+    if constexpr (R == 16) {
+      // w8 = w4^2, w9..w15 = w8 w1..w7: at most four products deep.
+      w[8] = cmul(w[4], w[4]);
+#pragma unroll
+      for (int q = 1; q < 8; ++q) w[8 + q] = cmul(w[8], w[q]);
+    }
+#pragma unroll
+    for (int r = 1; r < R; ++r) y[r] = cmul(y[r], w[r]);
+  }
+  dft_r<R, INV>(y);
+}
+
 // One radix-R Stockham pass over NF = 2^LG points in LDS after sub-transforms
 // of size ns = 2^lns: butterfly b (k = b mod ns) reads b + r NF/R, multiplies
-// input r by W^{r k} of size R ns (tw[t] = exp(-j 2 pi t / NF); powers by
-// repeated products), and writes output r to (b - k) R + k + r ns.  All reads
-// complete (barrier) before any write.
+// input r by W^{r k} of size R ns (twiddle_dft), and writes output r to
+// (b - k) R + k + r ns.  All reads complete (barrier) before any write.
 template <int LG, int T, int R, bool INV>
-__device__ __forceinline__ void stockham_pass(float2* buf, const float2* __restrict__ tw, int lns) {
+__device__ __forceinline__ void stockham_pass(float2* buf, const float2* __restrict__ tw, int lns,
+                                              int tid = threadIdx.x) {
   constexpr int NF = 1 << LG;
   constexpr int kT = T;
   constexpr int NB = NF / R;
   constexpr int PER = (NB + kT - 1) / kT;
   constexpr int LGR = R == 16 ? 4 : (R == 8 ? 3 : (R == 4 ? 2 : 1));
-  const int tid = threadIdx.x;
   const int ns = 1 << lns;
   float2 y[PER][R];
 #pragma unroll
@@ -129,33 +165,7 @@ __device__ __forceinline__ void stockham_pass(float2* buf, const float2* __restr
     if (NB % kT == 0 || b < NB) {
 #pragma unroll
       for (int r = 0; r < R; ++r) y[p][r] = buf[slot<LG>(b + r * NB)];
-      if (lns > 0) {
-        const int k = b & (ns - 1);
-        float2 w[R];
-        w[1] = tw[k << (LG - lns - LGR)];
-        if (INV) w[1].y = -w[1].y;
-        // Powers by few products: w2 = w1^2, w3 = w2 w1, w4 = w2^2,
-        // w5..w7 = w4 w1..w3.
-        if constexpr (R >= 4) {
-          w[2] = cmul(w[1], w[1]);
-          w[3] = cmul(w[2], w[1]);
-        }
-        if constexpr (R >= 8) {
-          w[4] = cmul(w[2], w[2]);
-          w[5] = cmul(w[4], w[1]);
-          w[6] = cmul(w[4], w[2]);
-          w[7] = cmul(w[4], w[3]);
-        }
-        if constexpr (R == 16) {
-          // w8 = w4^2, w9..w15 = w8 w1..w7: at most four products deep.
-          w[8] = cmul(w[4], w[4]);
-#pragma unroll
-          for (int q = 1; q < 8; ++q) w[8 + q] = cmul(w[8], w[q]);
-        }
-#pragma unroll
-        for (int r = 1; r < R; ++r) y[p][r] = cmul(y[p][r], w[r]);
-      }
-      dft_r<R, INV>(y[p]);
+      twiddle_dft<LG, R, INV>(y[p], tw, b, lns);
     }
   }
   __syncthreads();
@@ -170,6 +180,47 @@ __device__ __forceinline__ void stockham_pass(float2* buf, const float2* __restr
     }
   }
   __syncthreads();
+}
+
+// FFT of NF = 2^LG points held in registers by T = NF/16 threads: on entry
+// x[e] is point tid + e T, on return x[e] is output tid + e T.  The first
+// pass (radix R0 = 2, 4, 8 or 16 as LG mod 4 is 1, 2, 3 or 0; butterfly
+// b = tid + p T reads b + r NF/R0, i.e. x[p + r 16/R0]) and the last radix-16
+// pass (butterfly tid writes tid + r T) touch only the thread's own points,
+// so only the passes between go through LDS.  buf (lds_words<LG>() words) is
+// scratch: every thread must be done reading it when the call starts, and a
+// barrier must separate the call's last reads of buf from the next writes.
+template <int LG, bool INV>
+__device__ __forceinline__ void fft_regs(float2* buf, const float2* __restrict__ tw, float2 (&x)[16]) {
+  constexpr int NF = 1 << LG, T = NF / 16;
+  constexpr int LGR0 = LG % 4 == 0 ? 4 : LG % 4;
+  constexpr int R0 = 1 << LGR0, PER0 = 16 / R0;
+  static_assert(LG >= 8, "two passes at least");
+  // The thread index goes through an empty asm per call, so that the LDS
+  // addresses, which depend on it alone, are recomputed in each transform
+  // instead of being kept live (and spilled) from one transform to the next.
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+#pragma unroll
+  for (int p = 0; p < PER0; ++p) {
+    float2 y[R0];
+#pragma unroll
+    for (int r = 0; r < R0; ++r) y[r] = x[p + r * PER0];
+    dft_r<R0, INV>(y);
+    const int dst = (tid + p * T) << LGR0;
+#pragma unroll
+    for (int r = 0; r < R0; ++r) buf[slot<LG>(dst + r)] = y[r];
+  }
+  __syncthreads();
+  int lns = LGR0;
+#pragma unroll
+  for (int pass = 0; pass < (LG - LGR0) / 4 - 1; ++pass) {
+    stockham_pass<LG, T, 16, INV>(buf, tw, lns, tid);
+    lns += 4;
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) x[r] = buf[slot<LG>(tid + r * T)];
+  twiddle_dft<LG, 16, INV>(x, tw, tid, lns);
 }
 
 // In-place FFT of NF = 2^LG points held in LDS at slots slot<LG>(i) by a

@@ -395,26 +395,33 @@ __device__ __forceinline__ int xcd_line(int b, int n) {
 
 constexpr int rows_threads(int lg) { return (1 << lg) / 16; }
 
-// Length-nr DFT of the line staged as buf[slot(n)] = z[n] conj(b[n]) (n < nr,
-// zeros above): on return buf[slot(k)] holds (z conj(b)) * b circularly
-// convolved, i.e. DFT(z)[k] = conj(b[k]) buf[k] for k < nr.
+// Chirp-z core of a length-nr DFT on a line held in registers (x[e] = point
+// tid + e T, fft::fft_regs).  On entry x holds z[n] conj(b[n]) for n < nr and
+// zeros above; on return it holds that sequence circularly convolved with b,
+// so DFT(z)[k] = conj(b[k]) x[k] for k < nr.  The chirp-spectrum product
+// between the two transforms needs no LDS pass.  The caller's reads of buf end
+// before the call, and it synchronises before its next writes of buf.
 template <int LG>
-__device__ __forceinline__ void chirp_convolve(float2* buf, const RowArgs& r) {
-  constexpr int NF = 1 << LG, T = rows_threads(LG), E = NF / T;
-  fft::fft_lds<LG, T, false>(buf, r.tw);
-#pragma unroll 4
-  for (int e = 0; e < E; ++e) {
-    const int i = threadIdx.x + e * T;
-    buf[fft::slot<LG>(i)] = cmul(buf[fft::slot<LG>(i)], r.bh[i]);
-  }
+__device__ __forceinline__ void chirp_convolve_regs(float2* buf, const RowArgs& r, float2 (&x)[16]) {
+  constexpr int T = rows_threads(LG);
+  fft::fft_regs<LG, false>(buf, r.tw, x);
+  // An opaque index keeps the 16 table loads after the transform (hoisted
+  // above it they would hold 32 registers through it).
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+#pragma unroll
+  for (int e = 0; e < 16; ++e) x[e] = cmul(x[e], r.bh[tid + e * T]);
   __syncthreads();
-  fft::fft_lds<LG, T, true>(buf, r.tw);
+  fft::fft_regs<LG, true>(buf, r.tw, x);
 }
 
 // One workgroup per azimuth-frequency line m; nr <= K T, 2 nr - 1 <= 2^LG.
+// Thread tid holds samples tid + e T of the line in registers from the read
+// to the write; LDS carries the transforms' inner passes and the Stolt taps.
 template <int LG, int K, int P>
 __global__ __launch_bounds__(rows_threads(LG), 4) void wk_rows(RowArgs r) {
-  constexpr int NF = 1 << LG, T = rows_threads(LG), E = NF / T;
+  constexpr int T = rows_threads(LG);
+  static_assert(K <= 16, "K samples per thread of the 16 held");
   extern __shared__ float2 buf[];
   const StoltArgs& a = r.st;
   const int m = xcd_line(blockIdx.x, a.na);
@@ -423,29 +430,30 @@ __global__ __launch_bounds__(rows_threads(LG), 4) void wk_rows(RowArgs r) {
   float2* out = a.out + (size_t)m * r.out_line;
   const int tid = threadIdx.x;
 
-  // Forward range DFT.
-#pragma unroll 4
-  for (int e = 0; e < E; ++e) {
+  // Forward range DFT of z conj(b), zeros outside [nz_lo, nz_hi).
+  float2 x[16];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
     const int i = tid + e * T;
-    buf[fft::slot<LG>(i)] = i >= r.nz_lo && i < r.nz_hi ? cmul(in[(size_t)i * r.in_sample], conjf2(r.chirp[i]))
-                                                         : make_float2(0.0f, 0.0f);
+    x[e] = i >= r.nz_lo && i < r.nz_hi ? cmul(in[(size_t)i * r.in_sample], conjf2(r.chirp[i]))
+                                       : make_float2(0.0f, 0.0f);
   }
-  __syncthreads();
-  chirp_convolve<LG>(buf, r);
+  chirp_convolve_regs<LG>(buf, r, x);
 
   // Range spectrum X[k] = conj(b[k]) c[k], times the reference function;
   // Stolt centres kept per output.
   const double kfa2 = line_kfa2(a, m);
   int fl[K];
   float fr_off[K];
+  __syncthreads();  // the transform's last reads of buf are done
 #pragma unroll
   for (int t = 0; t < K; ++t) {
     const int j = tid + t * T;
     fl[t] = 0;
     fr_off[t] = 0.0f;
     if (j < nr) {
-      const float2 x = cmul(buf[fft::slot<LG>(j)], conjf2(r.chirp[j]));
-      buf[fft::slot<LG>(j)] = stolt_sample(a, j, kfa2, x, &fl[t], &fr_off[t]);
+      const float2 v = cmul(x[t], conjf2(r.chirp[j]));
+      buf[fft::slot<LG>(j)] = stolt_sample(a, j, kfa2, v, &fl[t], &fr_off[t]);
     }
   }
   __syncthreads();
@@ -455,27 +463,23 @@ __global__ __launch_bounds__(rows_threads(LG), 4) void wk_rows(RowArgs r) {
     const int j = tid + t * T;
     y[t] = j < nr ? stolt_output<LG, P>(a, buf, fl[t], fr_off[t]) : make_float2(0.0f, 0.0f);
   }
-  __syncthreads();
+  __syncthreads();  // the Stolt taps' reads of buf are done
 
   // Inverse range DFT: x[n] = conj(DFT(conj(Y))[n]) = b[n] conj(c[n]) with the
   // line staged as conj(Y[n] b[n]).
 #pragma unroll
+  for (int e = 0; e < 16; ++e) x[e] = make_float2(0.0f, 0.0f);
+#pragma unroll
   for (int t = 0; t < K; ++t) {
     const int j = tid + t * T;
-    if (j < nr) buf[fft::slot<LG>(j)] = conjf2(cmul(y[t], r.chirp[j]));
+    if (j < nr) x[t] = conjf2(cmul(y[t], r.chirp[j]));
   }
-#pragma unroll 4
-  for (int e = 0; e < E; ++e) {
-    const int i = tid + e * T;
-    if (i >= nr) buf[fft::slot<LG>(i)] = make_float2(0.0f, 0.0f);
-  }
-  __syncthreads();
-  chirp_convolve<LG>(buf, r);
+  chirp_convolve_regs<LG>(buf, r, x);
 #pragma unroll
   for (int t = 0; t < K; ++t) {
     const int j = tid + t * T;
     if (j < nr) {
-      const float2 v = cmul(r.chirp[j], conjf2(buf[fft::slot<LG>(j)]));
+      const float2 v = cmul(r.chirp[j], conjf2(x[t]));
       int o = j + r.out_shift;
       if (o >= nr) o -= nr;
       out[(size_t)o * r.out_sample] = cmul(v, r.out_rot);
@@ -734,13 +738,14 @@ hipError_t launch_az_inv(float2* img, const float2* tw, int nr, int lg, hipStrea
 // pad_rg + c of the [nr][na] workspace.
 template <int LG>
 __global__ __launch_bounds__(rows_threads(LG)) void wk_az_fwd_bs(AzArgs a, RowArgs t) {
-  constexpr int NF = 1 << LG, T = rows_threads(LG), E = NF / T;
+  constexpr int T = rows_threads(LG);
   extern __shared__ float2 buf[];
   const int c = xcd_line(blockIdx.x, a.nr_in);
   const int tid = threadIdx.x;
   const int na = a.na;
-#pragma unroll 4
-  for (int e = 0; e < E; ++e) {
+  float2 x[16];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
     const int m = tid + e * T;
     float2 v = make_float2(0.0f, 0.0f);
     if (m < na) {
@@ -749,42 +754,40 @@ __global__ __launch_bounds__(rows_threads(LG)) void wk_az_fwd_bs(AzArgs a, RowAr
         const double rev = a.rev_per_line * ((double)m - 0.5 * (double)na);
         float sn, cs;
         sincospif(2.0f * (float)(rev - rint(rev)), &sn, &cs);
-        const float2 x = a.echo[(size_t)mi * a.nr_in + c];
-        v = make_float2(x.x * cs - x.y * sn, x.x * sn + x.y * cs);
+        const float2 z = a.echo[(size_t)mi * a.nr_in + c];
+        v = make_float2(z.x * cs - z.y * sn, z.x * sn + z.y * cs);
       }
       v = cmul(v, conjf2(t.chirp[m]));
     }
-    buf[fft::slot<LG>(m)] = v;
+    x[e] = v;
   }
-  __syncthreads();
-  chirp_convolve<LG>(buf, t);
+  chirp_convolve_regs<LG>(buf, t, x);
   float2* row = a.out + (size_t)(a.pad_rg + c) * na;
-#pragma unroll 4
-  for (int e = 0; e < E; ++e) {
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
     const int k = tid + e * T;
-    if (k < na) row[k] = cmul(buf[fft::slot<LG>(k)], conjf2(t.chirp[k]));
+    if (k < na) row[k] = cmul(x[e], conjf2(t.chirp[k]));
   }
 }
 
 // Inverse (unscaled) of image column r in place: conj(DFT(conj(x))).
 template <int LG>
 __global__ __launch_bounds__(rows_threads(LG)) void wk_az_inv_bs(float2* img, RowArgs t, int na, int nr) {
-  constexpr int NF = 1 << LG, T = rows_threads(LG), E = NF / T;
+  constexpr int T = rows_threads(LG);
   extern __shared__ float2 buf[];
   const int r = xcd_line(blockIdx.x, nr);
   const int tid = threadIdx.x;
-#pragma unroll 4
-  for (int e = 0; e < E; ++e) {
+  float2 x[16];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
     const int m = tid + e * T;
-    buf[fft::slot<LG>(m)] =
-        m < na ? cmul(conjf2(img[(size_t)m * nr + r]), conjf2(t.chirp[m])) : make_float2(0.0f, 0.0f);
+    x[e] = m < na ? cmul(conjf2(img[(size_t)m * nr + r]), conjf2(t.chirp[m])) : make_float2(0.0f, 0.0f);
   }
-  __syncthreads();
-  chirp_convolve<LG>(buf, t);
-#pragma unroll 4
-  for (int e = 0; e < E; ++e) {
+  chirp_convolve_regs<LG>(buf, t, x);
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
     const int k = tid + e * T;
-    if (k < na) img[(size_t)k * nr + r] = conjf2(cmul(buf[fft::slot<LG>(k)], conjf2(t.chirp[k])));
+    if (k < na) img[(size_t)k * nr + r] = conjf2(cmul(x[e], conjf2(t.chirp[k])));
   }
 }
 
