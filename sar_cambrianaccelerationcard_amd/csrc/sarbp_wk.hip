@@ -629,7 +629,8 @@ __global__ __launch_bounds__((1 << LG) / 16) void wk_az_fwd(AzArgs a) {
   extern __shared__ float2 buf[];
   const int c = xcd_line(blockIdx.x, a.nr_in);
   const int tid = threadIdx.x;
-#pragma unroll 4
+  float2 x[16];
+#pragma unroll
   for (int e = 0; e < 16; ++e) {
     const int m = tid + e * T;
     const int mi = m - a.pad_az;
@@ -639,24 +640,20 @@ __global__ __launch_bounds__((1 << LG) / 16) void wk_az_fwd(AzArgs a) {
       const double rev = a.rev_per_line * ((double)m - 0.5 * (double)NF);
       float sn, cs;
       sincospif(2.0f * (float)(rev - rint(rev)), &sn, &cs);
-      const float2 x = a.echo[(size_t)mi * a.nr_in + c];
-      v = make_float2(x.x * cs - x.y * sn, x.x * sn + x.y * cs);
+      const float2 z = a.echo[(size_t)mi * a.nr_in + c];
+      v = make_float2(z.x * cs - z.y * sn, z.x * sn + z.y * cs);
     }
-    buf[fft::slot<LG>(m)] = v;
+    x[e] = v;
   }
-  __syncthreads();
-  fft::fft_lds<LG, T, false>(buf, a.tw);
+  fft::fft_regs<LG, false>(buf, a.tw, x);
   float2* row = a.out + (size_t)(a.pad_rg + c) * NF;
-#pragma unroll 4
-  for (int e = 0; e < 16; ++e) {
-    const int k = tid + e * T;
-    row[k] = buf[fft::slot<LG>(k)];
-  }
+#pragma unroll
+  for (int e = 0; e < 16; ++e) row[tid + e * T] = x[e];
 }
 
-// Azimuth inverse FFT of image column r ([na][nr], na = 2^LG) in LDS, in
-// place: the column is read whole before it is written; columns in flight on
-// one XCD share cache lines in its L2.  Unscaled (the line kernel applies
+// Azimuth inverse FFT of image column r ([na][nr], na = 2^LG), in place: the
+// column is read whole before it is written; columns in flight on one XCD
+// share cache lines in its L2.  Unscaled (the line kernel applies
 // 1/(na nr)).
 template <int LG>
 __global__ __launch_bounds__((1 << LG) / 16) void wk_az_inv_col(float2* img, const float2* __restrict__ tw, int nr) {
@@ -664,56 +661,43 @@ __global__ __launch_bounds__((1 << LG) / 16) void wk_az_inv_col(float2* img, con
   extern __shared__ float2 buf[];
   const int r = xcd_line(blockIdx.x, nr);
   const int tid = threadIdx.x;
-#pragma unroll 4
-  for (int e = 0; e < 16; ++e) {
-    const int m = tid + e * T;
-    buf[fft::slot<LG>(m)] = img[(size_t)m * nr + r];
-  }
-  __syncthreads();
-  fft::fft_lds<LG, T, true>(buf, tw);
-#pragma unroll 4
-  for (int e = 0; e < 16; ++e) {
-    const int k = tid + e * T;
-    img[(size_t)k * nr + r] = buf[fft::slot<LG>(k)];
-  }
+  float2 x[16];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) x[e] = img[(size_t)(tid + e * T) * nr + r];
+  fft::fft_regs<LG, true>(buf, tw, x);
+#pragma unroll
+  for (int e = 0; e < 16; ++e) img[(size_t)(tid + e * T) * nr + r] = x[e];
 }
 
 // Two adjacent columns (2p, 2p + 1) per workgroup for even nr: 16-byte
-// accesses per image row.
+// accesses per image row; the two transforms take turns in one LDS buffer.
 template <int LG>
 __global__ __launch_bounds__((1 << LG) / 16) void wk_az_inv_col2(float2* img, const float2* __restrict__ tw, int nr) {
-  constexpr int NF = 1 << LG, T = NF / 16, W = fft::lds_words<LG>();
+  constexpr int NF = 1 << LG, T = NF / 16;
   extern __shared__ float2 buf[];
   const int r = 2 * xcd_line(blockIdx.x, nr >> 1);
   const int tid = threadIdx.x;
-#pragma unroll 4
+  float2 x[16], y[16];
+#pragma unroll
   for (int e = 0; e < 16; ++e) {
-    const int m = tid + e * T;
-    const float4 v = *reinterpret_cast<const float4*>(img + (size_t)m * nr + r);
-    buf[fft::slot<LG>(m)] = make_float2(v.x, v.y);
-    buf[W + fft::slot<LG>(m)] = make_float2(v.z, v.w);
+    const float4 v = *reinterpret_cast<const float4*>(img + (size_t)(tid + e * T) * nr + r);
+    x[e] = make_float2(v.x, v.y);
+    y[e] = make_float2(v.z, v.w);
   }
-  __syncthreads();
-  fft::fft_lds<LG, T, true>(buf, tw);
-  fft::fft_lds<LG, T, true>(buf + W, tw);
-#pragma unroll 4
-  for (int e = 0; e < 16; ++e) {
-    const int k = tid + e * T;
-    const float2 a = buf[fft::slot<LG>(k)], b = buf[W + fft::slot<LG>(k)];
-    *reinterpret_cast<float4*>(img + (size_t)k * nr + r) = make_float4(a.x, a.y, b.x, b.y);
-  }
+  fft::fft_regs<LG, true>(buf, tw, x);
+  __syncthreads();  // the first transform's last reads of buf are done
+  fft::fft_regs<LG, true>(buf, tw, y);
+#pragma unroll
+  for (int e = 0; e < 16; ++e)
+    *reinterpret_cast<float4*>(img + (size_t)(tid + e * T) * nr + r) = make_float4(x[e].x, x[e].y, y[e].x, y[e].y);
 }
 
 template <int LG>
 hipError_t launch_az_inv_lg(float2* img, const float2* tw, int nr, hipStream_t s) {
   // 16-byte accesses need an even row length and a 16-byte aligned image.
   if ((nr & 1) == 0 && (reinterpret_cast<uintptr_t>(img) & 15) == 0) {
-    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&wk_az_inv_col2<LG>),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                                             (int)(2 * fft::lds_words<LG>() * sizeof(float2)));
-    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(wk_az_inv_col2<LG>, dim3((unsigned)(nr / 2)), dim3((1 << LG) / 16),
-                       (size_t)2 * fft::lds_words<LG>() * sizeof(float2), s, img, tw, nr);
+                       (size_t)fft::lds_words<LG>() * sizeof(float2), s, img, tw, nr);
     return hipGetLastError();
   }
   hipLaunchKernelGGL(wk_az_inv_col<LG>, dim3((unsigned)nr), dim3((1 << LG) / 16),
