@@ -833,6 +833,8 @@ hipError_t launch_az(const AzArgs& a, int lg, hipStream_t s) {
 
 // Smallest LG in [11, 14] with 2 nr - 1 <= 2^LG (nr <= kRowsMaxRg).
 int rows_lg(int nr) {
+  static_assert(kRowsMaxRg >= kWkMaxRg && 2 * kRowsMaxRg - 1 <= (1 << 14),
+                "every line check_stolt admits fits the 16384-point line kernel");
   int lg = 11;
   while ((1 << lg) < 2 * nr - 1) ++lg;
   return lg;
