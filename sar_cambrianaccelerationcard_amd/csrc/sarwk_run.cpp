@@ -14,7 +14,7 @@
 //             [--vr 7062] [--f-nc -6900] [--taps 6] [--window floor|centered]
 //             [--range-ref r0|centre] [--device 0] [--info | -i] [--out | -o [FILE]]
 //
-// Vr and f_nc default to the driver's constants (wk.cc:62-64); Kr is taken
+// Vr and f_nc default to the constants of wk.cc:62-64; Kr is taken
 // as stored in the parameter file (signed: negative for a down-chirp).
 // --range-ref centre (not in the reference) takes R0 as the slant range of
 // the first echo sample and references the window's centre, so targets in

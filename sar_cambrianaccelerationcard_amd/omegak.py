@@ -135,7 +135,7 @@ def simulate(scene: WkScene, targets, amplitudes=None) -> np.ndarray:
 def radarsat_params(rp, *, f_nc: float | None = None, taps: int = 6, window="floor",
                     range_ref="r0") -> native.WkParams:
     """omega-K parameters of the RADARSAT-1 scene (radarsat.RunParams): the
-    down-chirp (kr = -Kr), r0 = R0, the driver's vr and Doppler centroid
+    down-chirp (kr = -Kr), r0 = R0, wk.cc's vr and Doppler centroid
     (wk.cc:62-64 / mian_wka_sinc.m:32-34) unless f_nc is given.  R0 is the
     slant range of the first sample, which is what range_ref "centre" takes."""
     return native.make_wk_params(c=rp.c, f0=rp.f0, fs=rp.Fr, prf=rp.PRF, kr=-rp.Kr, vr=rp.Vr, r0=rp.R0,

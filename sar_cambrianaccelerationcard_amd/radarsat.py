@@ -60,7 +60,7 @@ def unpack4(codes: np.ndarray) -> np.ndarray:
 @dataclass
 class RunParams:
     """The scene's radar parameters (CD_run_params.mat) plus the omega-K
-    driver's velocity and Doppler-centroid constants."""
+    velocity and Doppler-centroid constants of wk.cc:62-64."""
     c: float
     f0: float
     Fr: float

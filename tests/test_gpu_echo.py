@@ -1,7 +1,7 @@
 """sarbp_launch_generate_echo (drop-in for launchGenerateEcho,
 code/SAR_BP/CUDA/BPA_CUDA_kernels.cu:79-89) against the FP64 echo oracle:
 the reference driver's own scene (all zero), delayed fast-time windows with
-the driver's five targets and with many random targets, argument checks."""
+the five targets of BPA_CUDA.cpp:25-48 and with many random targets, argument checks."""
 import ctypes as C
 import math
 import os
@@ -37,7 +37,7 @@ def test_reference_driver_scene():
 
 
 def test_reference_targets_in_a_delayed_window():
-    # The driver's five targets and slow times, fast times starting before
+    # The five targets and slow times of BPA_CUDA.cpp, fast times starting before
     # the nearest echo and long enough for the farthest.
     p = native.SarParams.reference_defaults()
     p.La = 60.0
