@@ -2,7 +2,7 @@
 // of radix 8 (n/8 threads) or 16 (n/16 threads), after one smaller-radix pass
 // when log2 n is not a multiple of 3 or 4; used by range compression
 // (sarbp_range.hip) and by the omega-K transforms (sarbp_wk.hip).  fft_regs
-// keeps the points in registers (16 per thread) and sends only the inner
+// keeps the points in registers (8 or 16 per thread) and sends only the inner
 // passes through LDS.  Not part of the public ABI.
 //
 // Element i of the transform lives at LDS slot fft::slot<LG>(i): one pad word
@@ -182,20 +182,23 @@ __device__ __forceinline__ void stockham_pass(float2* buf, const float2* __restr
   __syncthreads();
 }
 
-// FFT of NF = 2^LG points held in registers by T = NF/16 threads: on entry
-// x[e] is point tid + e T, on return x[e] is output tid + e T.  The first
-// pass (radix R0 = 2, 4, 8 or 16 as LG mod 4 is 1, 2, 3 or 0; butterfly
-// b = tid + p T reads b + r NF/R0, i.e. x[p + r 16/R0]) and the last radix-16
-// pass (butterfly tid writes tid + r T) touch only the thread's own points,
-// so only the passes between go through LDS.  buf (lds_words<LG>() words) is
-// scratch: every thread must be done reading it when the call starts, and a
-// barrier must separate the call's last reads of buf from the next writes.
-template <int LG, bool INV>
-__device__ __forceinline__ void fft_regs(float2* buf, const float2* __restrict__ tw, float2 (&x)[16]) {
-  constexpr int NF = 1 << LG, T = NF / 16;
-  constexpr int LGR0 = LG % 4 == 0 ? 4 : LG % 4;
-  constexpr int R0 = 1 << LGR0, PER0 = 16 / R0;
-  static_assert(LG >= 8, "two passes at least");
+// FFT of NF = 2^LG points held in registers by T = NF/PT threads, PT = 8 or
+// 16 points each: on entry x[e] is point tid + e T, on return x[e] is output
+// tid + e T.  The first pass (radix R0 = 2^(LG mod log2 PT), or PT when that
+// is 1; butterfly b = tid + p T reads b + r NF/R0, i.e. x[p + r PT/R0]) and the
+// last radix-PT pass (butterfly tid writes tid + r T) touch only the thread's
+// own points, so only the passes between go through LDS.  buf
+// (lds_words<LG>() words) is scratch: every thread must be done reading it
+// when the call starts, and a barrier must separate the call's last reads of
+// buf from the next writes.
+template <int LG, bool INV, int PT = 16>
+__device__ __forceinline__ void fft_regs(float2* buf, const float2* __restrict__ tw, float2 (&x)[PT]) {
+  static_assert(PT == 8 || PT == 16, "8 or 16 points per thread");
+  constexpr int LGP = PT == 16 ? 4 : 3;
+  constexpr int NF = 1 << LG, T = NF / PT;
+  constexpr int LGR0 = LG % LGP == 0 ? LGP : LG % LGP;
+  constexpr int R0 = 1 << LGR0, PER0 = PT / R0;
+  static_assert(LG >= 2 * LGP, "two passes at least");
   // The thread index goes through an empty asm per call, so that the LDS
   // addresses, which depend on it alone, are recomputed in each transform
   // instead of being kept live (and spilled) from one transform to the next.
@@ -214,13 +217,13 @@ __device__ __forceinline__ void fft_regs(float2* buf, const float2* __restrict__
   __syncthreads();
   int lns = LGR0;
 #pragma unroll
-  for (int pass = 0; pass < (LG - LGR0) / 4 - 1; ++pass) {
-    stockham_pass<LG, T, 16, INV>(buf, tw, lns, tid);
-    lns += 4;
+  for (int pass = 0; pass < (LG - LGR0) / LGP - 1; ++pass) {
+    stockham_pass<LG, T, PT, INV>(buf, tw, lns, tid);
+    lns += LGP;
   }
 #pragma unroll
-  for (int r = 0; r < 16; ++r) x[r] = buf[slot<LG>(tid + r * T)];
-  twiddle_dft<LG, 16, INV>(x, tw, tid, lns);
+  for (int r = 0; r < PT; ++r) x[r] = buf[slot<LG>(tid + r * T)];
+  twiddle_dft<LG, PT, INV>(x, tw, tid, lns);
 }
 
 // In-place FFT of NF = 2^LG points held in LDS at slots slot<LG>(i) by a

@@ -67,6 +67,40 @@ __device__ __forceinline__ void store_out(void* out, size_t o, float2 v, int dty
     static_cast<uint32_t*>(out)[o] = bf16_rne(v.x) | (bf16_rne(v.y) << 16);
 }
 
+// Upsampled samples [c NF, (c + 1) NF) = q in [c NF / LT, (c + 1) NF / LT):
+// the LT phases (keep[e][r] = phase r at q = tid + e T) are interleaved in LDS
+// one chunk at a time, then the chunk's part of the output window is stored
+// with whole cache lines per store instruction.
+template <int LG, int LT, int T, int E>
+__device__ __forceinline__ void rc_interleave_store(const RcArgs& a, float2* buf, const float2 (&keep)[E][LT],
+                                                    size_t obase) {
+  constexpr int NF = 1 << LG;
+  constexpr int QC = NF / LT;  // q per chunk
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int c = 0; c < LT; ++c) {
+    const int u0 = c * NF;  // first upsampled index of the chunk
+    if (u0 - a.crop >= a.n_out || u0 + NF - a.crop <= 0) continue;  // uniform
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int q = tid + e * T;
+      if (q >= c * QC && q < (c + 1) * QC) {
+#pragma unroll
+        for (int r = 0; r < LT; ++r) buf[fft::slot<LG>((q - c * QC) * LT + r)] = keep[e][r];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = tid + e * T;
+      const int o = u0 + i - a.crop;
+      if ((NF % T == 0 || i < NF) && o >= 0 && o < a.n_out)
+        store_out(a.out, obase + (size_t)o, buf[fft::slot<LG>(i)], a.dtype);
+    }
+  }
+}
+
 // Block m compresses pulse m: raw[m][0, n_raw) -> out[m][0, n_out).
 // LT > 0 (= L in {1, 2, 4}): every phase's samples are kept in registers and
 // the L interleaved phases are written back through LDS in contiguous chunks,
@@ -81,21 +115,53 @@ __global__ __launch_bounds__(rc_threads(LG)) void rc_kernel(RcArgs a) {
   const int tid = threadIdx.x;
   const size_t m = blockIdx.x;
   const float2* row = a.raw + m * (size_t)a.n_raw;
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int i = tid + e * kRcThreads;
-    if (NF % kRcThreads == 0 || i < NF) buf[fft::slot<LG>(i)] = i < a.n_raw ? row[i] : make_float2(0.f, 0.f);
-  }
-  __syncthreads();
-  fft::fft_lds<LG, rc_threads(LG), false>(buf, a.tw);
+  // 8 points per thread (NF >= 2048): the transforms run on registers
+  // (fft::fft_regs), only their inner passes through LDS.
+  constexpr bool kRegs = kRcThreads * 8 == NF && LT > 0;
   float2 spec[E];
+  if constexpr (kRegs) {
+    float2 x[8];
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int i = tid + e * kRcThreads;
-    if (NF % kRcThreads == 0 || i < NF) spec[e] = cmul(buf[fft::slot<LG>(i)], a.hc[i]);
+    for (int e = 0; e < 8; ++e) {
+      const int i = tid + e * kRcThreads;
+      x[e] = i < a.n_raw ? row[i] : make_float2(0.f, 0.f);
+    }
+    fft::fft_regs<LG, false, 8>(buf, a.tw, x);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) spec[e] = cmul(x[e], a.hc[tid + e * kRcThreads]);
+  } else {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = tid + e * kRcThreads;
+      if (NF % kRcThreads == 0 || i < NF) buf[fft::slot<LG>(i)] = i < a.n_raw ? row[i] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    fft::fft_lds<LG, rc_threads(LG), false>(buf, a.tw);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = tid + e * kRcThreads;
+      if (NF % kRcThreads == 0 || i < NF) spec[e] = cmul(buf[fft::slot<LG>(i)], a.hc[i]);
+    }
   }
   const size_t obase = m * (size_t)a.n_out;
-  if constexpr (LT > 0) {
+  if constexpr (kRegs) {
+    float2 keep[8][LT];
+#pragma unroll
+    for (int r = 0; r < LT; ++r) {
+      __syncthreads();  // the previous transform's last reads of buf are done
+      float2 x[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) x[e] = spec[e];
+      fft::fft_regs<LG, true, 8>(buf, a.tw, x);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) keep[e][r] = x[e];
+      if (r + 1 < LT) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) spec[e] = cmul(spec[e], a.ramp[tid + e * kRcThreads]);
+      }
+    }
+    rc_interleave_store<LG, LT, kRcThreads, 8>(a, buf, keep, obase);
+  } else if constexpr (LT > 0) {
     float2 keep[E][LT];
 #pragma unroll
     for (int r = 0; r < LT; ++r) {
@@ -120,31 +186,7 @@ __global__ __launch_bounds__(rc_threads(LG)) void rc_kernel(RcArgs a) {
         }
       }
     }
-    // Upsampled samples [c NF, (c + 1) NF) = q in [c NF / LT, (c + 1) NF / LT):
-    // interleave the phases in LDS, then store the chunk's part of the window.
-    constexpr int QC = NF / LT;  // q per chunk
-#pragma unroll
-    for (int c = 0; c < LT; ++c) {
-      const int u0 = c * NF;  // first upsampled index of the chunk
-      if (u0 - a.crop >= a.n_out || u0 + NF - a.crop <= 0) continue;  // uniform
-      __syncthreads();
-#pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const int q = tid + e * kRcThreads;
-        if (q >= c * QC && q < (c + 1) * QC) {
-#pragma unroll
-          for (int r = 0; r < LT; ++r) buf[fft::slot<LG>((q - c * QC) * LT + r)] = keep[e][r];
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const int i = tid + e * kRcThreads;
-        const int o = u0 + i - a.crop;
-        if ((NF % kRcThreads == 0 || i < NF) && o >= 0 && o < a.n_out)
-          store_out(a.out, obase + (size_t)o, buf[fft::slot<LG>(i)], a.dtype);
-      }
-    }
+    rc_interleave_store<LG, LT, kRcThreads, E>(a, buf, keep, obase);
   } else {
     for (int r = 0; r < a.L; ++r) {
       // outputs o = L q + r - crop in [0, n_out) for q in [q_lo, q_hi]
