@@ -651,6 +651,42 @@ __global__ __launch_bounds__((1 << LG) / 16) void wk_az_fwd(AzArgs a) {
   for (int e = 0; e < 16; ++e) row[tid + e * T] = x[e];
 }
 
+// Two adjacent echo columns (2p, 2p + 1) per workgroup for even nr_in and a
+// 16-byte aligned echo: 16-byte reads per echo row, one down-conversion
+// factor per row for both, and the two transforms taking turns in one LDS
+// buffer; written as workspace rows pad_rg + 2p and pad_rg + 2p + 1.
+template <int LG>
+__global__ __launch_bounds__((1 << LG) / 16) void wk_az_fwd2(AzArgs a) {
+  constexpr int NF = 1 << LG, T = NF / 16;
+  extern __shared__ float2 buf[];
+  const int c = 2 * xcd_line(blockIdx.x, a.nr_in >> 1);
+  const int tid = threadIdx.x;
+  float2 x[16], y[16];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    const int m = tid + e * T;
+    const int mi = m - a.pad_az;
+    x[e] = y[e] = make_float2(0.0f, 0.0f);
+    if (mi >= 0 && mi < a.na_in) {
+      const double rev = a.rev_per_line * ((double)m - 0.5 * (double)NF);
+      float sn, cs;
+      sincospif(2.0f * (float)(rev - rint(rev)), &sn, &cs);
+      const float4 z = *reinterpret_cast<const float4*>(a.echo + (size_t)mi * a.nr_in + c);
+      x[e] = make_float2(z.x * cs - z.y * sn, z.x * sn + z.y * cs);
+      y[e] = make_float2(z.z * cs - z.w * sn, z.z * sn + z.w * cs);
+    }
+  }
+  fft::fft_regs<LG, false>(buf, a.tw, x);
+  __syncthreads();  // the first transform's last reads of buf are done
+  fft::fft_regs<LG, false>(buf, a.tw, y);
+  float2* row = a.out + (size_t)(a.pad_rg + c) * NF;
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    row[tid + e * T] = x[e];
+    row[NF + tid + e * T] = y[e];
+  }
+}
+
 // Azimuth inverse FFT of image column r ([na][nr], na = 2^LG), in place: the
 // column is read whole before it is written; columns in flight on one XCD
 // share cache lines in its L2.  Unscaled (the line kernel applies
@@ -816,8 +852,13 @@ int az_lg(int na) {
 
 template <int LG>
 hipError_t launch_az_lg(const AzArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(wk_az_fwd<LG>, dim3((unsigned)a.nr_in), dim3((1 << LG) / 16),
-                     (size_t)fft::lds_words<LG>() * sizeof(float2), s, a);
+  // 16-byte reads need an even echo row length and a 16-byte aligned echo.
+  if ((a.nr_in & 1) == 0 && (reinterpret_cast<uintptr_t>(a.echo) & 15) == 0)
+    hipLaunchKernelGGL(wk_az_fwd2<LG>, dim3((unsigned)(a.nr_in / 2)), dim3((1 << LG) / 16),
+                       (size_t)fft::lds_words<LG>() * sizeof(float2), s, a);
+  else
+    hipLaunchKernelGGL(wk_az_fwd<LG>, dim3((unsigned)a.nr_in), dim3((1 << LG) / 16),
+                       (size_t)fft::lds_words<LG>() * sizeof(float2), s, a);
   return hipGetLastError();
 }
 
