@@ -231,6 +231,99 @@ def leg_mixed(torch, ctx, native, W, args) -> dict:
             "batched_ms": t_batch * 1e3, "batched_equals_plain_bitwise": bool(same)}
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def median_time(fn, runs: int = 3) -> float:
+    """Median wall time of `runs` calls of fn() (the caller runs a warm-up first)."""
+    ts = []
+    for _ in range(runs):
+        t = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t)
+    return float(np.median(ts))
+
+
+def leg_cpu(sc, gpu_img, interp, args):
+    """The reference's CPU BP semantics (BPA_SAR.m:178-197) restated in C
+    (oracle/sarbp_oracle.c): FP64 and FP32, all host threads and one thread,
+    median of 3 timed runs after a warm-up, on a fixed row sample of the
+    headline image and on the whole small plain image (config 0).  The FP64
+    warm-up rows are also the accuracy reference of the GPU image."""
+    import oracle.oracle as O
+    from sar_cambrianaccelerationcard_amd import metrics as M
+    from sar_cambrianaccelerationcard_amd import scene as S
+    nthr = O.max_threads()
+    rows = sorted({int(r) for r in sc.target_pixels[:, 1]})
+    r_lo = max(0, rows[0] - args.cpu_rows // 2)
+    r_hi = min(sc.grid["ny"], r_lo + args.cpu_rows)
+    p = O.params(sc.c, sc.fc, sc.r0, sc.dr, interp, sc.kaiser_beta)
+    nx = sc.grid["nx"]
+    w_rows = (r_hi - r_lo) * nx * sc.n_pulses
+    tc = time.perf_counter()
+    ref = O.backproject(sc.profiles, sc.pos, sc.grid, p, rows=(r_lo, r_hi), n_threads=nthr)
+    t_warm = time.perf_counter() - tc
+    acc = {"rel_l2_rows": M.rel_l2(gpu_img[r_lo:r_hi], ref), "rows": [r_lo, r_hi]}
+    t64 = median_time(lambda: O.backproject(sc.profiles, sc.pos, sc.grid, p, rows=(r_lo, r_hi),
+                                            n_threads=nthr))
+    f32 = O.backproject_f32(sc.profiles, sc.pos, sc.grid, p, rows=(r_lo, r_hi), n_threads=nthr)
+    t32 = median_time(lambda: O.backproject_f32(sc.profiles, sc.pos, sc.grid, p, rows=(r_lo, r_hi),
+                                                n_threads=nthr))
+    # one thread: a 512-pixel segment of the first sample row
+    seg = {"origin": list(sc.grid["origin"]), "ux": list(sc.grid["ux"]), "uy": list(sc.grid["uy"]),
+           "nx": min(512, nx), "ny": sc.grid["ny"]}
+    w_seg = seg["nx"] * sc.n_pulses
+    O.backproject(sc.profiles, sc.pos, seg, p, rows=(r_lo, r_lo + 1), n_threads=1)
+    t64_1 = median_time(lambda: O.backproject(sc.profiles, sc.pos, seg, p, rows=(r_lo, r_lo + 1), n_threads=1))
+    O.backproject_f32(sc.profiles, sc.pos, seg, p, rows=(r_lo, r_lo + 1), n_threads=1)
+    t32_1 = median_time(lambda: O.backproject_f32(sc.profiles, sc.pos, seg, p, rows=(r_lo, r_lo + 1),
+                                                  n_threads=1))
+    log(f"CPU oracle rows {r_lo}-{r_hi} ({w_rows:.3e} W): FP64 {w_rows / t64:.3e} W/s on {nthr} threads "
+        f"(warm-up {t_warm:.1f}s), FP32 {w_rows / t32:.3e} W/s; one thread FP64 {w_seg / t64_1:.3e}, "
+        f"FP32 {w_seg / t32_1:.3e} W/s; GPU rows rel-L2 {acc['rel_l2_rows']:.3e}")
+    # config 0 (512 pulses x 2048 samples -> 512 x 512 at 0.5 m, linear) in full
+    s0 = S.make_scene(S.load_config("config0"))
+    p0 = O.params(s0.c, s0.fc, s0.r0, s0.dr, s0.interp, s0.kaiser_beta)
+    w0 = s0.work
+    O.backproject(s0.profiles, s0.pos, s0.grid, p0, n_threads=nthr)
+    c0_64 = median_time(lambda: O.backproject(s0.profiles, s0.pos, s0.grid, p0, n_threads=nthr))
+    O.backproject_f32(s0.profiles, s0.pos, s0.grid, p0, n_threads=nthr)
+    c0_32 = median_time(lambda: O.backproject_f32(s0.profiles, s0.pos, s0.grid, p0, n_threads=nthr))
+    rows0 = (0, 32)  # one thread: the first 32 rows
+    w0_1 = 32 * s0.grid["nx"] * s0.n_pulses
+    O.backproject(s0.profiles, s0.pos, s0.grid, p0, rows=rows0, n_threads=1)
+    c0_64_1 = median_time(lambda: O.backproject(s0.profiles, s0.pos, s0.grid, p0, rows=rows0, n_threads=1))
+    log(f"CPU oracle config 0 in full ({w0:.3e} W, {s0.interp}): FP64 {w0 / c0_64:.3e} W/s, FP32 "
+        f"{w0 / c0_32:.3e} W/s on {nthr} threads; one thread FP64 {w0_1 / c0_64_1:.3e} W/s")
+    cpu = {"value": w_rows / t64, "unit": "pixel-pulses/s", "cores": nthr, "kind": "port",
+           "sample": f"FP64 oracle (oracle/sarbp_oracle.c), {interp}, image rows [{r_lo},{r_hi}) x "
+                     f"{nx} px x {sc.n_pulses} pulses ({w_rows:.3e} W), median of 3 after a warm-up",
+           "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+           "affinity_cpus": len(os.sched_getaffinity(0)),
+           "runs": {
+               "headline_rows_fp64": {"W_per_s": w_rows / t64, "threads": nthr, "s": t64},
+               "headline_rows_fp32": {"W_per_s": w_rows / t32, "threads": nthr, "s": t32,
+                                      "rel_l2_vs_fp64": M.rel_l2(f32, ref)},
+               "headline_segment_fp64_1thread": {"W_per_s": w_seg / t64_1, "threads": 1, "s": t64_1,
+                                                 "sample": f"row {r_lo}, {seg['nx']} px"},
+               "headline_segment_fp32_1thread": {"W_per_s": w_seg / t32_1, "threads": 1, "s": t32_1,
+                                                 "sample": f"row {r_lo}, {seg['nx']} px"},
+               "config0_full_fp64": {"W_per_s": w0 / c0_64, "threads": nthr, "s": c0_64,
+                                     "interp": s0.interp, "work": w0},
+               "config0_full_fp32": {"W_per_s": w0 / c0_32, "threads": nthr, "s": c0_32,
+                                     "interp": s0.interp, "work": w0},
+               "config0_rows_fp64_1thread": {"W_per_s": w0_1 / c0_64_1, "threads": 1, "s": c0_64_1,
+                                             "sample": "rows [0,32)"}}}
+    return cpu, acc
+
+
 def plain_e2e(torch, ctx, native, W, inputs) -> dict:
     """The reference host flow per image (SAR_Processor, BPA_CUDA.cpp:51-89):
     copy the job's profiles and positions to the device, back-project, copy
@@ -330,6 +423,42 @@ def leg_served(native, W, args, rate: float, inputs, raw_inputs=None) -> dict:
     return out
 
 
+def valu_probe(args) -> int:
+    """Issue rate of independent FP32 FMA chains, hardware-sine chains and their
+    mixes (sarbp_valu_probe): wave64 VALU instructions per CU per cycle at the
+    2.4 GHz peak clock, from HIP events."""
+    import torch
+    from sar_cambrianaccelerationcard_amd import native
+    props = torch.cuda.get_device_properties(0)
+    n_cu = props.multi_processor_count
+    n_blocks, iters = n_cu * 16, 2048
+    out = torch.empty(n_blocks * 256, dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream()
+    res = {"n_cu": n_cu, "n_blocks": n_blocks, "threads_per_block": 256, "iters": iters, "kinds": {}}
+    names = {0: "128 fma (VOP3)", 1: "128 sin", 2: "120 fma + 8 sin", 3: "112 fma + 16 sin",
+             4: "128 fmac (VOP2)", 5: "128 mul (VOP2)", 6: "128 add (VOP2)", 7: "128 fract (VOP1)",
+             8: "128 med3 (VOP3)", 9: "64 fma + 64 fmac", 10: "64 fma + 64 add_u32",
+             11: "64 fma + 64 mad_u32_u24", 12: "64 fma + 64 cvt_f32_i32", 13: "64 fma + 64 fract",
+             14: "64 fma + 64 med3", 15: "64 fma + 64 mul", 16: "64 fma + 64 mov",
+             17: "64 fma + 64 add_f32", 18: "64 fma + 64 lshrrev_b32", 19: "64 fma + 64 and_b32",
+             20: "64 fma + 64 or_b32", 21: "64 fma + 64 min_u32", 22: "64 fma + 64 cndmask_b32",
+             23: "64 fma + 64 lshl_add_u32", 24: "64 fma + 64 add3_u32", 25: "64 fma + 64 cvt_i32_f32",
+             26: "64 fma + 64 max_f32", 27: "64 fma + 64 floor_f32", 28: "64 fma + 64 sub_u32",
+             29: "64 fma + 64 bfe_u32"}
+    for kind in range(30):
+        fn = lambda: native.valu_probe(kind, n_blocks, iters, out.data_ptr(), stream.cuda_stream)
+        fn()
+        t = timed(torch, stream, fn, 5)
+        wave_instr = n_blocks * 4 * iters * 128
+        per_cu_cycle = wave_instr / (n_cu * 2.4e9 * t)
+        res["kinds"][names[kind]] = {"ms": t * 1e3, "wave_instr": wave_instr,
+                                     "wave_instr_per_cu_cycle_at_2p4GHz": per_cu_cycle}
+        log(f"VALU probe {names[kind]}: {t * 1e3:.3f} ms, {per_cu_cycle:.3f} wave-instructions per "
+            f"CU-cycle at 2.4 GHz")
+    print(json.dumps({"valu_probe": res}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -358,6 +487,8 @@ def main() -> int:
     ap.add_argument("--served-raw", type=int, default=1,
                     help="also serve the stream as raw echoes (burst), range-compressed on the device")
     ap.add_argument("--pulses-per-epoch", type=int, default=1024)
+    ap.add_argument("--valu-probe", action="store_true",
+                    help="only run the vector-ALU issue-rate probes (roofline calibration)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -371,6 +502,9 @@ def main() -> int:
         torch.cuda.synchronize()
         log(f"copied {args.copy_bytes} bytes 3 times")
         return 0
+
+    if args.valu_probe:
+        return valu_probe(args)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the BP imager is a single-GPU build "
@@ -435,23 +569,7 @@ def main() -> int:
 
     cpu = acc = None
     if args.cpu_rows > 0:
-        import oracle.oracle as O
-        gpu_img = img.cpu().numpy()
-        rows = sorted({int(r) for r in sc.target_pixels[:, 1]})
-        r_lo = max(0, rows[0] - args.cpu_rows // 2)
-        r_hi = min(sc.grid["ny"], r_lo + args.cpu_rows)
-        p = O.params(sc.c, sc.fc, sc.r0, sc.dr, interp, sc.kaiser_beta)
-        nthr = O.max_threads()
-        tc = time.perf_counter()
-        ref = O.backproject(sc.profiles, sc.pos, sc.grid, p, rows=(r_lo, r_hi), n_threads=nthr)
-        tcpu = time.perf_counter() - tc
-        w_cpu = (r_hi - r_lo) * sc.grid["nx"] * sc.n_pulses
-        cpu = {"value": w_cpu / tcpu, "unit": "pixel-pulses/s", "cores": nthr, "kind": "port",
-               "sample": f"FP64 oracle (oracle/sarbp_oracle.c), image rows [{r_lo},{r_hi}) x "
-                         f"{sc.grid['nx']} px x {sc.n_pulses} pulses ({w_cpu:.3e} W, {tcpu:.1f} s)"}
-        acc = {"rel_l2_rows": M.rel_l2(gpu_img[r_lo:r_hi], ref), "rows": [r_lo, r_hi]}
-        log(f"oracle rows {r_lo}-{r_hi}: {tcpu:.1f}s ({cpu['value']:.3e} W/s on {nthr} threads), "
-            f"rel-L2 {acc['rel_l2_rows']:.3e}")
+        cpu, acc = leg_cpu(sc, img.cpu().numpy(), interp, args)
     del ds, img
     torch.cuda.empty_cache()
 

@@ -610,6 +610,22 @@ sarbp_status sarbp_sinc8_coefficients(double beta, float* out);
  * in the weight of tap t.  Host-only. */
 sarbp_status sarbp_sinc8_coefficients_whole(double beta, float* out);
 
+/* Vector-ALU issue-rate probe (roofline calibration; no counterpart in the
+ * reference).  Launches n_blocks workgroups of 256 threads; each thread runs
+ * `iters` rounds of 128 VALU instructions over 8 independent FP32 chains and
+ * writes one float to d_out[n_blocks * 256] (device).  kind 0: 128 v_fma_f32
+ * (VOP3) per round; 1: 128 v_sin_f32; 2: 120 v_fma_f32 + 8 v_sin_f32; 3: 112
+ * v_fma_f32 + 16 v_sin_f32; 4: 128 v_fmac_f32 (VOP2); 5: 128 v_mul_f32 (VOP2);
+ * 6: 128 v_add_f32 (VOP2); 7: 128 v_fract_f32 (VOP1); 8: 128 v_med3_f32
+ * (VOP3); 9: 64 v_fma_f32 + 64 v_fmac_f32 alternating; 10-16: 64 v_fma_f32
+ * interleaved with 64 of v_add_u32, v_mad_u32_u24, v_cvt_f32_i32, v_fract_f32,
+ * v_med3_f32, v_mul_f32, v_mov_b32, v_add_f32, v_lshrrev_b32, v_and_b32,
+ * v_or_b32, v_min_u32, v_cndmask_b32, v_lshl_add_u32, v_add3_u32,
+ * v_cvt_i32_f32, v_max_f32, v_floor_f32, v_sub_u32, v_bfe_u32 respectively.
+ * Asynchronous on `stream`; the caller times it. */
+sarbp_status sarbp_valu_probe(int32_t kind, int32_t n_blocks, int32_t iters,
+                              float* d_out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

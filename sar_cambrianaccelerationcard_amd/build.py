@@ -40,7 +40,7 @@ HIP_FLAGS = [
 SARBP_SOURCES = ("sarbp_kernels_nearest.hip", "sarbp_kernels_linear.hip", "sarbp_kernels_sinc8.hip",
                  "sarbp_kernels_sinc8w.hip",
                  "sarbp_api.hip", "sarbp_batch.hip", "sarbp_server.hip",
-                 "sarbp_range.hip", "sarbp_wk.hip", "sarbp_echo.hip")
+                 "sarbp_range.hip", "sarbp_wk.hip", "sarbp_echo.hip", "sarbp_diag.hip")
 
 
 def _run(cmd: list[str]) -> None:

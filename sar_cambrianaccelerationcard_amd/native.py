@@ -213,6 +213,7 @@ _SIGS = {
     "sarbp_wk_padded_size": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "sarbp_omega_k": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.POINTER(WkParams),
                                 C.c_void_p, C.c_void_p]),
+    "sarbp_valu_probe": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]),
     "sarbp_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Grid),
                                  C.POINTER(Params), C.c_void_p, C.c_size_t, C.c_int32, C.c_void_p,
                                  C.POINTER(AutotuneResult)]),
@@ -464,6 +465,12 @@ def sinc8_coefficients_whole(beta: float):
     out = (C.c_float * 64)()
     check(load().sarbp_sinc8_coefficients_whole(float(beta), out), "sarbp_sinc8_coefficients_whole")
     return np.frombuffer(out, dtype=np.float32).reshape(8, 8).copy()
+
+
+def valu_probe(kind: int, n_blocks: int, iters: int, d_out: int, stream: int = 0) -> None:
+    """Enqueues the vector-ALU issue-rate probe (sarbp_valu_probe)."""
+    check(load().sarbp_valu_probe(int(kind), int(n_blocks), int(iters), C.c_void_p(d_out),
+                                  C.c_void_p(stream)), "sarbp_valu_probe")
 
 
 class Plan:

@@ -8,10 +8,12 @@ Mixed-size jobs (64 by default): square images, side log-uniform in
 ground range.  The served stream adds a fraction of large jobs (8192 x 8192
 image, 8192 pulses x 16384 samples).
 
-Back-projection cost does not depend on the sample values, so the jobs of a
-mix read their profiles from shared seeded random buffers (one per size
-class); parity of batched and served jobs against the oracle is covered by
-the test suite on simulated scenes.
+Back-projection cost does not depend on the sample values, so the bulk of a
+mix reads its profiles from shared seeded random buffers (one per size
+class).  JobSpec.simulate() gives a job its own simulated scene instead --
+point targets over complex-Gaussian clutter and AWGN, as the realistic
+4096-pulse workload -- with the job's sizes, spacing and radar; the served and
+batched checks against the oracle run on such jobs.
 """
 from __future__ import annotations
 
@@ -20,6 +22,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import native
+from . import scene
 
 C_LIGHT = 299792458.0
 FC = 9.6e9
@@ -70,6 +73,26 @@ class JobSpec:
         r0 = float(np.hypot(YC, ALT)) - 0.5 * self.n_samples * DR
         return dict(c=C_LIGHT, bandwidth=150e6, pulse_width=2e-6, fs=200e6, t0=2.0 * r0 / C_LIGHT,
                     n_raw=self.n_raw, upsample=4, crop_offset=0, n_out=self.n_samples)
+
+    def scene_config(self, n_targets: int = 8, n_clutter: int = 1024) -> dict:
+        """A scene config (scene.make_scene) with this job's sizes: the
+        realistic workload's radar, track (0.5 m deviations low-passed over
+        200 pulses) and scene statistics, n_targets point targets and
+        n_clutter clutter scatterers, seeded by the job's seed."""
+        cfg = scene.load_config("config1")
+        cfg = scene.scaled_config(cfg, n_pulses=self.n_pulses, nx=self.side, ny=self.side,
+                                  n_samples=self.n_samples, n_clutter=n_clutter,
+                                  n_random_targets=n_targets)
+        cfg["name"] = f"job-{self.seed}"
+        cfg["seed"] = int(self.seed)
+        cfg["image"]["spacing"] = float(self.spacing)
+        cfg["scene"]["target_margin"] = max(24.0, 40.0 * self.spacing)
+        return cfg
+
+    def simulate(self, n_targets: int = 8, n_clutter: int = 1024) -> "scene.Scene":
+        """This job's own simulated scene (its positions, grid and range axis
+        replace positions(), grid() and params()' r0)."""
+        return scene.make_scene(self.scene_config(n_targets, n_clutter))
 
     def params(self, interp="sinc8", precision="ref", dtype="c64") -> native.Params:
         r0 = float(np.hypot(YC, ALT)) - 0.5 * self.n_samples * DR
