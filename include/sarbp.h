@@ -301,6 +301,25 @@ typedef struct sarbp_job {
 sarbp_status sarbp_backproject_batch(sarbp_context* ctx, const sarbp_job* jobs,
                                      int32_t n_jobs, void* stream);
 
+/*
+ * sarbp_backproject_batch without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * same jobs, kernels and results -- each image bitwise that of
+ * sarbp_backproject_batch -- but the work list lives in a fixed device array
+ * of the library (one per GPU, kept until the process ends; its address is
+ * looked up by sarbp_create), written by one host-to-device copy on `stream`
+ * from a static host staging area, and the persistent kernels are launched
+ * directly instead of from captured graphs.  Calls no hipMalloc, hipFree or
+ * other allocating routine.  Limits: at most 256 jobs per call (more:
+ * SARBP_ERR_INVALID_ARGUMENT) and 8 tile shapes (SARBP_ERR_UNSUPPORTED).
+ * Every call on one GPU shares that array: calls whose work overlaps on the
+ * device (other streams, or other host threads) race and must be serialised
+ * by the caller; calls on one stream are ordered.
+ */
+sarbp_status sarbp_backproject_batch_noalloc(sarbp_context* ctx, const sarbp_job* jobs,
+                                             int32_t n_jobs, void* stream);
+
 /* ---- serving --------------------------------------------------------------------
  *
  * A server owns one device.  Requests are queued on submit; an admission

@@ -18,17 +18,19 @@
 
 namespace sarbp {
 
-thread_local std::string g_detail;
+// Detail text of the calling thread's last error: a fixed buffer, so that
+// reporting an error allocates nothing.
+thread_local char g_detail[512];
 
-const std::string& last_detail() { return g_detail; }
+const char* last_detail() { return g_detail; }
 
 sarbp_status fail(sarbp_status st, const char* fmt, ...) {
-  char buf[512];
+  char buf[sizeof g_detail];
   va_list ap;
   va_start(ap, fmt);
   vsnprintf(buf, sizeof buf, fmt, ap);
   va_end(ap);
-  g_detail = buf;
+  std::memcpy(g_detail, buf, sizeof buf);
   return st;
 }
 
@@ -107,6 +109,47 @@ bool finite_all(const double* v, int n) {
 }
 
 double norm3(const double* v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+
+// ---- device scratch of the allocation-free batched launch ----------------
+// Layout: kNoAllocGroups slots of kNoAllocSlot bytes (one per tile-shape
+// group: its BatchHeader at offset 0 and its work counter at offset 64),
+// then the groups' job tables back to back (at most kNoAllocJobs JobDescs).
+constexpr int kNoAllocJobs = 256;
+constexpr int kNoAllocGroups = 8;
+constexpr size_t kNoAllocSlot = 256;
+constexpr size_t kNoAllocCounter = 64;
+constexpr size_t kNoAllocBytes = kNoAllocGroups * kNoAllocSlot + kNoAllocJobs * sizeof(JobDesc);
+constexpr int kMaxDevices = 64;
+
+}  // namespace sarbp
+
+// File-scope device array (each GPU holds its own copy at its own address,
+// for the life of the process).  Host code never uses this name as an
+// address: sarbp_create looks up each device's copy with hipGetSymbolAddress
+// and keeps that pointer in the context.
+static __device__ __align__(256) unsigned char scratch[sarbp::kNoAllocBytes];
+
+namespace sarbp {
+
+std::mutex g_scratch_mu;               // guards g_scratch_dev and the host staging
+void* g_scratch_dev[kMaxDevices] = {};  // each device's address of `scratch`
+alignas(256) unsigned char g_noalloc_host[kNoAllocBytes];  // host image of the scratch
+JobDesc g_noalloc_plan[kNoAllocJobs];  // planned descriptors, submission order
+
+// Looks up `scratch` on `device` (current) once per device; *out = its address.
+sarbp_status scratch_address(int device, void** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return SARBP_OK;  // entry point then reports it
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_dev[device] = p;
+  }
+  *out = g_scratch_dev[device];
+  return SARBP_OK;
+}
 
 }  // namespace sarbp
 
@@ -429,7 +472,7 @@ const char* sarbp_strerror(sarbp_status s) {
 
 int32_t sarbp_abi_version(void) { return SARBP_ABI_VERSION; }
 
-const char* sarbp_last_error_detail(void) { return g_detail.c_str(); }
+const char* sarbp_last_error_detail(void) { return g_detail; }
 
 sarbp_status sarbp_create(sarbp_context** out, int32_t device) {
   if (!out) return fail(SARBP_ERR_INVALID_ARGUMENT, "ctx out-pointer is NULL");
@@ -444,6 +487,15 @@ sarbp_status sarbp_create(sarbp_context** out, int32_t device) {
   int cu = 0;
   if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cu > 0)
     c->n_cu = cu;
+  {
+    DeviceGuard dg(device);
+    sarbp_status st = dg.err != hipSuccess ? hip_fail(dg.err, "hipSetDevice")
+                                           : scratch_address(device, &c->scratch);
+    if (st != SARBP_OK) {
+      delete c;
+      return st;
+    }
+  }
   *out = c;
   return SARBP_OK;
 }
@@ -581,22 +633,39 @@ sarbp_status sarbp_plan_get_info(const sarbp_plan* pl, sarbp_plan_info* info) {
 
 void sarbp_plan_destroy(sarbp_plan* pl) { delete pl; }
 
-sarbp_status sarbp_backproject_batch(sarbp_context* ctx, const sarbp_job* jobs, int32_t n_jobs,
-                                     void* stream) {
-  if (!ctx || (!jobs && n_jobs > 0) || n_jobs < 0)
-    return fail(SARBP_ERR_INVALID_ARGUMENT, "NULL context or job list");
-  if (n_jobs == 0) return SARBP_OK;
+}  // extern "C"
+
+namespace sarbp {
+
+// Checks a batch's jobs: each valid, all sharing job 0's interp, precision and
+// dtype, device pointers present where there is work.
+sarbp_status validate_batch(const sarbp_job* jobs, int32_t n_jobs) {
   const sarbp_params& p0 = jobs[0].params;
   for (int32_t j = 0; j < n_jobs; ++j) {
     const sarbp_job& jb = jobs[j];
     sarbp_status st = validate(&jb.grid, &jb.params);
-    if (st != SARBP_OK) return fail(st, "job %d: %s", j, g_detail.c_str());
+    if (st != SARBP_OK) return fail(st, "job %d: %s", j, g_detail);
     if (jb.params.interp != p0.interp || jb.params.precision != p0.precision || jb.params.dtype != p0.dtype)
       return fail(SARBP_ERR_INVALID_ARGUMENT, "job %d: interp, precision and dtype must match job 0", j);
     if ((jb.params.n_pulses > 0 && (!jb.d_profiles || !jb.d_pos)) ||
         (!jb.d_image && jb.grid.nx > 0 && jb.grid.ny > 0))
       return fail(SARBP_ERR_INVALID_ARGUMENT, "job %d: device pointer is NULL", j);
   }
+  return SARBP_OK;
+}
+
+}  // namespace sarbp
+
+extern "C" {
+
+sarbp_status sarbp_backproject_batch(sarbp_context* ctx, const sarbp_job* jobs, int32_t n_jobs,
+                                     void* stream) {
+  if (!ctx || (!jobs && n_jobs > 0) || n_jobs < 0)
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "NULL context or job list");
+  if (n_jobs == 0) return SARBP_OK;
+  const sarbp_params& p0 = jobs[0].params;
+  sarbp_status vst = validate_batch(jobs, n_jobs);
+  if (vst != SARBP_OK) return vst;
   std::lock_guard<std::mutex> lk(ctx->mu);
   DeviceGuard dg(ctx->device);
   if (dg.err != hipSuccess) return hip_fail(dg.err, "hipSetDevice");
@@ -623,7 +692,7 @@ sarbp_status sarbp_backproject_batch(sarbp_context* ctx, const sarbp_job* jobs, 
     const sarbp_tuning t = fit_tuning(base, p0.precision, &jb.grid, &jb.params, jb.h_pos);
     JobDesc d;
     sarbp_status st = plan_args(jb.h_pos, &jb.grid, &jb.params, t, &d.a, nullptr);
-    if (st != SARBP_OK) return fail(st, "job %d: %s", j, g_detail.c_str());
+    if (st != SARBP_OK) return fail(st, "job %d: %s", j, g_detail);
     d.a.prof = jb.d_profiles;
     d.a.pos = jb.d_pos;
     d.a.img = reinterpret_cast<float2*>(jb.d_image);
@@ -644,6 +713,116 @@ sarbp_status sarbp_backproject_batch(sarbp_context* ctx, const sarbp_job* jobs, 
     const sarbp_status st = ctx->batch[g]->run(groups[g].desc, groups[g].kinterp, p0.precision, p0.dtype,
                                                groups[g].tune, ctx->n_cu, (hipStream_t)stream);
     if (st != SARBP_OK) return st;
+  }
+  return SARBP_OK;
+}
+
+sarbp_status sarbp_backproject_batch_noalloc(sarbp_context* ctx, const sarbp_job* jobs, int32_t n_jobs,
+                                             void* stream) {
+  if (!ctx || (!jobs && n_jobs > 0) || n_jobs < 0)
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "NULL context or job list");
+  if (n_jobs > kNoAllocJobs)
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "%d jobs: the allocation-free batch takes at most %d", n_jobs,
+                kNoAllocJobs);
+  if (n_jobs == 0) return SARBP_OK;
+  const sarbp_params& p0 = jobs[0].params;
+  sarbp_status st = validate_batch(jobs, n_jobs);
+  if (st != SARBP_OK) return st;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (!ctx->scratch)
+    return fail(SARBP_ERR_UNSUPPORTED, "device %d has no scratch address (devices 0..%d)", ctx->device,
+                kMaxDevices - 1);
+  DeviceGuard dg(ctx->device);
+  if (dg.err != hipSuccess) return hip_fail(dg.err, "hipSetDevice");
+  // The host image of the scratch is shared by every context: one call at a time.
+  std::lock_guard<std::mutex> sk(g_scratch_mu);
+  const sarbp_tuning base = get_tuning(ctx, p0.interp, p0.precision, p0.dtype);
+  // Largest pulse count first, ties in submission order (insertion sort: no
+  // temporary buffer), as sarbp_backproject_batch orders its work list.
+  int32_t order[kNoAllocJobs];
+  for (int32_t j = 0; j < n_jobs; ++j) {
+    int32_t k = j;
+    while (k > 0 && jobs[order[k - 1]].params.n_pulses < jobs[j].params.n_pulses) {
+      order[k] = order[k - 1];
+      --k;
+    }
+    order[k] = j;
+  }
+  // Plan every job with the tile shape its plain launch would use and put it
+  // in the group of its (tile shape, sinc8 kernel).
+  struct Group {
+    sarbp_tuning tune;
+    int kinterp, n;
+    size_t shm;
+    int64_t items;
+  } groups[kNoAllocGroups];
+  int n_groups = 0, n_planned = 0;
+  int8_t group_of[kNoAllocJobs];
+  for (int32_t r = 0; r < n_jobs; ++r) {
+    const sarbp_job& jb = jobs[order[r]];
+    if (jb.grid.nx == 0 || jb.grid.ny == 0) continue;
+    const sarbp_tuning t = fit_tuning(base, p0.precision, &jb.grid, &jb.params, jb.h_pos);
+    JobDesc& d = g_noalloc_plan[n_planned];
+    st = plan_args(jb.h_pos, &jb.grid, &jb.params, t, &d.a, nullptr);
+    if (st != SARBP_OK) return fail(st, "job %d: %s", order[r], g_detail);
+    d.a.prof = jb.d_profiles;
+    d.a.pos = jb.d_pos;
+    d.a.img = reinterpret_cast<float2*>(jb.d_image);
+    int g = 0;
+    while (g < n_groups && !(groups[g].tune.cols_per_lane == t.cols_per_lane &&
+                             groups[g].tune.rows_per_lane == t.rows_per_lane &&
+                             groups[g].kinterp == d.a.kernel_interp))
+      ++g;
+    if (g == n_groups) {
+      if (n_groups == kNoAllocGroups)
+        return fail(SARBP_ERR_UNSUPPORTED, "more than %d tile shapes in one batch", kNoAllocGroups);
+      groups[n_groups++] = Group{t, d.a.kernel_interp, 0, 0, 0};
+    }
+    Group& G = groups[g];
+    d.item_begin = G.items;
+    G.items += (int64_t)d.a.tiles_x * d.a.tiles_y;
+    G.shm = std::max(G.shm, lds_layout(G.kinterp, d.a.win, d.a.pulses_per_stage).total);
+    ++G.n;
+    group_of[n_planned++] = (int8_t)g;
+  }
+  if (n_planned == 0) return SARBP_OK;
+  // Host image of the scratch: headers (counters zero), then each group's
+  // table in planning order.  One copy on the stream writes all of it, so
+  // the counters start at zero for this call's launches.
+  std::memset(g_noalloc_host, 0, kNoAllocGroups * kNoAllocSlot);
+  size_t table_off[kNoAllocGroups];
+  size_t off = kNoAllocGroups * kNoAllocSlot;
+  for (int g = 0; g < n_groups; ++g) {
+    BatchHeader h;
+    h.n_items = groups[g].items;
+    h.n_jobs = groups[g].n;
+    h.reserved = 0;
+    std::memcpy(g_noalloc_host + g * kNoAllocSlot, &h, sizeof h);
+    table_off[g] = off;
+    off += (size_t)groups[g].n * sizeof(JobDesc);
+  }
+  int filled[kNoAllocGroups] = {};
+  for (int i = 0; i < n_planned; ++i) {
+    const int g = group_of[i];
+    std::memcpy(g_noalloc_host + table_off[g] + (size_t)filled[g]++ * sizeof(JobDesc), &g_noalloc_plan[i],
+                sizeof(JobDesc));
+  }
+  hipStream_t s = (hipStream_t)stream;
+  char* dev = static_cast<char*>(ctx->scratch);
+  // Pageable source: the call returns once the bytes are staged, so the host
+  // image may be rewritten by the next call.
+  hipError_t e = hipMemcpyAsync(dev, g_noalloc_host, off, hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(scratch)");
+  for (int g = 0; g < n_groups; ++g) {
+    const Group& G = groups[g];
+    const int per_cu = batch_blocks_per_cu(G.kinterp, p0.precision, p0.dtype, G.tune.cols_per_lane,
+                                           G.tune.rows_per_lane, G.shm);
+    e = launch_batch(reinterpret_cast<const JobDesc*>(dev + table_off[g]),
+                     reinterpret_cast<const BatchHeader*>(dev + g * kNoAllocSlot),
+                     reinterpret_cast<unsigned long long*>(dev + g * kNoAllocSlot + kNoAllocCounter),
+                     G.kinterp, p0.precision, p0.dtype, G.tune.cols_per_lane, G.tune.rows_per_lane, G.shm,
+                     std::max(1, ctx->n_cu * per_cu), s);
+    if (e != hipSuccess) return hip_fail(e, "allocation-free batched launch");
   }
   return SARBP_OK;
 }

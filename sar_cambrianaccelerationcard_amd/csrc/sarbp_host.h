@@ -22,7 +22,7 @@ constexpr size_t kLdsBudget = 64 * 1024;  // dynamic LDS per workgroup
 // Records a detail message for sarbp_last_error_detail and returns st.
 sarbp_status fail(sarbp_status st, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 sarbp_status hip_fail(hipError_t e, const char* what);
-const std::string& last_detail();
+const char* last_detail();
 
 // Makes `dev` current for the scope and restores the caller's device after.
 struct DeviceGuard {
@@ -112,6 +112,8 @@ struct sarbp_context {
   // Graph-replayed batched launches, one executor per tile shape of a call
   // (jobs whose grids need different shapes run as separate launches).
   sarbp::BatchExec* batch[8] = {};
+  void* scratch = nullptr;               // this device's address of the
+                                         // allocation-free batch's scratch
   sarbp::RangeCache* range = nullptr;    // range-compression tables
   sarbp::WkCache* wk = nullptr;          // omega-K transform plans
 };

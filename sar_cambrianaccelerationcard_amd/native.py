@@ -178,6 +178,7 @@ _SIGS = {
     "sarbp_set_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(Tuning)]),
     "sarbp_get_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(Tuning)]),
     "sarbp_backproject_batch": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_int32, C.c_void_p]),
+    "sarbp_backproject_batch_noalloc": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_int32, C.c_void_p]),
     "sarbp_sinc8_coefficients": (C.c_int, [C.c_double, C.POINTER(C.c_float)]),
     "sarbp_sinc8_coefficients_whole": (C.c_int, [C.c_double, C.POINTER(C.c_float)]),
     "sarbp_launch_backprojection": (C.c_int, [C.c_void_p, C.c_void_p, SarParams, C.c_int32,
@@ -449,6 +450,12 @@ class Context:
         arr = (Job * len(jobs))(*jobs)
         check(self.lib.sarbp_backproject_batch(self.handle, arr, len(jobs), C.c_void_p(stream)),
               "sarbp_backproject_batch")
+
+    def backproject_batch_noalloc(self, jobs: Sequence[Job], stream: int = 0) -> None:
+        """sarbp_backproject_batch_noalloc: the same batch without allocating."""
+        arr = (Job * max(len(jobs), 1))(*jobs)
+        check(self.lib.sarbp_backproject_batch_noalloc(self.handle, arr, len(jobs), C.c_void_p(stream)),
+              "sarbp_backproject_batch_noalloc")
 
 
 def sinc8_coefficients(beta: float):
