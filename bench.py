@@ -324,46 +324,83 @@ def leg_cpu(sc, gpu_img, interp, args):
     return cpu, acc
 
 
-def plain_e2e(torch, ctx, native, W, inputs) -> dict:
+class ServedInputs:
+    """A served stream: the jobs, shared seeded noise profiles for the bulk,
+    simulated scenes for the spot-checked jobs, FP64 positions and host image
+    buffers (one shared buffer per size class for the bulk, whose images are
+    not inspected; an own buffer for each spot-checked job)."""
+
+    def __init__(self, W, args):
+        self.jobs = W.mixed_jobs(args.served_jobs, seed=20260041, large_fraction=0.1)
+        jobs = self.jobs
+        small_max = max((j.n_pulses * j.n_samples for j in jobs if not j.large), default=1)
+        large_max = max((j.n_pulses * j.n_samples for j in jobs if j.large), default=0)
+        t0 = time.time()
+        self.small_prof = W.random_profiles(small_max, 41)
+        self.large_prof = W.random_profiles(large_max, 42) if large_max else None
+        self.pos = [j.positions() for j in jobs]
+        self.small_img = np.empty(max((j.side ** 2 for j in jobs if not j.large), default=1), np.complex64)
+        self.large_img = np.empty(max((j.side ** 2 for j in jobs if j.large), default=1), np.complex64)
+        # spot-checked jobs: the first large job and the first two small ones
+        self.sims = {}
+        if args.cpu_rows > 0:
+            larges = [k for k, j in enumerate(jobs) if j.large][:1]
+            smalls = [k for k, j in enumerate(jobs) if not j.large][:2]
+            for k in larges + smalls:
+                j = jobs[k]
+                self.sims[k] = j.simulate(n_targets=12 if j.large else 6, n_clutter=4096 if j.large else 512)
+        self.images = {}
+        log(f"served: {len(jobs)} jobs ({sum(j.large for j in jobs)} large, "
+            f"{sum(j.work for j in jobs):.3e} W), host inputs and {len(self.sims)} simulated scenes in "
+            f"{time.time() - t0:.1f}s")
+
+    def data(self, k):
+        """(profiles, positions, grid, params, host image) of job k."""
+        j = self.jobs[k]
+        if k in self.sims:
+            sc = self.sims[k]
+            img = self.images.setdefault(k, np.empty((j.side, j.side), np.complex64))
+            return sc.profiles, sc.pos, sc.native_grid(), sc.native_params(interp="sinc8"), img
+        src = self.large_prof if j.large else self.small_prof
+        buf = self.large_img if j.large else self.small_img
+        return (src[: j.n_pulses * j.n_samples], self.pos[k], j.grid(), j.params(),
+                buf[: j.side * j.side].reshape(j.side, j.side))
+
+
+def plain_e2e(torch, ctx, native, inp, n_sample: int) -> dict:
     """The reference host flow per image (SAR_Processor, BPA_CUDA.cpp:51-89):
-    copy the job's profiles and positions to the device, back-project, copy
-    the image back, synchronise -- one job after another."""
-    jobs, small_prof, large_prof = inputs
+    copy the job's profiles and positions to the device, back-project with the
+    unplanned call, copy the image back, synchronise -- one job after another,
+    on a sample of the stream that keeps its share of large jobs (the first
+    ones of each class in stream order)."""
+    jobs = inp.jobs
+    n_large = max(1, round(0.1 * n_sample)) if any(j.large for j in jobs) else 0
+    ks = [k for k, j in enumerate(jobs) if j.large][:n_large] + \
+         [k for k, j in enumerate(jobs) if not j.large][:n_sample - n_large]
+    ks.sort()
     dev = torch.device("cuda", 0)
-    max_prof = max(j.n_pulses * j.n_samples for j in jobs)
-    max_pix = max(j.side * j.side for j in jobs)
+    max_prof = max(jobs[k].n_pulses * jobs[k].n_samples for k in ks)
+    max_pix = max(jobs[k].side ** 2 for k in ks)
     d_prof = torch.empty(max_prof, dtype=torch.complex64, device=dev)
     d_img = torch.empty(max_pix, dtype=torch.complex64, device=dev)
-    poss = [j.positions() for j in jobs]
+    stream = torch.cuda.current_stream().cuda_stream
     t0 = time.perf_counter()
-    for j, pos in zip(jobs, poss):
-        src = large_prof if j.large else small_prof
-        n = j.n_pulses * j.n_samples
-        d_prof[:n].copy_(torch.from_numpy(src[:n]))
+    for k in ks:
+        prof, pos, grid, params, _ = inp.data(k)
+        j = jobs[k]
+        d_prof[: j.n_pulses * j.n_samples].copy_(torch.from_numpy(np.ascontiguousarray(prof).reshape(-1)))
         d_pos = torch.from_numpy(pos).to(dev)
-        ctx.backproject(d_prof.data_ptr(), d_pos.data_ptr(), j.grid(), j.params(), d_img.data_ptr(),
-                        torch.cuda.current_stream().cuda_stream)
+        ctx.backproject(d_prof.data_ptr(), d_pos.data_ptr(), grid, params, d_img.data_ptr(), stream)
         out = d_img[: j.side * j.side].cpu()
         del out
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    work = sum(j.work for j in jobs)
+    work = sum(jobs[k].work for k in ks)
     del d_prof, d_img
     torch.cuda.empty_cache()
-    log(f"plain end-to-end, one job after another: {wall:.2f}s, {work / wall:.3e} W/s")
-    return {"W_per_s": work / wall, "wall_s": wall, "n_jobs": len(jobs), "work": work}
-
-
-def served_inputs(W, args):
-    jobs = W.mixed_jobs(args.served_jobs, seed=20260041, large_fraction=0.1)
-    small_max = max((j.n_pulses * j.n_samples for j in jobs if not j.large), default=1)
-    large_max = max((j.n_pulses * j.n_samples for j in jobs if j.large), default=0)
-    t0 = time.time()
-    small_prof = W.random_profiles(small_max, 41)
-    large_prof = W.random_profiles(large_max, 42) if large_max else None
-    log(f"served: {len(jobs)} jobs ({sum(j.large for j in jobs)} large), host profiles in "
-        f"{time.time() - t0:.1f}s")
-    return jobs, small_prof, large_prof
+    log(f"plain end-to-end, one job after another ({len(ks)} jobs, {n_large} large): {wall:.2f}s, "
+        f"{work / wall:.3e} W/s")
+    return {"W_per_s": work / wall, "wall_s": wall, "n_jobs": len(ks), "n_large": n_large, "work": work}
 
 
 def served_raw_inputs(W, jobs):
@@ -373,53 +410,85 @@ def served_raw_inputs(W, jobs):
     return W.random_profiles(small_max, 43), (W.random_profiles(large_max, 44) if large_max else None)
 
 
-def leg_served(native, W, args, rate: float, inputs, raw_inputs=None) -> dict:
-    """A Poisson stream (rate jobs/s; 0 = all at once) through the server.  With
-    raw_inputs the jobs are submitted as raw echoes and range-compressed on
-    the device at admission."""
+def leg_served(native, W, args, rate: float, inp, n_jobs: int, raw_inputs=None) -> dict:
+    """The first n_jobs of the stream as Poisson arrivals (rate jobs/s; 0 =
+    all at once) through the server.  With raw_inputs the jobs are submitted
+    as raw echoes and range-compressed on the device at admission."""
     from sar_cambrianaccelerationcard_amd import device as D
-    jobs, small_prof, large_prof = inputs
+    jobs = inp.jobs[:n_jobs]
     srv = native.Server(interp="sinc8", hbm_budget_bytes=args.hbm_budget,
                         pulses_per_epoch=args.pulses_per_epoch)
     rng = np.random.default_rng(7)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, len(jobs))) if rate > 0 else np.zeros(len(jobs))
-    poss = [j.positions() for j in jobs]
+    tag = f"served{' raw' if raw_inputs is not None else ''} @ {rate if rate > 0 else 'burst'}/s"
     ids = []
-    images = []
+    step = max(1, len(jobs) // 5)
     start = time.perf_counter()
-    for j, at, pos in zip(jobs, arrivals, poss):
+    for k, (j, at) in enumerate(zip(jobs, arrivals)):
         now = time.perf_counter() - start
         if at > now:
             time.sleep(at - now)
-        img = np.empty((j.side, j.side), dtype=np.complex64)
+        prof, pos, grid, params, img = inp.data(k)
         if raw_inputs is None:
-            src = large_prof if j.large else small_prof
-            ids.append(srv.submit(src[: j.n_pulses * j.n_samples], pos, j.grid(), j.params(), img))
+            ids.append(srv.submit(prof, pos, grid, params, img))
         else:
             src = raw_inputs[1] if j.large else raw_inputs[0]
             ids.append(srv.submit_raw(src[: j.n_pulses * j.n_raw], D.range_params(j.radar()), pos,
-                                      j.grid(), j.params(), img))
-        images.append(img)
-    infos = [srv.wait(i, timeout_s=600) for i in ids]
+                                      grid, params, img))
+        if (k + 1) % step == 0 and k + 1 < len(jobs):
+            log(f"{tag}: {k + 1}/{len(jobs)} submitted at {time.perf_counter() - start:.1f}s")
+    infos = []
+    for k, i in enumerate(ids):
+        infos.append(srv.wait(i, timeout_s=900))
+        if (k + 1) % step == 0:
+            log(f"{tag}: {k + 1}/{len(jobs)} done at {time.perf_counter() - start:.1f}s")
     st = srv.stats()
     srv.close()
     lat = np.array([i["done_s"] - i["submit_s"] for i in infos])
     lat_small = np.array([i["done_s"] - i["submit_s"] for i, j in zip(infos, jobs) if not j.large])
+    lat_large = np.array([i["done_s"] - i["submit_s"] for i, j in zip(infos, jobs) if j.large])
     span = max(i["done_s"] for i in infos) - min(i["submit_s"] for i in infos)
     work = sum(j.work for j in jobs)
+    pct = lambda a, q: float(np.percentile(a, q)) if a.size else None
     out = {"rate_jobs_per_s": rate, "n_jobs": len(jobs), "n_large": int(sum(j.large for j in jobs)),
-           "work": work, "W_per_s": work / span, "W_per_s_gpu_busy": work / max(st["gpu_busy_s"], 1e-9),
-           "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
-           "p50_latency_small_s": float(np.percentile(lat_small, 50)) if lat_small.size else None,
-           "p99_latency_small_s": float(np.percentile(lat_small, 99)) if lat_small.size else None,
+           "work": work, "span_s": span, "W_per_s": work / span,
+           "W_per_s_gpu_busy": work / max(st["gpu_busy_s"], 1e-9),
+           "p50_latency_s": pct(lat, 50), "p99_latency_s": pct(lat, 99),
+           "p50_latency_small_s": pct(lat_small, 50), "p99_latency_small_s": pct(lat_small, 99),
+           "p50_latency_large_s": pct(lat_large, 50), "p99_latency_large_s": pct(lat_large, 99),
            "epochs": st["epochs"], "hbm_peak_bytes": st["hbm_peak"], "failed": st["failed"],
            "pulses_per_epoch": args.pulses_per_epoch, "hbm_budget_bytes": args.hbm_budget,
            "input": "raw echoes" if raw_inputs is not None else "profiles",
            "h2d_input_bytes": int(sum(j.n_pulses * (j.n_raw if raw_inputs is not None else j.n_samples) * 8
                                       for j in jobs))}
-    log(f"served{' raw' if raw_inputs is not None else ''} @ {rate if rate > 0 else 'burst'}/s: {out['W_per_s']:.3e} W/s over {span:.1f}s, p50 {out['p50_latency_s']:.3f}s "
-        f"p99 {out['p99_latency_s']:.3f}s, {st['epochs']} epochs, failed {st['failed']}")
-    out["_images"] = (jobs, poss, images)
+    log(f"{tag}: {out['W_per_s']:.3e} W/s over {span:.1f}s (GPU-busy {out['W_per_s_gpu_busy']:.3e}), "
+        f"p50 {out['p50_latency_s']:.3f}s p99 {out['p99_latency_s']:.3f}s, {st['epochs']} epochs, "
+        f"failed {st['failed']}")
+    return out
+
+
+def served_spot_checks(inp, args) -> list:
+    """Oracle spot checks of the served jobs that carry simulated scenes: whole
+    rows plus 64 x 64 chips centred on point targets (2 rows + 2 chips for a
+    large job, 1 + 1 for a small one)."""
+    import oracle.oracle as O
+    import oracle.spot as SP
+    out = []
+    for k, sc in sorted(inp.sims.items()):
+        j = inp.jobs[k]
+        img = inp.images[k]
+        p = O.params(sc.c, sc.fc, sc.r0, sc.dr, "sinc8", sc.kaiser_beta)
+        tp = sc.target_pixels
+        n = 2 if j.large else 1
+        rows = [int(tp[0][1]), sc.grid["ny"] // 2][:n]
+        chips = [tuple(int(v) for v in t) for t in tp[:n]]
+        t0 = time.perf_counter()
+        r = SP.spot_check(sc.profiles, sc.pos, sc.grid, p, img, rows, chips)
+        r.update({"job": k, "large": j.large, "side": j.side, "n_pulses": j.n_pulses,
+                  "n_samples": j.n_samples, "spacing_m": j.spacing, "oracle_s": time.perf_counter() - t0})
+        out.append(r)
+        log(f"served spot check job {k} ({'large' if j.large else 'small'} {j.side}^2 x {j.n_pulses} "
+            f"pulses): rel-L2 {r['rel_l2']:.2e}, peaks {'ok' if all(r['chip_peak_match']) else 'MISMATCH'}")
     return out
 
 
@@ -444,8 +513,8 @@ def valu_probe(args) -> int:
              20: "64 fma + 64 or_b32", 21: "64 fma + 64 min_u32", 22: "64 fma + 64 cndmask_b32",
              23: "64 fma + 64 lshl_add_u32", 24: "64 fma + 64 add3_u32", 25: "64 fma + 64 cvt_i32_f32",
              26: "64 fma + 64 max_f32", 27: "64 fma + 64 floor_f32", 28: "64 fma + 64 sub_u32",
-             29: "64 fma + 64 bfe_u32"}
-    for kind in range(30):
+             29: "64 fma + 64 bfe_u32", 30: "64 fma + 64 ldexp_f32", 31: "64 fma + 64 ashrrev_i32"}
+    for kind in range(32):
         fn = lambda: native.valu_probe(kind, n_blocks, iters, out.data_ptr(), stream.cuda_stream)
         fn()
         t = timed(torch, stream, fn, 5)
@@ -479,13 +548,16 @@ def main() -> int:
                     help="only run the omega-K leg and print its JSON (profiling)")
     ap.add_argument("--mixed-jobs", type=int, default=64, help="batched leg jobs (0: skip)")
     ap.add_argument("--mixed-reps", type=int, default=2)
-    ap.add_argument("--served-jobs", type=int, default=96, help="served leg jobs (0: skip)")
-    ap.add_argument("--served-rates", default="6,12,24,0",
-                    help="Poisson arrival rates (jobs/s) to sweep, comma separated; 0 = every "
-                         "job submitted at once")
+    ap.add_argument("--served-jobs", type=int, default=2000, help="served leg jobs (0: skip)")
+    ap.add_argument("--served-rates", default="16",
+                    help="Poisson arrival rates (jobs/s) of the served stream, comma separated; "
+                         "0 = every job submitted at once")
+    ap.add_argument("--e2e-jobs", type=int, default=100,
+                    help="jobs of the stream run through the reference host flow (baseline)")
     ap.add_argument("--hbm-budget", type=int, default=200 * 10**9)
-    ap.add_argument("--served-raw", type=int, default=1,
-                    help="also serve the stream as raw echoes (burst), range-compressed on the device")
+    ap.add_argument("--served-raw-jobs", type=int, default=96,
+                    help="also serve the first jobs of the stream as raw echoes (burst), "
+                         "range-compressed on the device (0: skip)")
     ap.add_argument("--pulses-per-epoch", type=int, default=1024)
     ap.add_argument("--valu-probe", action="store_true",
                     help="only run the vector-ALU issue-rate probes (roofline calibration)")
@@ -578,42 +650,30 @@ def main() -> int:
     batched = leg_mixed(torch, ctx, native, W, args) if args.mixed_jobs > 0 else None
     served = None
     if args.served_jobs > 0:
-        inputs = served_inputs(W, args)
-        small_prof = inputs[1]
-        runs = []
-        for rate in [float(r) for r in args.served_rates.split(",") if r]:
-            runs.append(leg_served(native, W, args, rate, inputs))
-            jobs, poss, images = runs[-1].pop("_images")
-        served = {"runs": runs, "peak_W_per_s": max(r["W_per_s"] for r in runs),
-                  "p50_latency_s_at_lowest_rate": runs[0]["p50_latency_s"],
-                  "p99_latency_s_at_lowest_rate": runs[0]["p99_latency_s"]}
-        if args.served_raw:
-            raw_inputs = served_raw_inputs(W, jobs)
-            rr = leg_served(native, W, args, 0.0, inputs, raw_inputs)
-            rr.pop("_images")
-            served["raw_echo_burst"] = rr
+        inp = ServedInputs(W, args)
+        runs = [leg_served(native, W, args, float(r), inp, len(inp.jobs))
+                for r in args.served_rates.split(",") if r]
+        best = max(runs, key=lambda r: r["W_per_s"])
+        served = {"workload": f"{len(inp.jobs)} Poisson jobs (10% large: 8192^2 image, 8192 pulses x 16384 "
+                              f"samples; others sized as the batched mix), 200 GB HBM budget, sinc8, REF",
+                  "runs": runs, "peak_W_per_s": best["W_per_s"],
+                  "p50_latency_s": runs[0]["p50_latency_s"], "p99_latency_s": runs[0]["p99_latency_s"]}
+        if args.served_raw_jobs > 0:
+            n_raw = min(args.served_raw_jobs, len(inp.jobs))
+            raw_inputs = served_raw_inputs(W, inp.jobs[:n_raw])
+            served["raw_echo_burst"] = leg_served(native, W, args, 0.0, inp, n_raw, raw_inputs)
             del raw_inputs
-        e2e = plain_e2e(torch, ctx, native, W, inputs)
+        e2e = plain_e2e(torch, ctx, native, inp, min(args.e2e_jobs, len(inp.jobs)))
         served["plain_end_to_end"] = e2e
         served["peak_speedup_vs_plain_end_to_end"] = served["peak_W_per_s"] / e2e["W_per_s"]
         if batched:
             served["peak_speedup_vs_plain_sequential"] = served["peak_W_per_s"] / batched["plain_sequential_W_per_s"]
-        if args.cpu_rows > 0:
-            import oracle.oracle as O
-            errs = []
-            for k in [i for i, j in enumerate(jobs) if not j.large][:2]:
-                j = jobs[k]
-                prof = small_prof[: j.n_pulses * j.n_samples].reshape(j.n_pulses, j.n_samples)
-                gp = j.grid()
-                grid = {"origin": list(gp.origin), "ux": list(gp.ux), "uy": list(gp.uy),
-                        "nx": gp.nx, "ny": gp.ny}
-                pa = j.params()
-                r = j.side // 2
-                ref = O.backproject(prof, poss[k], grid, O.params(pa.c, pa.fc, pa.r0, pa.dr, "sinc8", 6.0),
-                                    rows=(r, r + 1))
-                errs.append(M.rel_l2(images[k][r:r + 1], ref))
-            served["spot_check_rel_l2"] = errs
-            log(f"served spot checks rel-L2 {errs}")
+        if inp.sims:
+            checks = served_spot_checks(inp, args)
+            served["spot_checks"] = checks
+            served["spot_check_rel_l2"] = [c["rel_l2"] for c in checks]
+            served["spot_check_max_rel_l2"] = max(c["rel_l2"] for c in checks)
+        del inp
 
     fw = F_W[interp]
     achieved = wps * fw / 1e12

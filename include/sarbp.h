@@ -640,7 +640,8 @@ sarbp_status sarbp_sinc8_coefficients_whole(double beta, float* out);
  * interleaved with 64 of v_add_u32, v_mad_u32_u24, v_cvt_f32_i32, v_fract_f32,
  * v_med3_f32, v_mul_f32, v_mov_b32, v_add_f32, v_lshrrev_b32, v_and_b32,
  * v_or_b32, v_min_u32, v_cndmask_b32, v_lshl_add_u32, v_add3_u32,
- * v_cvt_i32_f32, v_max_f32, v_floor_f32, v_sub_u32, v_bfe_u32 respectively.
+ * v_cvt_i32_f32, v_max_f32, v_floor_f32, v_sub_u32, v_bfe_u32, v_ldexp_f32,
+ * v_ashrrev_i32 respectively.
  * Asynchronous on `stream`; the caller times it. */
 sarbp_status sarbp_valu_probe(int32_t kind, int32_t n_blocks, int32_t iters,
                               float* d_out, void* stream);

@@ -10,7 +10,10 @@ from __future__ import annotations
 
 import numpy as np
 
-import oracle as O
+try:  # imported as oracle.spot (bench.py)
+    from . import oracle as O
+except ImportError:  # oracle/ on sys.path (tests)
+    import oracle as O
 
 
 def rel_l2(img, ref) -> float:

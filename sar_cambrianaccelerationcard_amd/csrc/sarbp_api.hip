@@ -201,7 +201,7 @@ sarbp_tuning default_tuning(int interp) {
   sarbp_tuning t;
   t.cols_per_lane = 1;
   t.rows_per_lane = interp == SARBP_INTERP_SINC8 ? 8 : 4;
-  t.pulses_per_stage = interp == SARBP_INTERP_SINC8 ? 6 : 32;
+  t.pulses_per_stage = interp == SARBP_INTERP_SINC8 ? 4 : 32;
   return t;
 }
 

@@ -27,6 +27,7 @@
 //  24  v_add3_u32 (VOP3)                 25  v_cvt_i32_f32 (VOP1)
 //  26  v_max_f32 (VOP2)                  27  v_floor_f32 (VOP1)
 //  28  v_sub_u32 (VOP2)                  29  v_bfe_u32 (VOP3)
+//  30  v_ldexp_f32 (VOP2)                31  v_ashrrev_i32 (VOP2)
 // The BP pixel loop issues 2 transcendentals per 28.4 VALU instructions, between
 // kinds 2 and 3, and mixes VOP3 and VOP1/VOP2 encodings.
 #include "sarbp.h"
@@ -110,8 +111,12 @@ __global__ __launch_bounds__(kProbeThreads) void valu_probe_kernel(float* __rest
           asm volatile("v_floor_f32 %0, %1" : "=v"(x[k]) : "v"(y[k]));
         } else if constexpr (KIND == 28) {
           asm volatile("v_sub_u32 %0, %0, %1" : "+v"(iv[k - 4]) : "v"(ic));
-        } else {
+        } else if constexpr (KIND == 29) {
           asm volatile("v_bfe_u32 %0, %0, 1, 20" : "+v"(iv[k - 4]));
+        } else if constexpr (KIND == 30) {
+          asm volatile("v_ldexp_f32 %0, %0, %1" : "+v"(x[k]) : "v"(ic));
+        } else {
+          asm volatile("v_ashrrev_i32 %0, 1, %0" : "+v"(iv[k - 4]));
         }
       }
     }
@@ -131,7 +136,7 @@ using namespace sarbp;
 
 extern "C" sarbp_status sarbp_valu_probe(int32_t kind, int32_t n_blocks, int32_t iters, float* d_out,
                                          void* stream) {
-  if (kind < 0 || kind > 29) return fail(SARBP_ERR_INVALID_ARGUMENT, "probe kind %d", kind);
+  if (kind < 0 || kind > 31) return fail(SARBP_ERR_INVALID_ARGUMENT, "probe kind %d", kind);
   if (n_blocks < 1 || iters < 1) return fail(SARBP_ERR_INVALID_ARGUMENT, "n_blocks and iters must be >= 1");
   if (!d_out) return fail(SARBP_ERR_INVALID_ARGUMENT, "d_out is NULL");
   hipStream_t s = (hipStream_t)stream;
@@ -145,6 +150,7 @@ extern "C" sarbp_status sarbp_valu_probe(int32_t kind, int32_t n_blocks, int32_t
     SARBP_PROBE(12) SARBP_PROBE(13) SARBP_PROBE(14) SARBP_PROBE(15) SARBP_PROBE(16) SARBP_PROBE(17)
     SARBP_PROBE(18) SARBP_PROBE(19) SARBP_PROBE(20) SARBP_PROBE(21) SARBP_PROBE(22) SARBP_PROBE(23)
     SARBP_PROBE(24) SARBP_PROBE(25) SARBP_PROBE(26) SARBP_PROBE(27) SARBP_PROBE(28) SARBP_PROBE(29)
+    SARBP_PROBE(30) SARBP_PROBE(31)
   }
 #undef SARBP_PROBE
   const hipError_t e = hipGetLastError();
