@@ -177,7 +177,7 @@ def leg_omega_k(torch, ctx, native, args) -> dict:
     return res
 
 
-def leg_mixed(torch, ctx, native, W, args) -> dict:
+def leg_mixed(torch, ctx, native, W, args, tuned_classes=()) -> dict:
     jobs = W.mixed_jobs(args.mixed_jobs)
     dev = torch.device("cuda", 0)
     max_prof = max(j.n_pulses * j.n_samples for j in jobs)
@@ -188,10 +188,52 @@ def leg_mixed(torch, ctx, native, W, args) -> dict:
     imgs = [torch.empty((j.side, j.side), dtype=torch.complex64, device=dev) for j in jobs]
     params = [j.params() for j in jobs]
     grids = [j.grid() for j in jobs]
-    plans = [ctx.plan(p, gr, pa) for p, gr, pa in zip(poss, grids, params)]
-    work = sum(j.work for j in jobs)
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
+    # autotune every size class of the mix on its largest job (the classes'
+    # tunings are kept by the context and used by plans, batches and servers)
+    classes = {}
+    for i in range(len(jobs)):
+        classes.setdefault(native.size_class(grids[i], params[i], poss[i]), []).append(i)
+    tuned = {}
+    t_tune = time.time()
+    shape = None
+    if args.class_shapes == "common":
+        # one tile shape for every class of the mix (a batch then needs one
+        # persistent launch per sinc8 variant): the candidate with the least
+        # work-weighted time over the classes' largest jobs
+        reps = {c: max(idx, key=lambda i: jobs[i].work) for c, idx in classes.items() if c not in tuned_classes}
+        best_total = None
+        for cand in ((1, 4), (1, 8)):
+            total = 0.0
+            for c, k in reps.items():
+                r = tune_stages(torch, ctx, native, stream, cand, c, prof, dpos[k], poss[k], grids[k], params[k],
+                                imgs[k], keep=False)
+                total += r["best_ms"] * sum(jobs[i].work for i in classes[c]) / jobs[k].work
+            log(f"common tile shape {cand[0]}x{cand[1]}: {total:.1f} ms work-weighted")
+            if best_total is None or total < best_total[0]:
+                best_total = (total, cand)
+        shape = best_total[1]
+    for c, idx in sorted(classes.items()):
+        if c in tuned_classes:  # tuned by the headline leg on a whole image
+            tuned[c] = {"jobs": len(idx), "work": int(sum(jobs[i].work for i in idx)), "tuned_on": "headline"}
+            continue
+        k = max(idx, key=lambda i: jobs[i].work)
+        if args.class_shapes == "free":
+            r = ctx.autotune(prof.data_ptr(), dpos[k].data_ptr(), poss[k], grids[k], params[k], imgs[k].data_ptr(),
+                             jobs[k].side ** 2, args.class_crop_px, sp)
+        else:
+            r = tune_stages(torch, ctx, native, stream, shape, c, prof, dpos[k], poss[k], grids[k], params[k],
+                            imgs[k])
+        tuned[c] = {"jobs": len(idx), "work": int(sum(jobs[i].work for i in idx)), "tuned_on": k,
+                    "spacing_m": jobs[k].spacing, "tile": [r["cols_per_lane"], r["rows_per_lane"]],
+                    "pulses_per_stage": r["pulses_per_stage"], "best_ms": r["best_ms"],
+                    "default_ms": r["default_ms"]}
+        log(f"class {c} ({len(idx)} jobs): {r['cols_per_lane']}x{r['rows_per_lane']} P{r['pulses_per_stage']}, "
+            f"{r['best_ms']:.2f} ms vs built-in {r['default_ms']:.2f} ms on job {k}")
+    log(f"per-class autotuning: {time.time() - t_tune:.1f}s")
+    plans = [ctx.plan(p, gr, pa) for p, gr, pa in zip(poss, grids, params)]
+    work = sum(j.work for j in jobs)
 
     def plain_all():
         for pl, dp, im in zip(plans, dpos, imgs):
@@ -228,7 +270,37 @@ def leg_mixed(torch, ctx, native, W, args) -> dict:
             "n_jobs": len(jobs), "work": work,
             "plain_sequential_W_per_s": work / t_plain, "batched_W_per_s": work / t_batch,
             "speedup_vs_plain": t_plain / t_batch, "plain_ms": t_plain * 1e3,
-            "batched_ms": t_batch * 1e3, "batched_equals_plain_bitwise": bool(same)}
+            "batched_ms": t_batch * 1e3, "batched_equals_plain_bitwise": bool(same),
+            "size_classes": tuned}
+
+
+def tune_stages(torch, ctx, native, stream, shape, cls, prof, dpos, pos, grid, params, img, keep=True) -> dict:
+    """Pulses per stage for one size class at a fixed tile shape (kept as the
+    class's tuning unless keep is False): jobs of all classes then share a
+    shape, so a batch or a server epoch needs one persistent launch per sinc8
+    variant rather than one per shape."""
+    I, Pm, Dt = native.INTERP_SINC8, native.PREC_REF, native.DTYPE_C64
+    default = ctx.get_class_tuning(I, Pm, Dt, cls)
+    best = None
+    for P in (1, 2, 3, 4, 6, 8):
+        ctx.set_class_tuning(I, Pm, Dt, cls, shape[1], shape[0], P)
+        pl = ctx.plan(pos, grid, params)
+        if pl.info()["pulses_per_stage"] != P:
+            continue  # the LDS budget cut it: the same as a smaller P
+        fn = lambda: pl.execute(prof.data_ptr(), dpos.data_ptr(), img.data_ptr(), stream.cuda_stream)
+        fn()
+        t = timed(torch, stream, fn, 2)
+        if best is None or t < best[0]:
+            best = (t, P)
+    ctx.set_class_tuning(I, Pm, Dt, cls, *default)
+    pl = ctx.plan(pos, grid, params)
+    fn = lambda: pl.execute(prof.data_ptr(), dpos.data_ptr(), img.data_ptr(), stream.cuda_stream)
+    fn()
+    t_def = timed(torch, stream, fn, 2)
+    if keep:
+        ctx.set_class_tuning(I, Pm, Dt, cls, shape[1], shape[0], best[1])
+    return {"cols_per_lane": shape[0], "rows_per_lane": shape[1], "pulses_per_stage": best[1],
+            "best_ms": best[0] * 1e3, "default_ms": t_def * 1e3}
 
 
 def cpu_model() -> str:
@@ -410,7 +482,7 @@ def served_raw_inputs(W, jobs):
     return W.random_profiles(small_max, 43), (W.random_profiles(large_max, 44) if large_max else None)
 
 
-def leg_served(native, W, args, rate: float, inp, n_jobs: int, raw_inputs=None) -> dict:
+def leg_served(native, W, args, rate: float, inp, n_jobs: int, raw_inputs=None, ctx=None) -> dict:
     """The first n_jobs of the stream as Poisson arrivals (rate jobs/s; 0 =
     all at once) through the server.  With raw_inputs the jobs are submitted
     as raw echoes and range-compressed on the device at admission."""
@@ -418,6 +490,8 @@ def leg_served(native, W, args, rate: float, inp, n_jobs: int, raw_inputs=None) 
     jobs = inp.jobs[:n_jobs]
     srv = native.Server(interp="sinc8", hbm_budget_bytes=args.hbm_budget,
                         pulses_per_epoch=args.pulses_per_epoch)
+    if ctx is not None:
+        srv.tune_like(ctx)  # the context's per-class (autotuned) launch configurations
     rng = np.random.default_rng(7)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, len(jobs))) if rate > 0 else np.zeros(len(jobs))
     tag = f"served{' raw' if raw_inputs is not None else ''} @ {rate if rate > 0 else 'burst'}/s"
@@ -467,7 +541,7 @@ def leg_served(native, W, args, rate: float, inp, n_jobs: int, raw_inputs=None) 
     return out
 
 
-def served_spot_checks(inp, args) -> list:
+def served_spot_checks(inp, images) -> list:
     """Oracle spot checks of the served jobs that carry simulated scenes: whole
     rows plus 64 x 64 chips centred on point targets (2 rows + 2 chips for a
     large job, 1 + 1 for a small one)."""
@@ -476,7 +550,7 @@ def served_spot_checks(inp, args) -> list:
     out = []
     for k, sc in sorted(inp.sims.items()):
         j = inp.jobs[k]
-        img = inp.images[k]
+        img = images[k]
         p = O.params(sc.c, sc.fc, sc.r0, sc.dr, "sinc8", sc.kaiser_beta)
         tp = sc.target_pixels
         n = 2 if j.large else 1
@@ -548,6 +622,11 @@ def main() -> int:
                     help="only run the omega-K leg and print its JSON (profiling)")
     ap.add_argument("--mixed-jobs", type=int, default=64, help="batched leg jobs (0: skip)")
     ap.add_argument("--mixed-reps", type=int, default=2)
+    ap.add_argument("--class-shapes", default="common", choices=("common", "free"),
+                    help="per-class tuning of the batched and served legs: the headline's tile shape "
+                         "with pulses per stage tuned per class (common), or the autotuner's free choice")
+    ap.add_argument("--class-crop-px", type=int, default=1536,
+                    help="autotuner crop (pixels per side budget) of the per-class tuning")
     ap.add_argument("--served-jobs", type=int, default=2000, help="served leg jobs (0: skip)")
     ap.add_argument("--served-rates", default="16",
                     help="Poisson arrival rates (jobs/s) of the served stream, comma separated; "
@@ -647,12 +726,16 @@ def main() -> int:
 
     rc = leg_range(torch, ctx, native, D, args) if args.range_pulses > 0 else None
     wk = leg_omega_k(torch, ctx, native, args) if args.omega_k else None
-    batched = leg_mixed(torch, ctx, native, W, args) if args.mixed_jobs > 0 else None
+    tuned_classes = (native.size_class(sc.native_grid(), params, sc.pos),) if tune else ()
+    batched = leg_mixed(torch, ctx, native, W, args, tuned_classes) if args.mixed_jobs > 0 else None
     served = None
     if args.served_jobs > 0:
         inp = ServedInputs(W, args)
-        runs = [leg_served(native, W, args, float(r), inp, len(inp.jobs))
+        runs = [leg_served(native, W, args, float(r), inp, len(inp.jobs), ctx=ctx)
                 for r in args.served_rates.split(",") if r]
+        # the simulated jobs' images of the profile runs, before the raw-echo
+        # run (which reuses the first jobs and their image buffers) overwrites them
+        sim_images = {k: img.copy() for k, img in inp.images.items()}
         best = max(runs, key=lambda r: r["W_per_s"])
         served = {"workload": f"{len(inp.jobs)} Poisson jobs (10% large: 8192^2 image, 8192 pulses x 16384 "
                               f"samples; others sized as the batched mix), 200 GB HBM budget, sinc8, REF",
@@ -661,7 +744,7 @@ def main() -> int:
         if args.served_raw_jobs > 0:
             n_raw = min(args.served_raw_jobs, len(inp.jobs))
             raw_inputs = served_raw_inputs(W, inp.jobs[:n_raw])
-            served["raw_echo_burst"] = leg_served(native, W, args, 0.0, inp, n_raw, raw_inputs)
+            served["raw_echo_burst"] = leg_served(native, W, args, 0.0, inp, n_raw, raw_inputs, ctx=ctx)
             del raw_inputs
         e2e = plain_e2e(torch, ctx, native, inp, min(args.e2e_jobs, len(inp.jobs)))
         served["plain_end_to_end"] = e2e
@@ -669,7 +752,7 @@ def main() -> int:
         if batched:
             served["peak_speedup_vs_plain_sequential"] = served["peak_W_per_s"] / batched["plain_sequential_W_per_s"]
         if inp.sims:
-            checks = served_spot_checks(inp, args)
+            checks = served_spot_checks(inp, sim_images)
             served["spot_checks"] = checks
             served["spot_check_rel_l2"] = [c["rel_l2"] for c in checks]
             served["spot_check_max_rel_l2"] = max(c["rel_l2"] for c in checks)

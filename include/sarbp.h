@@ -235,8 +235,26 @@ typedef struct sarbp_tuning {
   int32_t pulses_per_stage;
 } sarbp_tuning;
 
+/* Size classes of the tuning cache.  A job's class is set by the range
+ * window w (samples per pulse staged in LDS) that a tile of 64 x 32 pixels
+ * needs, as the planner bounds it -- from the host positions when the call
+ * has them, else from the grid alone: class 0 for w < 80, 1 for w < 120, 2
+ * for w < 176, 3 for w < 272, 4 above.  w grows with the pixel spacing over
+ * dr, with the aperture and at close range (X band at 5 km: 0.25 / 0.35 /
+ * 0.5 / 0.7 / 1 m pixels fall in classes 0-4; 0.25 m pixels under an
+ * 8192-pulse aperture in class 1).  It sets a stage's LDS footprint per pulse
+ * and the sinc8 precompute per pixel, so each class has its own best launch
+ * configuration. */
+#define SARBP_SIZE_CLASSES 5
+
+/* The size class of a job (host only); h_pos: optional host copy of the
+ * [n_pulses][3] positions (planned bound).  -1 for invalid arguments. */
+int32_t sarbp_size_class(const sarbp_grid* grid, const sarbp_params* params,
+                         const double* h_pos);
+
 /* Sets the configuration used by later calls with this (interp, precision,
- * dtype) triple on this context; all-zero restores the built-in default.
+ * dtype) triple on this context, for every size class (replacing per-class
+ * entries); all-zero restores the built-in per-class defaults.
  * With SARBP_PREC_REF the tile shape is an upper bound: a call whose grid
  * would put tile pixels more than 48 m from the tile centre (FP32
  * differential range) runs with the largest compiled shape within it that
@@ -246,9 +264,19 @@ typedef struct sarbp_tuning {
 sarbp_status sarbp_set_tuning(sarbp_context* ctx, int32_t interp,
                               int32_t precision, int32_t dtype,
                               const sarbp_tuning* tuning);
+/* The configuration in effect for size class 0. */
 sarbp_status sarbp_get_tuning(sarbp_context* ctx, int32_t interp,
                               int32_t precision, int32_t dtype,
                               sarbp_tuning* tuning);
+/* Per size class: set (all-zero removes the class's entry) and get the
+ * configuration in effect (the class's entry, else the all-class one, else
+ * the built-in default). */
+sarbp_status sarbp_set_class_tuning(sarbp_context* ctx, int32_t interp,
+                                    int32_t precision, int32_t dtype,
+                                    int32_t size_class, const sarbp_tuning* tuning);
+sarbp_status sarbp_get_class_tuning(sarbp_context* ctx, int32_t interp,
+                                    int32_t precision, int32_t dtype,
+                                    int32_t size_class, sarbp_tuning* tuning);
 
 /* Result of sarbp_autotune. */
 typedef struct sarbp_autotune_result {
@@ -267,7 +295,9 @@ typedef struct sarbp_autotune_result {
  * tiles to fill the device -- with HIP events on `stream`, and stores the
  * fastest as this context's tuning for the job's (interp, precision, dtype).
  * d_scratch receives the crop images and must hold scratch_pixels >=
- * crop_nx * crop_ny complex64 values.  Synchronous.
+ * crop_nx * crop_ny complex64 values.  The built-in default of the job's
+ * size class stays unless a candidate is more than 2% faster.  The result is
+ * stored for the job's size class (sarbp_size_class) only.  Synchronous.
  */
 sarbp_status sarbp_autotune(sarbp_context* ctx, const void* d_profiles,
                             const double* d_pos, const double* h_pos,
@@ -343,7 +373,9 @@ typedef struct sarbp_server_config {
   size_t hbm_budget_bytes;          /* device arena for resident jobs         */
   int32_t max_active_jobs;          /* resident jobs at once (0: no limit)    */
   int32_t pulses_per_epoch;         /* pulse chunk per job per epoch (0: all) */
-  int32_t rows_per_lane;            /* launch tuning (0: built-in default)    */
+  int32_t rows_per_lane;            /* launch tuning for every size class (0:
+                                       the built-in per-class defaults, or
+                                       sarbp_server_set_class_tuning)         */
   int32_t cols_per_lane;
   int32_t pulses_per_stage;
   double epoch_watchdog_s;          /* epochs slower than this fail their jobs
@@ -387,6 +419,11 @@ typedef struct sarbp_server_stats {
 } sarbp_server_stats;
 
 sarbp_status sarbp_server_create(const sarbp_server_config* cfg, sarbp_server** server);
+/* Launch configuration of later submissions of one size class (for example
+ * a context's autotuned sarbp_get_class_tuning); all-zero restores the
+ * built-in default of the class. */
+sarbp_status sarbp_server_set_class_tuning(sarbp_server* server, int32_t size_class,
+                                           const sarbp_tuning* tuning);
 /* Rejects requests that can never fit the arena with SARBP_ERR_OVER_BUDGET. */
 sarbp_status sarbp_server_submit(sarbp_server* server, const sarbp_request* req,
                                  uint64_t* job_id);

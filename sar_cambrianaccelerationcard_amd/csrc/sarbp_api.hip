@@ -100,7 +100,12 @@ void sinc8_poly_fit(double beta, int halves, float* out) {
 void sinc8_poly(double beta, float* out) { sinc8_poly_fit<kSincCoef>(beta, kSincHalves, out); }
 void sinc8_poly_whole(double beta, float* out) { sinc8_poly_fit<kSincCoefW>(beta, 1, out); }
 
-int tuning_key(int interp, int prec, int dtype) { return interp * 100 + prec * 10 + dtype; }
+// Tuning cache key: (interp, precision, dtype, size class); class kAllClasses
+// holds a tuning set for every class at once (sarbp_set_tuning).
+constexpr int kAllClasses = SARBP_SIZE_CLASSES;
+int tuning_key(int interp, int prec, int dtype, int cls) {
+  return ((interp * 10 + prec) * 10 + dtype) * 10 + cls;
+}
 
 bool finite_all(const double* v, int n) {
   for (int i = 0; i < n; ++i)
@@ -166,6 +171,15 @@ struct sarbp_plan {
 
 namespace sarbp {
 
+sarbp_status check_tuning(int precision, const sarbp_tuning* t) {
+  if (!tile_shape_supported(precision, t->cols_per_lane, t->rows_per_lane))
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "unsupported tile shape: %d cols x %d rows per lane",
+                t->cols_per_lane, t->rows_per_lane);
+  if (t->pulses_per_stage < 1 || t->pulses_per_stage > 256)
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "pulses_per_stage must be in [1, 256]");
+  return SARBP_OK;
+}
+
 sarbp_status validate(const sarbp_grid* g, const sarbp_params* p) {
   if (!g || !p) return fail(SARBP_ERR_INVALID_ARGUMENT, "grid or params is NULL");
   if (g->nx < 0 || g->ny < 0) return fail(SARBP_ERR_INVALID_ARGUMENT, "negative image size %d x %d", g->nx, g->ny);
@@ -187,21 +201,38 @@ sarbp_status validate(const sarbp_grid* g, const sarbp_params* p) {
   return SARBP_OK;
 }
 
-sarbp_tuning get_tuning(sarbp_context* ctx, int interp, int prec, int dtype) {
-  auto it = ctx->tuning.find(tuning_key(interp, prec, dtype));
+sarbp_tuning get_tuning(sarbp_context* ctx, int interp, int prec, int dtype, int cls) {
+  auto it = ctx->tuning.find(tuning_key(interp, prec, dtype, cls));
   if (it != ctx->tuning.end()) return it->second;
-  return default_tuning(interp);
+  it = ctx->tuning.find(tuning_key(interp, prec, dtype, kAllClasses));
+  if (it != ctx->tuning.end()) return it->second;
+  return default_tuning(interp, cls, prec);
 }
 
-sarbp_tuning default_tuning(int interp) {
-  // Built-in defaults: the best points of a (tile shape x pulses/stage) sweep
-  // on MI355X at 0.25 m pixels over a 4096-pulse aperture.  Sinc8's LDS
-  // footprint per pulse is ~9x linear's, so it stages fewer pulses to keep
-  // several workgroups resident per CU.
+sarbp_tuning default_tuning(int interp, int cls, int precision) {
+  // Built-in defaults: the best one-column points of (tile shape x
+  // pulses/stage) sweeps on MI355X with the project autotuner.  Sinc8, per
+  // size class (range windows of 72, 96-108, 144, 212-236 and 332 samples:
+  // X band at 5 km, 0.25 m pixels under 2048- and 4096-8192-pulse apertures,
+  // 0.5, 0.7 and 1 m pixels): wider windows fit fewer pulses in a stage at
+  // the occupancy that wins.  Two-column tiles win on some grids but lose up
+  // to 37% under long apertures (an 8192-pulse aperture at 0.25 m: 2 x 4
+  // 0.76 s, 1 x 8 0.556 s), where their width adds range spread; the
+  // autotuner picks them where they pay.  Linear and nearest stage 32 pulses
+  // of their 8-byte windows.
   sarbp_tuning t;
   t.cols_per_lane = 1;
-  t.rows_per_lane = interp == SARBP_INTERP_SINC8 ? 8 : 4;
-  t.pulses_per_stage = interp == SARBP_INTERP_SINC8 ? 4 : 32;
+  t.rows_per_lane = 4;
+  t.pulses_per_stage = 32;
+  if (interp == SARBP_INTERP_SINC8) {
+    static const int kShape[SARBP_SIZE_CLASSES][3] = {{1, 4, 6}, {1, 8, 4}, {1, 8, 2}, {1, 4, 4}, {1, 4, 2}};
+    const int c = std::min(std::max(cls, 0), SARBP_SIZE_CLASSES - 1);
+    t.cols_per_lane = kShape[c][0];
+    t.rows_per_lane = kShape[c][1];
+    t.pulses_per_stage = kShape[c][2];
+  }
+  // the F64 and F32 modes compile one column per lane
+  if (!tile_shape_supported(precision, t.cols_per_lane, t.rows_per_lane)) t.cols_per_lane = 1;
   return t;
 }
 
@@ -361,6 +392,21 @@ Extent tile_extent(const sarbp_grid* g, int px, int py, const double* h_pos, int
   return e;
 }
 
+// The size class of a job: the range window (samples) a 64 x 32-pixel tile
+// needs, planned as plan_args plans it (with the host positions when given).
+// The window sets a stage's LDS footprint per pulse and the sinc8 precompute
+// per pixel, so it decides the pulses per stage and the tile shape that win;
+// it grows with the pixel spacing over dr, with the aperture and with
+// proximity.
+int size_class(const sarbp_grid* g, const sarbp_params* p, const double* h_pos) {
+  const Extent ext = tile_extent(g, 1, 8, h_pos, p->n_pulses);
+  const double win = 2.0 * (std::ceil(ext.half_m / p->dr) + 6.0) + 2.0;
+  const double bounds[SARBP_SIZE_CLASSES - 1] = {80.0, 120.0, 176.0, 272.0};
+  int c = 0;
+  while (c < SARBP_SIZE_CLASSES - 1 && win >= bounds[c]) ++c;
+  return c;
+}
+
 // Fills a launch descriptor.  Caller holds ctx->mu and the device guard.
 sarbp_status plan_args(const double* h_pos, const sarbp_grid* g,
                        const sarbp_params* p, const sarbp_tuning& tune, BpArgs* a,
@@ -517,16 +563,43 @@ sarbp_status sarbp_set_tuning(sarbp_context* ctx, int32_t interp, int32_t precis
   if (interp < 0 || interp > 2 || precision < 0 || precision > 2 || dtype < 0 || dtype > 1)
     return fail(SARBP_ERR_INVALID_ARGUMENT, "bad (interp, precision, dtype)");
   std::lock_guard<std::mutex> lk(ctx->mu);
+  for (int c = 0; c <= kAllClasses; ++c) ctx->tuning.erase(tuning_key(interp, precision, dtype, c));
+  if (t->rows_per_lane == 0 && t->cols_per_lane == 0 && t->pulses_per_stage == 0) return SARBP_OK;
+  sarbp_status st = check_tuning(precision, t);
+  if (st != SARBP_OK) return st;
+  ctx->tuning[tuning_key(interp, precision, dtype, kAllClasses)] = *t;
+  return SARBP_OK;
+}
+
+int32_t sarbp_size_class(const sarbp_grid* grid, const sarbp_params* params, const double* h_pos) {
+  if (validate(grid, params) != SARBP_OK) return -1;
+  if (h_pos && !finite_all(h_pos, 3 * params->n_pulses)) return -1;
+  return size_class(grid, params, h_pos);
+}
+
+sarbp_status sarbp_set_class_tuning(sarbp_context* ctx, int32_t interp, int32_t precision, int32_t dtype,
+                                    int32_t size_cls, const sarbp_tuning* t) {
+  if (!ctx || !t) return fail(SARBP_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (interp < 0 || interp > 2 || precision < 0 || precision > 2 || dtype < 0 || dtype > 1 || size_cls < 0 ||
+      size_cls >= SARBP_SIZE_CLASSES)
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "bad (interp, precision, dtype, size class)");
+  std::lock_guard<std::mutex> lk(ctx->mu);
   if (t->rows_per_lane == 0 && t->cols_per_lane == 0 && t->pulses_per_stage == 0) {
-    ctx->tuning.erase(tuning_key(interp, precision, dtype));
+    ctx->tuning.erase(tuning_key(interp, precision, dtype, size_cls));
     return SARBP_OK;
   }
-  if (!tile_shape_supported(precision, t->cols_per_lane, t->rows_per_lane))
-    return fail(SARBP_ERR_INVALID_ARGUMENT, "unsupported tile shape: %d cols x %d rows per lane",
-                t->cols_per_lane, t->rows_per_lane);
-  if (t->pulses_per_stage < 1 || t->pulses_per_stage > 256)
-    return fail(SARBP_ERR_INVALID_ARGUMENT, "pulses_per_stage must be in [1, 256]");
-  ctx->tuning[tuning_key(interp, precision, dtype)] = *t;
+  sarbp_status st = check_tuning(precision, t);
+  if (st != SARBP_OK) return st;
+  ctx->tuning[tuning_key(interp, precision, dtype, size_cls)] = *t;
+  return SARBP_OK;
+}
+
+sarbp_status sarbp_get_class_tuning(sarbp_context* ctx, int32_t interp, int32_t precision, int32_t dtype,
+                                    int32_t size_cls, sarbp_tuning* t) {
+  if (!ctx || !t) return fail(SARBP_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (size_cls < 0 || size_cls >= SARBP_SIZE_CLASSES) return fail(SARBP_ERR_INVALID_ARGUMENT, "size class %d", size_cls);
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  *t = get_tuning(ctx, interp, precision, dtype, size_cls);
   return SARBP_OK;
 }
 
@@ -534,7 +607,7 @@ sarbp_status sarbp_get_tuning(sarbp_context* ctx, int32_t interp, int32_t precis
                               sarbp_tuning* t) {
   if (!ctx || !t) return fail(SARBP_ERR_INVALID_ARGUMENT, "NULL argument");
   std::lock_guard<std::mutex> lk(ctx->mu);
-  *t = get_tuning(ctx, interp, precision, dtype);
+  *t = get_tuning(ctx, interp, precision, dtype, 0);
   return SARBP_OK;
 }
 
@@ -550,8 +623,8 @@ sarbp_status sarbp_backproject(sarbp_context* ctx, const void* d_profiles, const
   DeviceGuard dg(ctx->device);
   if (dg.err != hipSuccess) return hip_fail(dg.err, "hipSetDevice");
   const sarbp_tuning tune =
-      fit_tuning(get_tuning(ctx, params->interp, params->precision, params->dtype), params->precision, grid, params,
-                 nullptr);
+      fit_tuning(get_tuning(ctx, params->interp, params->precision, params->dtype, size_class(grid, params, nullptr)),
+                 params->precision, grid, params, nullptr);
   BpArgs a;
   st = plan_args(nullptr, grid, params, tune, &a, nullptr);
   if (st != SARBP_OK) return st;
@@ -589,8 +662,8 @@ sarbp_status sarbp_plan_create(sarbp_context* ctx, const double* h_pos, const sa
     return hip_fail(dg.err, "hipSetDevice");
   }
   const sarbp_tuning tune =
-      fit_tuning(get_tuning(ctx, params->interp, params->precision, params->dtype), params->precision, grid, params,
-                 h_pos);
+      fit_tuning(get_tuning(ctx, params->interp, params->precision, params->dtype, size_class(grid, params, h_pos)),
+                 params->precision, grid, params, h_pos);
   std::memset(&pl->info, 0, sizeof pl->info);
   st = plan_args(h_pos, grid, params, tune, &pl->args, &pl->info);
   if (st != SARBP_OK) {
@@ -669,7 +742,6 @@ sarbp_status sarbp_backproject_batch(sarbp_context* ctx, const sarbp_job* jobs, 
   std::lock_guard<std::mutex> lk(ctx->mu);
   DeviceGuard dg(ctx->device);
   if (dg.err != hipSuccess) return hip_fail(dg.err, "hipSetDevice");
-  const sarbp_tuning base = get_tuning(ctx, p0.interp, p0.precision, p0.dtype);
   // Largest per-tile cost first (pulses per tile), ties by submission order:
   // the persistent grid drains items in this order.
   std::vector<int32_t> order((size_t)n_jobs);
@@ -689,7 +761,8 @@ sarbp_status sarbp_backproject_batch(sarbp_context* ctx, const sarbp_job* jobs, 
   for (int32_t j : order) {
     const sarbp_job& jb = jobs[j];
     if (jb.grid.nx == 0 || jb.grid.ny == 0) continue;
-    const sarbp_tuning t = fit_tuning(base, p0.precision, &jb.grid, &jb.params, jb.h_pos);
+    const sarbp_tuning t = fit_tuning(get_tuning(ctx, p0.interp, p0.precision, p0.dtype, size_class(&jb.grid, &jb.params, jb.h_pos)),
+                                      p0.precision, &jb.grid, &jb.params, jb.h_pos);
     JobDesc d;
     sarbp_status st = plan_args(jb.h_pos, &jb.grid, &jb.params, t, &d.a, nullptr);
     if (st != SARBP_OK) return fail(st, "job %d: %s", j, g_detail);
@@ -736,7 +809,6 @@ sarbp_status sarbp_backproject_batch_noalloc(sarbp_context* ctx, const sarbp_job
   if (dg.err != hipSuccess) return hip_fail(dg.err, "hipSetDevice");
   // The host image of the scratch is shared by every context: one call at a time.
   std::lock_guard<std::mutex> sk(g_scratch_mu);
-  const sarbp_tuning base = get_tuning(ctx, p0.interp, p0.precision, p0.dtype);
   // Largest pulse count first, ties in submission order (insertion sort: no
   // temporary buffer), as sarbp_backproject_batch orders its work list.
   int32_t order[kNoAllocJobs];
@@ -761,7 +833,8 @@ sarbp_status sarbp_backproject_batch_noalloc(sarbp_context* ctx, const sarbp_job
   for (int32_t r = 0; r < n_jobs; ++r) {
     const sarbp_job& jb = jobs[order[r]];
     if (jb.grid.nx == 0 || jb.grid.ny == 0) continue;
-    const sarbp_tuning t = fit_tuning(base, p0.precision, &jb.grid, &jb.params, jb.h_pos);
+    const sarbp_tuning t = fit_tuning(get_tuning(ctx, p0.interp, p0.precision, p0.dtype, size_class(&jb.grid, &jb.params, jb.h_pos)),
+                                      p0.precision, &jb.grid, &jb.params, jb.h_pos);
     JobDesc& d = g_noalloc_plan[n_planned];
     st = plan_args(jb.h_pos, &jb.grid, &jb.params, t, &d.a, nullptr);
     if (st != SARBP_OK) return fail(st, "job %d: %s", order[r], g_detail);
@@ -885,7 +958,8 @@ sarbp_status sarbp_autotune(sarbp_context* ctx, const void* d_profiles, const do
     return SARBP_OK;
   };
   // default first (built-in, ignoring any stored tuning)
-  const sarbp_tuning def = default_tuning(p.interp);
+  const int cls = size_class(grid, params, h_pos);
+  const sarbp_tuning def = default_tuning(p.interp, cls, p.precision);
   int eff = 0;
   double def_ms = INFINITY;
   st = time_one(def, &def_ms, &eff);
@@ -897,7 +971,7 @@ sarbp_status sarbp_autotune(sarbp_context* ctx, const void* d_profiles, const do
   std::vector<std::tuple<int, int, int>> seen;
   seen.emplace_back(def.cols_per_lane, def.rows_per_lane, eff);
   const int shapes[][2] = {{1, 2}, {1, 4}, {1, 8}, {2, 2}, {2, 4}, {2, 8}, {4, 2}, {4, 4}};
-  const int stages[] = {4, 6, 8, 12, 16, 24, 32, 48, 64};
+  const int stages[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64};
   for (const auto& sh : shapes) {
     if (st != SARBP_OK) break;
     if (!tile_shape_supported(p.precision, sh[0], sh[1])) continue;
@@ -908,13 +982,20 @@ sarbp_status sarbp_autotune(sarbp_context* ctx, const void* d_profiles, const do
       t.pulses_per_stage = P;
       BpArgs probe;
       if (plan_args(h_pos, &crop, &p, t, &probe, nullptr) != SARBP_OK) continue;
+      // fewer than 4 pulses per stage only pay where 4 would leave fewer
+      // than 4 workgroups per CU (40 KiB of LDS each)
+      if (P < 4 && lds_layout(probe.kernel_interp, probe.win, 4).total <= 40 * 1024) continue;
       const auto key = std::make_tuple(sh[0], sh[1], probe.pulses_per_stage);
       if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
       seen.push_back(key);
       double ms = 0.0;
       if ((st = time_one(t, &ms, &eff)) != SARBP_OK) break;
       ++n;
-      if (ms < best_ms) {
+      // a candidate replaces the built-in default only when it is more than
+      // 2% faster: near-ties on one crop do not carry to the other jobs of
+      // the class (a two-column tile that ties on a 4096-pulse image loses
+      // 37% under an 8192-pulse aperture)
+      if (ms < best_ms && ms < 0.98 * def_ms) {
         best_ms = ms;
         best = t;
         best.pulses_per_stage = eff;
@@ -924,7 +1005,7 @@ sarbp_status sarbp_autotune(sarbp_context* ctx, const void* d_profiles, const do
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (st != SARBP_OK) return st;
-  ctx->tuning[tuning_key(p.interp, p.precision, p.dtype)] = best;
+  ctx->tuning[tuning_key(p.interp, p.precision, p.dtype, cls)] = best;
   out->best = best;
   out->best_ms = best_ms;
   out->default_ms = def_ms;

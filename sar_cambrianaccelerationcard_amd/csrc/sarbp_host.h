@@ -39,7 +39,12 @@ struct DeviceGuard {
 };
 
 sarbp_status validate(const sarbp_grid* g, const sarbp_params* p);
-sarbp_tuning default_tuning(int interp);
+// Size class of a job (0 .. SARBP_SIZE_CLASSES - 1; sarbp_size_class).
+int size_class(const sarbp_grid* g, const sarbp_params* p, const double* h_pos);
+// Built-in tuning of an interpolator for a size class and precision mode.
+sarbp_tuning default_tuning(int interp, int cls, int precision);
+// SARBP_ERR_INVALID_ARGUMENT unless the shape is compiled and 1 <= P <= 256.
+sarbp_status check_tuning(int precision, const sarbp_tuning* t);
 // The tuning with its tile shape reduced, if needed, to the largest compiled
 // shape within it whose tiles fit the REF tile-radius limit on this grid
 // (h_pos, if given, bounds the range from below and relaxes the limit).

@@ -128,7 +128,7 @@ using namespace sarbp;
 
 struct sarbp_server {
   sarbp_server_config cfg{};
-  sarbp_tuning tune{};
+  sarbp_tuning tune[SARBP_SIZE_CLASSES]{};  // per size class; guarded by mu
   int n_cu = 256;
   Arena arena;
   std::mutex mu;
@@ -460,15 +460,15 @@ sarbp_status sarbp_server_create(const sarbp_server_config* cfg, sarbp_server** 
   std::unique_ptr<sarbp_server> s(new (std::nothrow) sarbp_server);
   if (!s) return fail(SARBP_ERR_OUT_OF_MEMORY, "server");
   s->cfg = *cfg;
-  s->tune = default_tuning(cfg->interp);
-  if (cfg->rows_per_lane) s->tune.rows_per_lane = cfg->rows_per_lane;
-  if (cfg->cols_per_lane) s->tune.cols_per_lane = cfg->cols_per_lane;
-  if (cfg->pulses_per_stage) s->tune.pulses_per_stage = cfg->pulses_per_stage;
-  if (!tile_shape_supported(cfg->precision, s->tune.cols_per_lane, s->tune.rows_per_lane))
-    return fail(SARBP_ERR_INVALID_ARGUMENT, "unsupported tile shape %d x %d per lane", s->tune.cols_per_lane,
-                s->tune.rows_per_lane);
-  if (s->tune.pulses_per_stage < 1 || s->tune.pulses_per_stage > 256)
-    return fail(SARBP_ERR_INVALID_ARGUMENT, "pulses_per_stage must be in [1, 256]");
+  for (int c = 0; c < SARBP_SIZE_CLASSES; ++c) {
+    sarbp_tuning& t = s->tune[c];
+    t = default_tuning(cfg->interp, c, cfg->precision);
+    if (cfg->rows_per_lane) t.rows_per_lane = cfg->rows_per_lane;
+    if (cfg->cols_per_lane) t.cols_per_lane = cfg->cols_per_lane;
+    if (cfg->pulses_per_stage) t.pulses_per_stage = cfg->pulses_per_stage;
+    sarbp_status st = check_tuning(cfg->precision, &t);
+    if (st != SARBP_OK) return st;
+  }
   DeviceGuard dg(cfg->device);
   if (dg.err != hipSuccess) return hip_fail(dg.err, "hipSetDevice");
   int cu = 0;
@@ -532,7 +532,12 @@ sarbp_status submit_job(sarbp_server* s, const sarbp_request* req, const sarbp_r
   std::unique_ptr<Job> j(new (std::nothrow) Job);
   if (!j) return fail(SARBP_ERR_OUT_OF_MEMORY, "job");
   j->req = *req;
-  j->tune = fit_tuning(s->tune, p.precision, &req->grid, &p, req->h_pos);
+  sarbp_tuning base;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    base = s->tune[size_class(&req->grid, &p, req->h_pos)];
+  }
+  j->tune = fit_tuning(base, p.precision, &req->grid, &p, req->h_pos);
   if ((st = plan_args(req->h_pos, &req->grid, &p, j->tune, &j->args, nullptr)) != SARBP_OK) return st;
   j->prof_bytes = (size_t)p.n_pulses * p.n_samples * (p.dtype == SARBP_DTYPE_C64 ? 8 : 4);
   j->pos_bytes = (size_t)p.n_pulses * 24;
@@ -569,6 +574,19 @@ sarbp_status submit_job(sarbp_server* s, const sarbp_request* req, const sarbp_r
 }
 
 }  // namespace
+
+sarbp_status sarbp_server_set_class_tuning(sarbp_server* s, int32_t size_cls, const sarbp_tuning* t) {
+  if (!s || !t) return fail(SARBP_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (size_cls < 0 || size_cls >= SARBP_SIZE_CLASSES) return fail(SARBP_ERR_INVALID_ARGUMENT, "size class %d", size_cls);
+  sarbp_tuning v = *t;
+  if (v.rows_per_lane == 0 && v.cols_per_lane == 0 && v.pulses_per_stage == 0)
+    v = default_tuning(s->cfg.interp, size_cls, s->cfg.precision);
+  sarbp_status st = check_tuning(s->cfg.precision, &v);
+  if (st != SARBP_OK) return st;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->tune[size_cls] = v;
+  return SARBP_OK;
+}
 
 sarbp_status sarbp_server_submit(sarbp_server* s, const sarbp_request* req, uint64_t* job_id) {
   if (!s || !req || !job_id) return fail(SARBP_ERR_INVALID_ARGUMENT, "NULL argument");

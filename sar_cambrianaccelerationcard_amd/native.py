@@ -177,6 +177,12 @@ _SIGS = {
     "sarbp_plan_destroy": (None, [C.c_void_p]),
     "sarbp_set_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(Tuning)]),
     "sarbp_get_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(Tuning)]),
+    "sarbp_size_class": (C.c_int32, [C.POINTER(Grid), C.POINTER(Params), C.c_void_p]),
+    "sarbp_set_class_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                         C.POINTER(Tuning)]),
+    "sarbp_get_class_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                         C.POINTER(Tuning)]),
+    "sarbp_server_set_class_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Tuning)]),
     "sarbp_backproject_batch": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_int32, C.c_void_p]),
     "sarbp_backproject_batch_noalloc": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_int32, C.c_void_p]),
     "sarbp_sinc8_coefficients": (C.c_int, [C.c_double, C.POINTER(C.c_float)]),
@@ -424,6 +430,18 @@ class Context:
               "sarbp_get_tuning")
         return t.rows_per_lane, t.cols_per_lane, t.pulses_per_stage
 
+    def set_class_tuning(self, interp: int, precision: int, dtype: int, size_class: int, rows: int,
+                         cols: int, pulses: int) -> None:
+        t = Tuning(rows, cols, pulses)
+        check(self.lib.sarbp_set_class_tuning(self.handle, interp, precision, dtype, size_class, C.byref(t)),
+              "sarbp_set_class_tuning")
+
+    def get_class_tuning(self, interp: int, precision: int, dtype: int, size_class: int) -> tuple:
+        t = Tuning()
+        check(self.lib.sarbp_get_class_tuning(self.handle, interp, precision, dtype, size_class, C.byref(t)),
+              "sarbp_get_class_tuning")
+        return t.rows_per_lane, t.cols_per_lane, t.pulses_per_stage
+
     def autotune(self, d_profiles: int, d_pos: int, h_pos, grid: Grid, params: Params,
                  d_scratch: int, scratch_pixels: int, max_crop_px: int = 1024,
                  stream: int = 0) -> dict:
@@ -472,6 +490,20 @@ def sinc8_coefficients_whole(beta: float):
     out = (C.c_float * 64)()
     check(load().sarbp_sinc8_coefficients_whole(float(beta), out), "sarbp_sinc8_coefficients_whole")
     return np.frombuffer(out, dtype=np.float32).reshape(8, 8).copy()
+
+
+SIZE_CLASSES = 5
+
+
+def size_class(grid: Grid, params: Params, h_pos=None) -> int:
+    """The tuning size class of a job (sarbp_size_class); h_pos: optional host
+    float64 [n_pulses][3] positions (the planned bound)."""
+    import numpy as np
+    hp = None
+    if h_pos is not None:
+        h_pos = np.ascontiguousarray(h_pos, dtype=np.float64)
+        hp = h_pos.ctypes.data_as(C.c_void_p)
+    return int(load().sarbp_size_class(C.byref(grid), C.byref(params), hp))
 
 
 def valu_probe(kind: int, n_blocks: int, iters: int, d_out: int, stream: int = 0) -> None:
@@ -545,6 +577,18 @@ class Server:
         self.handle = h
         self.config = c
         self._keep = {}
+
+    def set_class_tuning(self, size_class: int, rows: int, cols: int, pulses: int) -> None:
+        t = Tuning(rows, cols, pulses)
+        check(self.lib.sarbp_server_set_class_tuning(self.handle, int(size_class), C.byref(t)),
+              "sarbp_server_set_class_tuning")
+
+    def tune_like(self, ctx: "Context") -> None:
+        """Takes every size class's tuning in effect on ctx (e.g. autotuned)."""
+        c = self.config
+        for k in range(SIZE_CLASSES):
+            r, co, p = ctx.get_class_tuning(c.interp, c.precision, c.dtype, k)
+            self.set_class_tuning(k, r, co, p)
 
     def submit(self, profiles, pos, grid: Grid, params: Params, image) -> int:
         """profiles: host array (complex64, or uint32 bf16x2 words); pos: float64

@@ -63,3 +63,30 @@ def test_missing_library_raises(monkeypatch):
     monkeypatch.setattr(native, "SARBP_PATH", "/nonexistent/libsarbp.so")
     with pytest.raises(native.NativeLibraryMissing):
         native.load()
+
+
+def test_size_classes_follow_the_planned_range_window():
+    """Classes grow with pixel spacing (the tile's range window), positions
+    tighten the bound, and invalid arguments give -1."""
+    import numpy as np
+    dr = 299792458.0 / (2 * 4 * 200e6)  # 0.1874 m
+    n = 512
+    pos = np.zeros((n, 3))
+    pos[:, 0] = (np.arange(n) - n / 2) * 0.15
+    pos[:, 2] = 3000.0
+    p = native.make_params(c=299792458.0, fc=9.6e9, r0=4000.0, dr=dr, n_pulses=n, n_samples=64)
+    with_pos, without = [], []
+    for spacing in (0.1, 0.25, 0.35, 0.5, 0.7, 1.0, 2.0):
+        g = native.make_grid([-100, 3900, 0], [spacing, 0, 0], [0, spacing, 0], 64, 64)
+        with_pos.append(native.size_class(g, p, pos))
+        without.append(native.size_class(g, p))
+    assert with_pos == sorted(with_pos) and without == sorted(without)
+    assert with_pos[0] == 0 and with_pos[-1] == 4
+    assert all(a <= b for a, b in zip(with_pos, without))
+    assert with_pos[1] < without[1]  # the positions' bound is tighter than |R - Rc| <= |d|
+    g = native.make_grid([0, 0, 0], [0, 0, 0], [0, 0.5, 0], 16, 16)
+    assert native.size_class(g, p) == -1
+    bad = pos.copy()
+    bad[3, 1] = float("nan")
+    g = native.make_grid([-100, 3900, 0], [0.25, 0, 0], [0, 0.25, 0], 64, 64)
+    assert native.size_class(g, p, bad) == -1

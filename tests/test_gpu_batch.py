@@ -114,3 +114,65 @@ def test_graph_replays_with_eager_work_between(ctx, jobs):
         torch.cuda.synchronize()
         for i in range(len(dss)):
             assert np.array_equal(imgs[i].cpu().numpy().view(np.uint64), eager[i].view(np.uint64)), (rep, i)
+
+
+def test_size_class_tuning_routes_jobs(ctx, jobs):
+    """Per size class tunings: a plan uses its class's configuration, and a
+    batch of jobs of two classes, run with their classes' shapes, stays
+    bitwise equal to the plain launches and matches the oracle."""
+    import torch
+    scs, dss = jobs
+    key = (native.INTERP_SINC8, native.PREC_REF, native.DTYPE_C64)
+    sc0 = scs[0]
+    c0 = native.size_class(sc0.native_grid(), sc0.native_params(interp="sinc8"), sc0.pos)
+    cfg = S.scaled_config(S.load_config("config0"), n_pulses=120, nx=96, ny=64)
+    cfg["seed"] = 1777
+    cfg["image"]["interp"] = "sinc8"
+    cfg["image"]["spacing"] = 1.0  # 1 m pixels: the coarsest class
+    coarse = S.make_scene(cfg)
+    ds_c = D.DeviceScene(coarse)
+    c1 = native.size_class(coarse.native_grid(), coarse.native_params(interp="sinc8"), coarse.pos)
+    assert c0 < c1  # 0.5 m and 1 m pixels
+    try:
+        ctx.set_class_tuning(*key, c0, 2, 2, 5)   # rows, cols, pulses per stage
+        ctx.set_class_tuning(*key, c1, 4, 1, 3)
+        assert ctx.get_class_tuning(*key, c0) == (2, 2, 5)
+        assert ctx.get_class_tuning(*key, c1) == (4, 1, 3)
+        i0 = ctx.plan(sc0.pos, sc0.native_grid(), sc0.native_params(interp="sinc8")).info()
+        i1 = ctx.plan(coarse.pos, coarse.native_grid(), coarse.native_params(interp="sinc8")).info()
+        assert (i0["cols_per_lane"], i0["rows_per_lane"]) == (2, 2)
+        assert (i1["cols_per_lane"], i1["rows_per_lane"]) == (1, 4)
+        stream = torch.cuda.current_stream().cuda_stream
+        pair = [(dss[0], dss[0].new_image()), (ds_c, ds_c.new_image())]
+        ctx.backproject_batch([_job(ds, im, h_pos=True) for ds, im in pair], stream)
+        for ds, im in pair:
+            plain = D.backproject(ctx, ds, interp="sinc8", plan=True).cpu().numpy()
+            assert np.array_equal(im.cpu().numpy().view(np.uint64), plain.view(np.uint64))
+        p = O.params(coarse.c, coarse.fc, coarse.r0, coarse.dr, "sinc8", coarse.kaiser_beta)
+        ref = O.backproject(coarse.profiles, coarse.pos, coarse.grid, p)
+        err = M.rel_l2(pair[1][1].cpu().numpy(), ref)
+        print(f"  coarse 1 m job, class {c1}: rel_l2={err:.2e}", flush=True)
+        assert err <= 1e-4
+    finally:
+        ctx.set_tuning(*key, 0, 0, 0)  # the built-in per-class defaults again
+
+
+def test_size_class_is_the_planned_window_class(ctx, jobs):
+    """A job's class is the class of the window a 64 x 32 tile plans for it."""
+    scs, dss = jobs
+    key = (native.INTERP_SINC8, native.PREC_REF, native.DTYPE_C64)
+    bounds = (80, 120, 176, 272)
+    try:
+        ctx.set_tuning(*key, 8, 1, 4)  # 1 x 8 pixels per lane: 64 x 32 tiles
+        for sc in scs:
+            for spacing in (0.25, 0.5, 1.0):
+                g = sc.native_grid()
+                g.ux[0] = spacing
+                g.uy[1] = spacing
+                pa = sc.native_params(interp="sinc8")
+                info = ctx.plan(sc.pos, g, pa).info()
+                if (info["cols_per_lane"], info["rows_per_lane"]) != (1, 8):
+                    continue  # shrunk to the radius limit: a different window
+                assert native.size_class(g, pa, sc.pos) == sum(info["window"] >= b for b in bounds)
+    finally:
+        ctx.set_tuning(*key, 0, 0, 0)

@@ -77,7 +77,11 @@ def test_config0_full(ctx):
 
 
 def test_plan_matches_plain(ctx, small):
+    """A plan (window bounded from the positions) and the unplanned call
+    (bounded from the grid alone) may fall in different size classes, so run
+    different launch configurations and round differently: they agree to
+    FP32 rounding, and each is held to the oracle by the tests above."""
     sc, ds = small
     a = D.backproject(ctx, ds, interp="sinc8").cpu().numpy()
     b = D.backproject(ctx, ds, interp="sinc8", plan=True).cpu().numpy()
-    assert M.rel_l2(b, a) <= 1e-6
+    assert M.rel_l2(b, a) <= 1e-5

@@ -197,3 +197,26 @@ def test_raw_request_validation():
                        sc.native_params(interp="sinc8"), img)
     assert e.value.status == native.ERR_UNSUPPORTED
     srv.close()
+
+
+def test_server_takes_class_tuning(ctx, scenes):
+    """sarbp_server_set_class_tuning (here through tune_like, from a context's
+    per-class tuning): a job served in one epoch is bitwise the planned plain
+    launch with the same configuration."""
+    import torch
+    from sar_cambrianaccelerationcard_amd import device as D
+    sc = scenes[2]
+    key = (native.INTERP_SINC8, native.PREC_REF, native.DTYPE_C64)
+    cls = native.size_class(sc.native_grid(), sc.native_params(interp="sinc8"), sc.pos)
+    try:
+        ctx.set_class_tuning(*key, cls, 2, 2, 5)
+        srv = native.Server(interp="sinc8", hbm_budget_bytes=1 << 30, pulses_per_epoch=0)
+        srv.tune_like(ctx)
+        jid, img = _submit(srv, sc)
+        srv.wait(jid, timeout_s=120)
+        srv.close()
+        plain = D.backproject(ctx, D.DeviceScene(sc), interp="sinc8", plan=True).cpu().numpy()
+        torch.cuda.synchronize()
+        assert np.array_equal(img.view(np.uint64), plain.view(np.uint64))
+    finally:
+        ctx.set_tuning(*key, 0, 0, 0)
