@@ -372,6 +372,7 @@ def leg_cpu(sc, gpu_img, interp, args):
     w0_1 = 32 * s0.grid["nx"] * s0.n_pulses
     O.backproject(s0.profiles, s0.pos, s0.grid, p0, rows=rows0, n_threads=1)
     c0_64_1 = median_time(lambda: O.backproject(s0.profiles, s0.pos, s0.grid, p0, rows=rows0, n_threads=1))
+    acc["_ref_rows"] = ref
     log(f"CPU oracle config 0 in full ({w0:.3e} W, {s0.interp}): FP64 {w0 / c0_64:.3e} W/s, FP32 "
         f"{w0 / c0_32:.3e} W/s on {nthr} threads; one thread FP64 {w0_1 / c0_64_1:.3e} W/s")
     cpu = {"value": w_rows / t64, "unit": "pixel-pulses/s", "cores": nthr, "kind": "port",
@@ -394,6 +395,50 @@ def leg_cpu(sc, gpu_img, interp, args):
                "config0_rows_fp64_1thread": {"W_per_s": w0_1 / c0_64_1, "threads": 1, "s": c0_64_1,
                                              "sample": "rows [0,32)"}}}
     return cpu, acc
+
+
+def leg_modes(torch, ctx, native, sc, ds, interp, args, acc) -> dict:
+    """The other accurate paths on the headline scene: the unplanned drop-in
+    entry point (sarbp_backproject: no host positions, so windows come from
+    the triangle inequality), and the precision modes F64 (FP64 range and
+    phase per pixel-pulse), DF (FP64 tile reference + double-float per-pixel
+    differential) and F32 (FP32 range; inaccurate), each planned.  HIP-event
+    time per launch, and rel-L2 on the oracle rows of the accuracy check."""
+    from sar_cambrianaccelerationcard_amd import metrics as M
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    img = ds.new_image()
+    out = {}
+    ref = acc.get("_ref_rows") if acc else None
+    for name, prec, planned in (("ref_unplanned", "ref", False), ("df", "df", True),
+                                ("f64", "f64", True), ("f32", "f32", True)):
+        params = sc.native_params(interp=interp, precision=prec)
+        if planned:
+            plan = ctx.plan(sc.pos, sc.native_grid(), params)
+            info = plan.info()
+
+            def fn():
+                plan.execute(ds.prof.data_ptr(), ds.pos.data_ptr(), img.data_ptr(), sp)
+        else:
+            info = None
+
+            def fn():
+                ctx.backproject(ds.prof.data_ptr(), ds.pos.data_ptr(), sc.native_grid(), params,
+                                img.data_ptr(), sp)
+        fn()
+        t = timed(torch, stream, fn, max(1, args.mode_reps))
+        r = {"W_per_s": sc.work / t, "ms": t * 1e3, "precision": prec, "planned": planned}
+        if info:
+            r["tile"] = f"{info['cols_per_lane']}x{info['rows_per_lane']} px/lane, " \
+                        f"{info['pulses_per_stage']} pulses/stage, window {info['window']}"
+        if ref is not None:
+            lo, hi = acc["rows"]
+            r["rel_l2_rows"] = M.rel_l2(img[lo:hi].cpu().numpy(), ref)
+        log(f"mode {name}: {t * 1e3:.2f} ms/launch, {r['W_per_s']:.4e} W/s"
+            + (f", rows rel-L2 {r['rel_l2_rows']:.2e}" if "rel_l2_rows" in r else ""))
+        out[name] = r
+    del img
+    return out
 
 
 class ServedInputs:
@@ -620,6 +665,8 @@ def main() -> int:
     ap.add_argument("--omega-k", type=int, default=1, help="omega-K leg (0: skip)")
     ap.add_argument("--omega-k-only", action="store_true",
                     help="only run the omega-K leg and print its JSON (profiling)")
+    ap.add_argument("--mode-reps", type=int, default=2,
+                    help="timed launches of the unplanned entry point and the DF / F64 / F32 modes (0: skip)")
     ap.add_argument("--mixed-jobs", type=int, default=64, help="batched leg jobs (0: skip)")
     ap.add_argument("--mixed-reps", type=int, default=2)
     ap.add_argument("--class-shapes", default="common", choices=("common", "free"),
@@ -721,6 +768,9 @@ def main() -> int:
     cpu = acc = None
     if args.cpu_rows > 0:
         cpu, acc = leg_cpu(sc, img.cpu().numpy(), interp, args)
+    modes = leg_modes(torch, ctx, native, sc, ds, interp, args, acc) if args.mode_reps > 0 else None
+    if acc:
+        acc.pop("_ref_rows", None)
     del ds, img
     torch.cuda.empty_cache()
 
@@ -798,6 +848,7 @@ def main() -> int:
                                          "frac": wps * FLOP_W[interp] / 1e12 / PEAK_TFLOPS}},
         "cpu_baseline": cpu,
         "accuracy": acc,
+        "other_paths": modes,
         "range_compression": rc,
         "omega_k": wk,
         "batched": batched,

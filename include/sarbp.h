@@ -59,10 +59,16 @@ typedef enum sarbp_interp {
 
 typedef enum sarbp_precision {
   /* FP64 per-(tile, pulse) reference range and phase, FP32 per-pixel
-   * differential with exact modular phase reduction.  Default. */
+   * differential with exact modular phase reduction.  Default.  The FP32
+   * differential bounds a tile's radius (48 m at 5 km, more at longer
+   * range); a job whose grid no tile shape fits runs SARBP_PREC_DF. */
   SARBP_PREC_REF = 0,
   SARBP_PREC_F64 = 1, /* FP64 slant range and phase for every pixel-pulse    */
-  SARBP_PREC_F32 = 2  /* FP32 positions, range and phase (inaccurate)         */
+  SARBP_PREC_F32 = 2, /* FP32 positions, range and phase (inaccurate)         */
+  /* FP64 per-(tile, pulse) reference plus a compensated double-float
+   * (FP32 pair) per-pixel differential range and phase: no tile-radius
+   * limit, about half the REF rate. */
+  SARBP_PREC_DF = 3
 } sarbp_precision;
 
 typedef enum sarbp_dtype {
@@ -206,7 +212,10 @@ typedef struct sarbp_plan_info {
   int32_t sinc8_halves;         /* SINC8 kernel: 2 = half-bin degree-5 tap
                                    polynomials, 1 = whole-bin degree 7; 0 for
                                    other interpolators (chosen per launch)   */
-  int32_t reserved;
+  int32_t precision;            /* arithmetic the kernels run: the requested
+                                   sarbp_precision, or SARBP_PREC_DF for a
+                                   SARBP_PREC_REF grid too coarse for its
+                                   tile-radius limit                        */
 } sarbp_plan_info;
 
 sarbp_status sarbp_plan_create(sarbp_context* ctx, const double* h_pos,
@@ -682,6 +691,15 @@ sarbp_status sarbp_sinc8_coefficients_whole(double beta, float* out);
  * Asynchronous on `stream`; the caller times it. */
 sarbp_status sarbp_valu_probe(int32_t kind, int32_t n_blocks, int32_t iters,
                               float* d_out, void* stream);
+
+/* The BP kernels' phase rotation evaluated on n phases (revolutions, FP32,
+ * device memory): d_out[2 i] = cos(2 pi ph[i]), d_out[2 i + 1] = sin(2 pi
+ * ph[i]) as the pixel loop computes them (hardware v_cos_f32 / v_sin_f32 on
+ * revolutions, the rotation of 1 + 0j).  For accuracy tests against FP64.
+ * No counterpart in the reference, whose kernel calls sincosf
+ * (code/SAR_BP/CUDA/BPA_CUDA_kernels.cu:67).  Asynchronous on `stream`. */
+sarbp_status sarbp_rotation_probe(const float* d_phase, float* d_out, int64_t n,
+                                  void* stream);
 
 #ifdef __cplusplus
 }

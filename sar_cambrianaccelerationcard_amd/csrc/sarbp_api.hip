@@ -188,7 +188,7 @@ sarbp_status validate(const sarbp_grid* g, const sarbp_params* p) {
   if (p->n_pulses < 0) return fail(SARBP_ERR_INVALID_ARGUMENT, "n_pulses < 0");
   if (p->n_samples < 1) return fail(SARBP_ERR_INVALID_ARGUMENT, "n_samples < 1");
   if (p->interp < 0 || p->interp > 2) return fail(SARBP_ERR_INVALID_ARGUMENT, "interp %d", p->interp);
-  if (p->precision < 0 || p->precision > 2) return fail(SARBP_ERR_INVALID_ARGUMENT, "precision %d", p->precision);
+  if (p->precision < 0 || p->precision > 3) return fail(SARBP_ERR_INVALID_ARGUMENT, "precision %d", p->precision);
   if (p->dtype < 0 || p->dtype > 1) return fail(SARBP_ERR_INVALID_ARGUMENT, "dtype %d", p->dtype);
   if (p->accumulate != 0 && p->accumulate != 1) return fail(SARBP_ERR_INVALID_ARGUMENT, "accumulate %d", p->accumulate);
   if (!(std::isfinite(p->c) && p->c > 0)) return fail(SARBP_ERR_INVALID_ARGUMENT, "c must be > 0");
@@ -300,17 +300,21 @@ double tile_radius(const sarbp_grid* g, int px, int py) {
   return r;
 }
 
+int plan_window(const sarbp_grid* g, const sarbp_params* p, int px, int py, const double* h_pos,
+                int* kinterp);
+
 // The tuning with its tile shape reduced, if needed, to the largest compiled
 // shape no wider and no taller than the requested one whose tiles stay within
 // the radius limit (SARBP_PREC_REF only).  A grid too coarse for every shape
-// keeps the request, which plan_args then refuses.
-sarbp_tuning fit_tuning(sarbp_tuning t, int precision, const sarbp_grid* g, const sarbp_params* p,
-                        const double* h_pos) {
-  if (precision != SARBP_PREC_REF) return t;
+// runs the double-float differential instead, with one column per lane and
+// the most rows per lane (at most the request's) whose range window fits LDS.
+Fitted fit_tuning(sarbp_tuning t, int precision, const sarbp_grid* g, const sarbp_params* p,
+                  const double* h_pos) {
+  if (precision != SARBP_PREC_REF) return Fitted{t, precision};
   auto fits = [&](int px, int py) {
     return tile_radius(g, px, py) <= radius_limit(min_range(g, px, py, h_pos, p->n_pulses), p->dr);
   };
-  if (fits(t.cols_per_lane, t.rows_per_lane)) return t;
+  if (fits(t.cols_per_lane, t.rows_per_lane)) return Fitted{t, kRef};
   int best_px = 0, best_py = 0;
   for (int px = t.cols_per_lane; px >= 1; px /= 2)
     for (int py = t.rows_per_lane; py >= 2; py /= 2)
@@ -321,8 +325,18 @@ sarbp_tuning fit_tuning(sarbp_tuning t, int precision, const sarbp_grid* g, cons
   if (best_px) {
     t.cols_per_lane = best_px;
     t.rows_per_lane = best_py;
+    return Fitted{t, kRef};
   }
-  return t;
+  t.cols_per_lane = 1;
+  int py = std::max(2, std::min(8, t.rows_per_lane));
+  while (py > 2) {
+    int kinterp = 0;
+    const int win = plan_window(g, p, 1, py, h_pos, &kinterp);
+    if (win > 0 && lds_layout(kinterp, win, 1).total <= kLdsBudget) break;
+    py /= 2;
+  }
+  t.rows_per_lane = py;
+  return Fitted{t, kDF};
 }
 
 // Range extent (metres, half-width) of one tile seen from any pulse, and the
@@ -407,34 +421,54 @@ int size_class(const sarbp_grid* g, const sarbp_params* p, const double* h_pos) 
   return c;
 }
 
+// LDS range window (samples) of a tile of px x py pixels per lane and the
+// interpolation kernel (kernel_interp) it runs; 0 if the window exceeds 8192
+// samples.  Sinc8 runs half bins unless the tile is coarse against the range
+// bins: per pixel-pulse the half-bin kernel issues ~4 VALU instructions fewer
+// in the pixel loop and ~0.6 * 76 / (tile pixels) more per window position
+// (about 60% of the window's positions are touched per pulse).
+int plan_window(const sarbp_grid* g, const sarbp_params* p, int px, int py, const double* h_pos,
+                int* kinterp) {
+  const Extent ext = tile_extent(g, px, py, h_pos, p->n_pulses);
+  // half window in samples: extent + sinc taps (3 left / 4 right) + 2 margin
+  const double half_bins = std::ceil(ext.half_m / p->dr) + 6.0;
+  *kinterp = p->interp;
+  if (!(half_bins <= 4096.0)) return 0;
+  const int win = ((2 * (int)half_bins + 2) + 3) & ~3;  // multiple of 4 samples
+  if (p->interp == SARBP_INTERP_SINC8) {
+    const double tile_px = (double)kTileX * px * kWaves * py;
+    *kinterp = 0.6 * 76.0 * (win - 7) < 4.0 * tile_px ? kSinc8 : kSinc8W;
+  }
+  return win;
+}
+
+// Inverse quantum 2^e of the DF direction cosines: every alpha_h di + beta_h dj
+// of a tile (|di| <= tx / 2, |dj| <= ty / 2, |alpha_h| <= |ux|, |beta_h| <= |uy|)
+// is then an integer multiple of 2^-e below 2^(23 - e), exact in FP32.
+double df_quant(const sarbp_grid* g, int px, int py) {
+  const double span = norm3(g->ux) * (kTileX * px / 2) + norm3(g->uy) * (kWaves * py / 2) + 1.0;
+  return std::ldexp(1.0, 23 - (int)std::ceil(std::log2(span)));
+}
+
 // Fills a launch descriptor.  Caller holds ctx->mu and the device guard.
 sarbp_status plan_args(const double* h_pos, const sarbp_grid* g,
-                       const sarbp_params* p, const sarbp_tuning& tune, BpArgs* a,
+                       const sarbp_params* p, const sarbp_tuning& tune, int kprec, BpArgs* a,
                        sarbp_plan_info* info) {
   const int px = tune.cols_per_lane, py = tune.rows_per_lane;
-  if (!tile_shape_supported(p->precision, px, py))
+  if (!tile_shape_supported(kprec, px, py))
     return fail(SARBP_ERR_INVALID_ARGUMENT, "unsupported tile shape: %d cols x %d rows per lane", px, py);
   const Extent ext = tile_extent(g, px, py, h_pos, p->n_pulses);
   const double rlim = radius_limit(min_range(g, px, py, h_pos, p->n_pulses), p->dr);
-  if (p->precision == SARBP_PREC_REF && ext.dmax > rlim)
+  if (kprec == kRef && ext.dmax > rlim)
     return fail(SARBP_ERR_UNSUPPORTED,
                 "tile of %d x %d pixels spans %.1f m from its centre (limit %.0f m): use fewer pixels per lane",
                 kTileX * px, kWaves * py, ext.dmax, rlim);
-  // half window in samples: extent + sinc taps (3 left / 4 right) + 2 margin
-  const double half_bins = std::ceil(ext.half_m / p->dr) + 6.0;
-  if (half_bins > 4096.0)
-    return fail(SARBP_ERR_UNSUPPORTED, "range window of a tile too long (%.0f samples): pixel spacing is too coarse for dr", 2 * half_bins);
-  const int win = ((2 * (int)half_bins + 2) + 3) & ~3;  // multiple of 4 samples
-  int P = std::max(1, tune.pulses_per_stage);
-  // Sinc8 kernel: half bins unless the tile is coarse against the range bins.
-  // Per pixel-pulse the half-bin kernel issues ~4 VALU instructions fewer in
-  // the pixel loop and ~0.6 * 76 / (tile pixels) more per window position
-  // (about 60% of the window's positions are touched per pulse).
   int kinterp = p->interp;
-  if (p->interp == SARBP_INTERP_SINC8) {
-    const double tile_px = (double)kTileX * px * kWaves * py;
-    kinterp = 0.6 * 76.0 * (win - 7) < 4.0 * tile_px ? kSinc8 : kSinc8W;
-  }
+  const int win = plan_window(g, p, px, py, h_pos, &kinterp);
+  if (win == 0)
+    return fail(SARBP_ERR_UNSUPPORTED, "range window of a tile too long (%.0f m): pixel spacing is too coarse for dr",
+                2 * ext.half_m);
+  int P = std::max(1, tune.pulses_per_stage);
   while (P > 1 && lds_layout(kinterp, win, P).total > kLdsBudget) --P;
   if (lds_layout(kinterp, win, P).total > kLdsBudget)
     return fail(SARBP_ERR_UNSUPPORTED, "range window of %d samples does not fit LDS", win);
@@ -451,6 +485,8 @@ sarbp_status plan_args(const double* h_pos, const sarbp_grid* g,
   a->k_rev = 2.0 * p->fc / p->c;
   a->inv_dr_f = (float)a->inv_dr;
   a->k_rev_f = (float)a->k_rev;
+  a->k_rev_lo = (float)(a->k_rev - (double)a->k_rev_f);
+  a->s_quant = df_quant(g, px, py);
   a->n_pulses = p->n_pulses;
   a->n_samples = p->n_samples;
   a->nx = g->nx;
@@ -467,6 +503,7 @@ sarbp_status plan_args(const double* h_pos, const sarbp_grid* g,
   a->dmax = ext.dmax;
   a->order_tol = 1e-5;
   a->kernel_interp = kinterp;
+  a->kernel_prec = kprec;
   if (kinterp == kSinc8) sinc8_poly(p->kaiser_beta, a->sinc_poly);
   if (kinterp == kSinc8W) sinc8_poly_whole(p->kaiser_beta, a->sinc_poly);
   if (info) {
@@ -480,6 +517,7 @@ sarbp_status plan_args(const double* h_pos, const sarbp_grid* g,
     info->cols_per_lane = px;
     info->lds_bytes = (int32_t)lds_layout(kinterp, win, P).total;
     info->sinc8_halves = kinterp == kSinc8 ? 2 : (kinterp == kSinc8W ? 1 : 0);
+    info->precision = kprec;
     const double rm = ext.rmin > ext.dmax ? ext.rmin - ext.dmax : 0.0;
     const double d2 = ext.dmax * ext.dmax;
     info->range_order = rm > 0 && 2 * kPi * a->k_rev * 0.625 * d2 * d2 / (rm * rm * rm) <= a->order_tol ? 1 : 0;
@@ -490,7 +528,7 @@ sarbp_status plan_args(const double* h_pos, const sarbp_grid* g,
 
 sarbp_status launch(const BpArgs& a, const sarbp_params* p, const sarbp_tuning& t, hipStream_t s) {
   if (a.nx == 0 || a.ny == 0) return SARBP_OK;
-  hipError_t e = launch_plain(a, a.kernel_interp, p->precision, p->dtype, t.cols_per_lane, t.rows_per_lane, s);
+  hipError_t e = launch_plain(a, a.kernel_interp, a.kernel_prec, p->dtype, t.cols_per_lane, t.rows_per_lane, s);
   if (e != hipSuccess) return hip_fail(e, "BP kernel launch");
   return SARBP_OK;
 }
@@ -560,7 +598,7 @@ void sarbp_destroy(sarbp_context* c) {
 sarbp_status sarbp_set_tuning(sarbp_context* ctx, int32_t interp, int32_t precision, int32_t dtype,
                               const sarbp_tuning* t) {
   if (!ctx || !t) return fail(SARBP_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (interp < 0 || interp > 2 || precision < 0 || precision > 2 || dtype < 0 || dtype > 1)
+  if (interp < 0 || interp > 2 || precision < 0 || precision > 3 || dtype < 0 || dtype > 1)
     return fail(SARBP_ERR_INVALID_ARGUMENT, "bad (interp, precision, dtype)");
   std::lock_guard<std::mutex> lk(ctx->mu);
   for (int c = 0; c <= kAllClasses; ++c) ctx->tuning.erase(tuning_key(interp, precision, dtype, c));
@@ -580,7 +618,7 @@ int32_t sarbp_size_class(const sarbp_grid* grid, const sarbp_params* params, con
 sarbp_status sarbp_set_class_tuning(sarbp_context* ctx, int32_t interp, int32_t precision, int32_t dtype,
                                     int32_t size_cls, const sarbp_tuning* t) {
   if (!ctx || !t) return fail(SARBP_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (interp < 0 || interp > 2 || precision < 0 || precision > 2 || dtype < 0 || dtype > 1 || size_cls < 0 ||
+  if (interp < 0 || interp > 2 || precision < 0 || precision > 3 || dtype < 0 || dtype > 1 || size_cls < 0 ||
       size_cls >= SARBP_SIZE_CLASSES)
     return fail(SARBP_ERR_INVALID_ARGUMENT, "bad (interp, precision, dtype, size class)");
   std::lock_guard<std::mutex> lk(ctx->mu);
@@ -622,16 +660,16 @@ sarbp_status sarbp_backproject(sarbp_context* ctx, const void* d_profiles, const
   std::lock_guard<std::mutex> lk(ctx->mu);
   DeviceGuard dg(ctx->device);
   if (dg.err != hipSuccess) return hip_fail(dg.err, "hipSetDevice");
-  const sarbp_tuning tune =
+  const Fitted fit =
       fit_tuning(get_tuning(ctx, params->interp, params->precision, params->dtype, size_class(grid, params, nullptr)),
                  params->precision, grid, params, nullptr);
   BpArgs a;
-  st = plan_args(nullptr, grid, params, tune, &a, nullptr);
+  st = plan_args(nullptr, grid, params, fit.t, fit.prec, &a, nullptr);
   if (st != SARBP_OK) return st;
   a.prof = d_profiles;
   a.pos = d_pos;
   a.img = reinterpret_cast<float2*>(d_image);
-  return launch(a, params, tune, (hipStream_t)stream);
+  return launch(a, params, fit.t, (hipStream_t)stream);
 }
 
 sarbp_status sarbp_backproject_sync(sarbp_context* ctx, const void* d_profiles, const double* d_pos,
@@ -661,11 +699,12 @@ sarbp_status sarbp_plan_create(sarbp_context* ctx, const double* h_pos, const sa
     delete pl;
     return hip_fail(dg.err, "hipSetDevice");
   }
-  const sarbp_tuning tune =
+  const Fitted fit =
       fit_tuning(get_tuning(ctx, params->interp, params->precision, params->dtype, size_class(grid, params, h_pos)),
                  params->precision, grid, params, h_pos);
+  const sarbp_tuning& tune = fit.t;
   std::memset(&pl->info, 0, sizeof pl->info);
-  st = plan_args(h_pos, grid, params, tune, &pl->args, &pl->info);
+  st = plan_args(h_pos, grid, params, tune, fit.prec, &pl->args, &pl->info);
   if (st != SARBP_OK) {
     delete pl;
     return st;
@@ -754,17 +793,18 @@ sarbp_status sarbp_backproject_batch(sarbp_context* ctx, const sarbp_job* jobs, 
   // launch's; jobs that share a shape share one persistent launch.
   struct Group {
     sarbp_tuning tune;
-    int kinterp;
+    int kinterp, kprec;
     std::vector<JobDesc> desc;
   };
   std::vector<Group> groups;
   for (int32_t j : order) {
     const sarbp_job& jb = jobs[j];
     if (jb.grid.nx == 0 || jb.grid.ny == 0) continue;
-    const sarbp_tuning t = fit_tuning(get_tuning(ctx, p0.interp, p0.precision, p0.dtype, size_class(&jb.grid, &jb.params, jb.h_pos)),
-                                      p0.precision, &jb.grid, &jb.params, jb.h_pos);
+    const Fitted f = fit_tuning(get_tuning(ctx, p0.interp, p0.precision, p0.dtype, size_class(&jb.grid, &jb.params, jb.h_pos)),
+                                p0.precision, &jb.grid, &jb.params, jb.h_pos);
+    const sarbp_tuning& t = f.t;
     JobDesc d;
-    sarbp_status st = plan_args(jb.h_pos, &jb.grid, &jb.params, t, &d.a, nullptr);
+    sarbp_status st = plan_args(jb.h_pos, &jb.grid, &jb.params, t, f.prec, &d.a, nullptr);
     if (st != SARBP_OK) return fail(st, "job %d: %s", j, g_detail);
     d.a.prof = jb.d_profiles;
     d.a.pos = jb.d_pos;
@@ -773,9 +813,9 @@ sarbp_status sarbp_backproject_batch(sarbp_context* ctx, const sarbp_job* jobs, 
     size_t g = 0;
     while (g < groups.size() && !(groups[g].tune.cols_per_lane == t.cols_per_lane &&
                                   groups[g].tune.rows_per_lane == t.rows_per_lane &&
-                                  groups[g].kinterp == d.a.kernel_interp))
+                                  groups[g].kinterp == d.a.kernel_interp && groups[g].kprec == f.prec))
       ++g;
-    if (g == groups.size()) groups.push_back(Group{t, d.a.kernel_interp, {}});
+    if (g == groups.size()) groups.push_back(Group{t, d.a.kernel_interp, f.prec, {}});
     groups[g].desc.push_back(d);
   }
   if (groups.size() > sizeof ctx->batch / sizeof ctx->batch[0])
@@ -783,7 +823,7 @@ sarbp_status sarbp_backproject_batch(sarbp_context* ctx, const sarbp_job* jobs, 
   for (size_t g = 0; g < groups.size(); ++g) {
     if (!ctx->batch[g] && !(ctx->batch[g] = new (std::nothrow) BatchExec))
       return fail(SARBP_ERR_OUT_OF_MEMORY, "batch executor");
-    const sarbp_status st = ctx->batch[g]->run(groups[g].desc, groups[g].kinterp, p0.precision, p0.dtype,
+    const sarbp_status st = ctx->batch[g]->run(groups[g].desc, groups[g].kinterp, groups[g].kprec, p0.dtype,
                                                groups[g].tune, ctx->n_cu, (hipStream_t)stream);
     if (st != SARBP_OK) return st;
   }
@@ -824,7 +864,7 @@ sarbp_status sarbp_backproject_batch_noalloc(sarbp_context* ctx, const sarbp_job
   // in the group of its (tile shape, sinc8 kernel).
   struct Group {
     sarbp_tuning tune;
-    int kinterp, n;
+    int kinterp, kprec, n;
     size_t shm;
     int64_t items;
   } groups[kNoAllocGroups];
@@ -833,10 +873,11 @@ sarbp_status sarbp_backproject_batch_noalloc(sarbp_context* ctx, const sarbp_job
   for (int32_t r = 0; r < n_jobs; ++r) {
     const sarbp_job& jb = jobs[order[r]];
     if (jb.grid.nx == 0 || jb.grid.ny == 0) continue;
-    const sarbp_tuning t = fit_tuning(get_tuning(ctx, p0.interp, p0.precision, p0.dtype, size_class(&jb.grid, &jb.params, jb.h_pos)),
-                                      p0.precision, &jb.grid, &jb.params, jb.h_pos);
+    const Fitted f = fit_tuning(get_tuning(ctx, p0.interp, p0.precision, p0.dtype, size_class(&jb.grid, &jb.params, jb.h_pos)),
+                                p0.precision, &jb.grid, &jb.params, jb.h_pos);
+    const sarbp_tuning& t = f.t;
     JobDesc& d = g_noalloc_plan[n_planned];
-    st = plan_args(jb.h_pos, &jb.grid, &jb.params, t, &d.a, nullptr);
+    st = plan_args(jb.h_pos, &jb.grid, &jb.params, t, f.prec, &d.a, nullptr);
     if (st != SARBP_OK) return fail(st, "job %d: %s", order[r], g_detail);
     d.a.prof = jb.d_profiles;
     d.a.pos = jb.d_pos;
@@ -844,12 +885,12 @@ sarbp_status sarbp_backproject_batch_noalloc(sarbp_context* ctx, const sarbp_job
     int g = 0;
     while (g < n_groups && !(groups[g].tune.cols_per_lane == t.cols_per_lane &&
                              groups[g].tune.rows_per_lane == t.rows_per_lane &&
-                             groups[g].kinterp == d.a.kernel_interp))
+                             groups[g].kinterp == d.a.kernel_interp && groups[g].kprec == f.prec))
       ++g;
     if (g == n_groups) {
       if (n_groups == kNoAllocGroups)
         return fail(SARBP_ERR_UNSUPPORTED, "more than %d tile shapes in one batch", kNoAllocGroups);
-      groups[n_groups++] = Group{t, d.a.kernel_interp, 0, 0, 0};
+      groups[n_groups++] = Group{t, d.a.kernel_interp, f.prec, 0, 0, 0};
     }
     Group& G = groups[g];
     d.item_begin = G.items;
@@ -888,12 +929,12 @@ sarbp_status sarbp_backproject_batch_noalloc(sarbp_context* ctx, const sarbp_job
   if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(scratch)");
   for (int g = 0; g < n_groups; ++g) {
     const Group& G = groups[g];
-    const int per_cu = batch_blocks_per_cu(G.kinterp, p0.precision, p0.dtype, G.tune.cols_per_lane,
+    const int per_cu = batch_blocks_per_cu(G.kinterp, G.kprec, p0.dtype, G.tune.cols_per_lane,
                                            G.tune.rows_per_lane, G.shm);
     e = launch_batch(reinterpret_cast<const JobDesc*>(dev + table_off[g]),
                      reinterpret_cast<const BatchHeader*>(dev + g * kNoAllocSlot),
                      reinterpret_cast<unsigned long long*>(dev + g * kNoAllocSlot + kNoAllocCounter),
-                     G.kinterp, p0.precision, p0.dtype, G.tune.cols_per_lane, G.tune.rows_per_lane, G.shm,
+                     G.kinterp, G.kprec, p0.dtype, G.tune.cols_per_lane, G.tune.rows_per_lane, G.shm,
                      std::max(1, ctx->n_cu * per_cu), s);
     if (e != hipSuccess) return hip_fail(e, "allocation-free batched launch");
   }
@@ -938,7 +979,7 @@ sarbp_status sarbp_autotune(sarbp_context* ctx, const void* d_profiles, const do
   }
   auto time_one = [&](const sarbp_tuning& t, double* ms, int* eff_p) -> sarbp_status {
     BpArgs a;
-    sarbp_status r = plan_args(h_pos, &crop, &p, t, &a, nullptr);
+    sarbp_status r = plan_args(h_pos, &crop, &p, t, p.precision, &a, nullptr);
     if (r != SARBP_OK) return r;
     *eff_p = a.pulses_per_stage;
     a.prof = d_profiles;
@@ -981,7 +1022,7 @@ sarbp_status sarbp_autotune(sarbp_context* ctx, const void* d_profiles, const do
       t.rows_per_lane = sh[1];
       t.pulses_per_stage = P;
       BpArgs probe;
-      if (plan_args(h_pos, &crop, &p, t, &probe, nullptr) != SARBP_OK) continue;
+      if (plan_args(h_pos, &crop, &p, t, p.precision, &probe, nullptr) != SARBP_OK) continue;
       // fewer than 4 pulses per stage only pay where 4 would leave fewer
       // than 4 workgroups per CU (40 KiB of LDS each)
       if (P < 4 && lds_layout(probe.kernel_interp, probe.win, 4).total <= 40 * 1024) continue;

@@ -30,8 +30,13 @@
 //  30  v_ldexp_f32 (VOP2)                31  v_ashrrev_i32 (VOP2)
 // The BP pixel loop issues 2 transcendentals per 28.4 VALU instructions, between
 // kinds 2 and 3, and mixes VOP3 and VOP1/VOP2 encodings.
+//
+// sarbp_rotation_probe evaluates the BP kernels' own phase rotation
+// (sin_rev / cos_rev / rotate_acc of sarbp_rotation.h) on given phases, so a
+// test can hold it against FP64 sin(2 pi x) and cos(2 pi x).
 #include "sarbp.h"
 #include "sarbp_host.h"
+#include "sarbp_rotation.h"
 
 namespace sarbp {
 namespace {
@@ -133,6 +138,29 @@ __global__ __launch_bounds__(kProbeThreads) void valu_probe_kernel(float* __rest
 }  // namespace sarbp
 
 using namespace sarbp;
+
+// out[2 i], out[2 i + 1] = the rotation of the sample 1 + 0j by exp(+j 2 pi
+// ph[i]) as the BP kernels accumulate it: (cos, sin) of ph[i] revolutions.
+__global__ void rotation_probe_kernel(const float* __restrict__ ph, float* __restrict__ out, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float ar = 0.0f, ai = 0.0f;
+  sarbp::rotate_acc(1.0f, 0.0f, ph[i], ar, ai);
+  out[2 * i] = ar;
+  out[2 * i + 1] = ai;
+}
+
+extern "C" sarbp_status sarbp_rotation_probe(const float* d_phase, float* d_out, int64_t n, void* stream) {
+  if (n < 0 || (n > 0 && (!d_phase || !d_out))) return fail(SARBP_ERR_INVALID_ARGUMENT, "NULL buffer or n < 0");
+  if (n == 0) return SARBP_OK;
+  if (n > ((int64_t)1 << 31) * kProbeThreads / 2)
+    return fail(SARBP_ERR_INVALID_ARGUMENT, "n = %lld phases: too many", (long long)n);
+  const unsigned blocks = (unsigned)((n + kProbeThreads - 1) / kProbeThreads);
+  hipLaunchKernelGGL(rotation_probe_kernel, dim3(blocks), dim3(kProbeThreads), 0, (hipStream_t)stream, d_phase,
+                     d_out, n);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? SARBP_OK : hip_fail(e, "rotation probe launch");
+}
 
 extern "C" sarbp_status sarbp_valu_probe(int32_t kind, int32_t n_blocks, int32_t iters, float* d_out,
                                          void* stream) {

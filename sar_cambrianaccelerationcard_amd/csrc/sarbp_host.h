@@ -45,14 +45,25 @@ int size_class(const sarbp_grid* g, const sarbp_params* p, const double* h_pos);
 sarbp_tuning default_tuning(int interp, int cls, int precision);
 // SARBP_ERR_INVALID_ARGUMENT unless the shape is compiled and 1 <= P <= 256.
 sarbp_status check_tuning(int precision, const sarbp_tuning* t);
+// A tuning fitted to one grid and the arithmetic (kRef, kF64, kF32, kDF) its
+// kernels run.
+struct Fitted {
+  sarbp_tuning t;
+  int prec;
+};
 // The tuning with its tile shape reduced, if needed, to the largest compiled
 // shape within it whose tiles fit the REF tile-radius limit on this grid
-// (h_pos, if given, bounds the range from below and relaxes the limit).
-sarbp_tuning fit_tuning(sarbp_tuning t, int precision, const sarbp_grid* g, const sarbp_params* p,
-                        const double* h_pos);
-// Fills a launch descriptor (device pointers left NULL) for one image.
+// (h_pos, if given, bounds the range from below and relaxes the limit).  A
+// REF job that no shape fits runs the double-float differential (kDF, no
+// radius limit) with one column per lane.
+Fitted fit_tuning(sarbp_tuning t, int precision, const sarbp_grid* g, const sarbp_params* p,
+                  const double* h_pos);
+// Fills a launch descriptor (device pointers left NULL) for one image, with
+// kernel arithmetic kprec.
 sarbp_status plan_args(const double* h_pos, const sarbp_grid* g, const sarbp_params* p,
-                       const sarbp_tuning& tune, BpArgs* a, sarbp_plan_info* info);
+                       const sarbp_tuning& tune, int kprec, BpArgs* a, sarbp_plan_info* info);
+// Launches the plain kernel of a planned descriptor (its kernel_interp and
+// kernel_prec, the dtype of p).
 sarbp_status launch(const BpArgs& a, const sarbp_params* p, const sarbp_tuning& t, hipStream_t s);
 
 // Batched persistent launch replayed from a captured hipGraph.  The graph is

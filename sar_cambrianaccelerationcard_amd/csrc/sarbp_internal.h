@@ -34,7 +34,7 @@ enum { kNearest = 0, kLinear = 1, kSinc8 = 2, kSinc8W = 3 };
 __host__ __device__ constexpr int sinc_rows(int interp) {
   return interp == kSinc8 ? kSincHalves * kSincCoef : (interp == kSinc8W ? kSincCoefW : 0);
 }
-enum { kRef = 0, kF64 = 1, kF32 = 2 };
+enum { kRef = 0, kF64 = 1, kF32 = 2, kDF = 3 };
 
 // Everything one plain launch needs; passed by value as the kernel argument.
 struct BpArgs {
@@ -45,7 +45,11 @@ struct BpArgs {
   double r0, dr, inv_dr, k_rev;
   double dmax;       // max |pixel - tile reference pixel| over a tile, m
   double order_tol;  // truncation tolerance of the range expansion, rad
+  double s_quant;    // kDF: 2^e, the inverse quantum of the direction cosines
+                     // e.ux, e.uy, so that their products with integer pixel
+                     // offsets, and the sums of those, are exact in FP32
   float inv_dr_f, k_rev_f;
+  float k_rev_lo;    // k_rev - k_rev_f (kDF's double-float k)
   int32_t n_pulses, n_samples, nx, ny;
   int32_t tiles_x, tiles_y;
   int32_t win;   // LDS window samples per pulse
@@ -54,6 +58,7 @@ struct BpArgs {
   int32_t pulse_begin, pulse_end;  // pulses [begin, end) are summed
   int32_t accumulate;
   int32_t kernel_interp;  // kNearest, kLinear, kSinc8 or kSinc8W
+  int32_t kernel_prec;    // kRef, kF64, kF32 or kDF: the arithmetic launched
   // kSinc8: C[(h * 6 + k) * 8 + t]; kSinc8W: C[k * 8 + t]
   float sinc_poly[kSincHalves * kSincCoef * kSincTaps];
 };
@@ -117,7 +122,7 @@ __host__ __device__ inline LdsLayout lds_layout(int interp, int win, int P) {
 }
 
 // Tile shapes (PX columns x PY rows per lane) with compiled kernels: every
-// shape for SARBP_PREC_REF, the PX = 1 shapes for the F64 and F32 modes.
+// shape for SARBP_PREC_REF, the PX = 1 shapes for the F64, F32 and DF modes.
 inline bool tile_shape_supported(int precision, int px, int py) {
   const bool x1 = px == 1 && (py == 2 || py == 4 || py == 8);
   if (precision != kRef) return x1;

@@ -293,7 +293,7 @@ void sarbp_server::sched_loop() {
     // chunks grouped by tile shape and kernel: one batched launch per group
     struct Group {
       sarbp_tuning tune;
-      int kinterp;
+      int kinterp, kprec;
       std::vector<JobDesc> desc;
     };
     std::vector<Group> groups;
@@ -335,9 +335,10 @@ void sarbp_server::sched_loop() {
         size_t g = 0;
         while (g < groups.size() && !(groups[g].tune.cols_per_lane == j.tune.cols_per_lane &&
                                       groups[g].tune.rows_per_lane == j.tune.rows_per_lane &&
-                                      groups[g].kinterp == j.args.kernel_interp))
+                                      groups[g].kinterp == j.args.kernel_interp &&
+                                      groups[g].kprec == j.args.kernel_prec))
           ++g;
-        if (g == groups.size()) groups.push_back(Group{j.tune, j.args.kernel_interp, {}});
+        if (g == groups.size()) groups.push_back(Group{j.tune, j.args.kernel_interp, j.args.kernel_prec, {}});
         groups[g].desc.push_back(d);
         j.next_pulse = d.a.pulse_end;
         j.started = true;
@@ -361,7 +362,7 @@ void sarbp_server::sched_loop() {
     if (e == hipSuccess) e = hipEventRecord(ep.ev0, s_compute);
     sarbp_status st = e == hipSuccess ? SARBP_OK : hip_fail(e, "epoch events");
     for (size_t g = 0; st == SARBP_OK && g < groups.size(); ++g)
-      st = g < 8 ? exec[k & 1][g].run(groups[g].desc, groups[g].kinterp, cfg.precision, cfg.dtype,
+      st = g < 8 ? exec[k & 1][g].run(groups[g].desc, groups[g].kinterp, groups[g].kprec, cfg.dtype,
                                       groups[g].tune, n_cu, s_compute)
                  : fail(SARBP_ERR_UNSUPPORTED, "more than 8 tile shapes in an epoch");
     if (st == SARBP_OK && (e = hipEventRecord(ep.ev1, s_compute)) != hipSuccess) st = hip_fail(e, "hipEventRecord");
@@ -448,7 +449,7 @@ extern "C" {
 sarbp_status sarbp_server_create(const sarbp_server_config* cfg, sarbp_server** out) {
   if (!cfg || !out) return fail(SARBP_ERR_INVALID_ARGUMENT, "NULL argument");
   *out = nullptr;
-  if (cfg->interp < 0 || cfg->interp > 2 || cfg->precision < 0 || cfg->precision > 2 || cfg->dtype < 0 ||
+  if (cfg->interp < 0 || cfg->interp > 2 || cfg->precision < 0 || cfg->precision > 3 || cfg->dtype < 0 ||
       cfg->dtype > 1)
     return fail(SARBP_ERR_INVALID_ARGUMENT, "bad (interp, precision, dtype)");
   if (cfg->max_active_jobs < 0 || cfg->pulses_per_epoch < 0 || cfg->epoch_watchdog_s < 0)
@@ -537,8 +538,9 @@ sarbp_status submit_job(sarbp_server* s, const sarbp_request* req, const sarbp_r
     std::lock_guard<std::mutex> lk(s->mu);
     base = s->tune[size_class(&req->grid, &p, req->h_pos)];
   }
-  j->tune = fit_tuning(base, p.precision, &req->grid, &p, req->h_pos);
-  if ((st = plan_args(req->h_pos, &req->grid, &p, j->tune, &j->args, nullptr)) != SARBP_OK) return st;
+  const Fitted fit = fit_tuning(base, p.precision, &req->grid, &p, req->h_pos);
+  j->tune = fit.t;
+  if ((st = plan_args(req->h_pos, &req->grid, &p, j->tune, fit.prec, &j->args, nullptr)) != SARBP_OK) return st;
   j->prof_bytes = (size_t)p.n_pulses * p.n_samples * (p.dtype == SARBP_DTYPE_C64 ? 8 : 4);
   j->pos_bytes = (size_t)p.n_pulses * 24;
   j->img_bytes = npix * 8;

@@ -70,20 +70,28 @@ class DeviceScene:
 
 def backproject(ctx: native.Context, ds: DeviceScene, *, interp: Optional[str] = None,
                 precision: str = "ref", dtype: str = "c64", image=None,
-                accumulate: bool = False, plan: bool = False, sync: bool = True):
+                accumulate: bool = False, plan: bool = False, sync: bool = True,
+                grid: Optional[dict] = None):
     """Runs one BP through the C ABI on torch's current stream; returns the
-    device image tensor."""
+    device image tensor.  `grid` (a scene-style grid dict) replaces the
+    scene's own image grid."""
     torch = require_gpu()
     sc = ds.scene
-    img = ds.new_image() if image is None else image
+    if grid is None:
+        ng = sc.native_grid()
+        img = ds.new_image() if image is None else image
+    else:
+        ng = native.make_grid(grid["origin"], grid["ux"], grid["uy"], grid["nx"], grid["ny"])
+        img = (torch.zeros((grid["ny"], grid["nx"]), dtype=torch.complex64, device=ds.device)
+               if image is None else image)
     params = sc.native_params(interp=interp, precision=precision, dtype=dtype,
                               accumulate=accumulate)
     stream = torch.cuda.current_stream(ds.device).cuda_stream
     if plan:
-        pl = ctx.plan(sc.pos, sc.native_grid(), params)
+        pl = ctx.plan(sc.pos, ng, params)
         pl.execute(ds.profiles_ptr(dtype), ds.pos.data_ptr(), img.data_ptr(), stream)
     else:
-        ctx.backproject(ds.profiles_ptr(dtype), ds.pos.data_ptr(), sc.native_grid(), params,
+        ctx.backproject(ds.profiles_ptr(dtype), ds.pos.data_ptr(), ng, params,
                         img.data_ptr(), stream)
     if sync:
         torch.cuda.synchronize(ds.device)

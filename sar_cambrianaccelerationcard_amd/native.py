@@ -28,11 +28,11 @@ ERR_TIMEOUT = 9
 ERR_SHUTDOWN = 10
 
 INTERP_NEAREST, INTERP_LINEAR, INTERP_SINC8 = 0, 1, 2
-PREC_REF, PREC_F64, PREC_F32 = 0, 1, 2
+PREC_REF, PREC_F64, PREC_F32, PREC_DF = 0, 1, 2, 3
 DTYPE_C64, DTYPE_BF16X2 = 0, 1
 
 INTERP_NAMES = {"nearest": INTERP_NEAREST, "linear": INTERP_LINEAR, "sinc8": INTERP_SINC8}
-PREC_NAMES = {"ref": PREC_REF, "f64": PREC_F64, "f32": PREC_F32}
+PREC_NAMES = {"ref": PREC_REF, "f64": PREC_F64, "f32": PREC_F32, "df": PREC_DF}
 DTYPE_NAMES = {"c64": DTYPE_C64, "bf16x2": DTYPE_BF16X2}
 
 
@@ -56,7 +56,7 @@ class PlanInfo(C.Structure):
                 ("rows_per_lane", C.c_int32), ("cols_per_lane", C.c_int32),
                 ("range_order", C.c_int32),
                 ("lds_bytes", C.c_int32), ("max_phase_err_rad", C.c_double),
-                ("sinc8_halves", C.c_int32), ("reserved", C.c_int32)]
+                ("sinc8_halves", C.c_int32), ("precision", C.c_int32)]
 
 
 class Tuning(C.Structure):
@@ -221,6 +221,7 @@ _SIGS = {
     "sarbp_omega_k": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.POINTER(WkParams),
                                 C.c_void_p, C.c_void_p]),
     "sarbp_valu_probe": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]),
+    "sarbp_rotation_probe": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     "sarbp_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Grid),
                                  C.POINTER(Params), C.c_void_p, C.c_size_t, C.c_int32, C.c_void_p,
                                  C.POINTER(AutotuneResult)]),
@@ -504,6 +505,13 @@ def size_class(grid: Grid, params: Params, h_pos=None) -> int:
         h_pos = np.ascontiguousarray(h_pos, dtype=np.float64)
         hp = h_pos.ctypes.data_as(C.c_void_p)
     return int(load().sarbp_size_class(C.byref(grid), C.byref(params), hp))
+
+
+def rotation_probe(d_phase: int, d_out: int, n: int, stream: int = 0) -> None:
+    """Enqueues the BP kernels' phase rotation on n FP32 phases (revolutions):
+    d_out holds n (cos, sin) float pairs (sarbp_rotation_probe)."""
+    check(load().sarbp_rotation_probe(C.c_void_p(d_phase), C.c_void_p(d_out), C.c_int64(n),
+                                      C.c_void_p(stream)), "sarbp_rotation_probe")
 
 
 def valu_probe(kind: int, n_blocks: int, iters: int, d_out: int, stream: int = 0) -> None:

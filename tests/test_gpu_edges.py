@@ -133,16 +133,66 @@ def test_tile_shape_falls_back_within_radius_limit(base):
     assert M.rel_l2(img, O.backproject(sc.profiles, sc.pos, sc.grid, o)) <= 1e-4
 
 
-def test_grid_too_coarse_for_every_tile_is_refused(ctx, base):
-    """At 1.6 m pixels even the smallest tile (64 x 8) spans 51 m."""
+def _scaled_grid(sc, f):
+    g = dict(sc.grid)
+    g["ux"] = [v * f for v in sc.grid["ux"]]
+    g["uy"] = [v * f for v in sc.grid["uy"]]
+    return g, native.make_grid(g["origin"], g["ux"], g["uy"], g["nx"], g["ny"])
+
+
+@pytest.mark.parametrize("interp,factor", [("sinc8", 3.2), ("linear", 8.0)])
+def test_grid_too_coarse_for_ref_runs_double_float(ctx, base, interp, factor):
+    """At 1.6 m pixels even the smallest tile (64 x 8) spans 51 m, beyond the
+    REF differential's radius limit; at 4 m (linear) 129 m.  The job runs the
+    double-float differential instead -- planned, unplanned and batched --
+    and matches the FP64 oracle."""
+    import torch
     sc, ds = base
-    g = sc.native_grid()
-    for k in range(3):
-        g.ux[k] *= 3.2
-        g.uy[k] *= 3.2
-    with pytest.raises(native.SarbpError) as ei:
-        _run(ctx, ds, g, sc.native_params(interp="sinc8"))
-    assert ei.value.status == native.ERR_UNSUPPORTED
+    g, ng = _scaled_grid(sc, factor)
+    params = sc.native_params(interp=interp)
+    info = ctx.plan(sc.pos, ng, params).info()
+    assert info["precision"] == native.PREC_DF and info["cols_per_lane"] == 1, info
+    ref = _oracle(sc, interp, g)
+    img = _run(ctx, ds, ng, params).cpu().numpy()  # unplanned: no host positions
+    err = M.rel_l2(img, ref)
+    print(f"  {interp} at {0.5 * factor} m: DF fallback rel_l2={err:.2e} ({info['tile_nx']}x{info['tile_ny']} tiles)",
+          flush=True)
+    assert err <= 1e-4
+    planned = D.backproject(ctx, ds, interp=interp, plan=True, grid=g).cpu().numpy()
+    assert M.rel_l2(planned, ref) <= 1e-4
+    # a batch of a fine (REF) job and the coarse (DF) job: each equals its plain planned launch
+    jobs = []
+    for gg in (sc.grid, g):
+        out = torch.zeros((gg["ny"], gg["nx"]), dtype=torch.complex64, device="cuda")
+        jb = native.Job()
+        jb.d_profiles, jb.d_pos, jb.h_pos = ds.prof.data_ptr(), ds.pos.data_ptr(), sc.pos.ctypes.data
+        jb.grid = native.make_grid(gg["origin"], gg["ux"], gg["uy"], gg["nx"], gg["ny"])
+        jb.params, jb.d_image = sc.native_params(interp=interp), out.data_ptr()
+        jobs.append((jb, out, gg))
+    ctx.backproject_batch([j for j, _, _ in jobs], torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    for jb, out, gg in jobs:
+        plain = D.backproject(ctx, ds, interp=interp, plan=True, grid=gg).cpu().numpy()
+        assert np.array_equal(out.cpu().numpy().view(np.uint64), plain.view(np.uint64))
+
+
+@pytest.mark.parametrize("rows", [2, 4, 8])
+def test_double_float_mode_every_shape(ctx, base, rows):
+    """SARBP_PREC_DF requested directly, every compiled tile shape, on the
+    0.5 m grid and at 2 m: the oracle bound holds."""
+    sc, ds = base
+    c = native.Context(0)
+    try:
+        c.set_tuning(native.INTERP_SINC8, native.PREC_DF, native.DTYPE_C64, rows, 1, 8)
+        for factor in (1.0, 4.0):
+            g, ng = _scaled_grid(sc, factor)
+            interp = "sinc8" if factor == 1.0 else "linear"
+            img = _run(c, ds, ng, sc.native_params(interp=interp, precision="df")).cpu().numpy()
+            err = M.rel_l2(img, _oracle(sc, interp, g))
+            print(f"  DF 1x{rows} at {0.5 * factor} m ({interp}): rel_l2={err:.2e}", flush=True)
+            assert err <= 1e-4
+    finally:
+        c.close()
 
 
 def test_sinc8_kernel_variants(ctx, base):

@@ -1,13 +1,14 @@
 """Precision variants on the realistic X-band scene (4096 pulses x 8192
 samples -> 4096 x 4096 at 0.25 m, sinc8): range precision {REF (FP64 tile
-reference + FP32 differential), F64, F32} x profile storage {complex64,
-bf16x2}, FP32 accumulation, against the FP64 oracle.
+reference + FP32 differential), F64, F32, DF (FP64 tile reference +
+compensated double-float per-pixel differential)} x profile storage
+{complex64, bf16x2}, FP32 accumulation, against the FP64 oracle.
 
 The oracle covers 64 x 64 chips around six point targets plus two full image
 rows (the whole image would take the FP64 oracle ~30 minutes).  Reported per
 variant: rel-L2 over those pixels, phase error at the target peaks (degrees),
 and the largest PSLR / ISLR difference to the oracle (dB).  Bounds asserted:
-REF and F64 on complex64 hold rel-L2 <= 1e-4, exact peaks and PSLR / ISLR
+REF, F64 and DF on complex64 hold rel-L2 <= 1e-4, exact peaks and PSLR / ISLR
 within 0.1 dB; F32 range cannot hold 1e-4 (asserted above it); bf16 storage
 (8-bit mantissa) is reported.
 """
@@ -48,7 +49,7 @@ def test_precision_variants(ctx):
     ds = D.DeviceScene(sc, bf16=True)
     table = []
     for dtype in ["c64", "bf16x2"]:
-        for prec in ["ref", "f64", "f32"]:
+        for prec in ["ref", "f64", "f32", "df"]:
             img = D.backproject(ctx, ds, interp="sinc8", precision=prec, dtype=dtype).cpu().numpy()
             num = den = 0.0
             ph, qd, peaks = [], [], True
@@ -74,7 +75,7 @@ def test_precision_variants(ctx):
         with open(os.path.join(out, "precision_variants.json"), "w") as f:
             json.dump(table, f, indent=1)
     by = {(r["precision"], r["dtype"]): r for r in table}
-    for prec in ["ref", "f64"]:
+    for prec in ["ref", "f64", "df"]:
         r = by[(prec, "c64")]
         assert r["rel_l2"] <= 1e-4 and r["peaks_exact"] and r["max_pslr_islr_diff_db"] <= 0.1, r
     assert by[("f32", "c64")]["rel_l2"] > 1e-4
