@@ -199,10 +199,11 @@ def leg_mixed(torch, ctx, native, W, args, tuned_classes=()) -> dict:
     t_tune = time.time()
     shape = None
     if args.class_shapes == "common":
-        # one tile shape for every class of the mix (a batch then needs one
-        # persistent launch per sinc8 variant): the candidate with the least
-        # work-weighted time over the classes' largest jobs
-        reps = {c: max(idx, key=lambda i: jobs[i].work) for c, idx in classes.items() if c not in tuned_classes}
+        # one tile shape for every class of the mix, the headline's class
+        # included (a batch then needs one persistent launch per sinc8
+        # variant): the candidate with the least work-weighted time over the
+        # classes' largest jobs
+        reps = {c: max(idx, key=lambda i: jobs[i].work) for c, idx in classes.items()}
         best_total = None
         for cand in ((1, 4), (1, 8)):
             total = 0.0
@@ -215,7 +216,9 @@ def leg_mixed(torch, ctx, native, W, args, tuned_classes=()) -> dict:
                 best_total = (total, cand)
         shape = best_total[1]
     for c, idx in sorted(classes.items()):
-        if c in tuned_classes:  # tuned by the headline leg on a whole image
+        kept = ctx.get_class_tuning(native.INTERP_SINC8, native.PREC_REF, native.DTYPE_C64, c)
+        if c in tuned_classes and (shape is None or (kept[1], kept[0]) == shape):
+            # tuned by the headline leg on a whole image, at the mix's shape
             tuned[c] = {"jobs": len(idx), "work": int(sum(jobs[i].work for i in idx)), "tuned_on": "headline"}
             continue
         k = max(idx, key=lambda i: jobs[i].work)
@@ -233,6 +236,14 @@ def leg_mixed(torch, ctx, native, W, args, tuned_classes=()) -> dict:
             f"{r['best_ms']:.2f} ms vs built-in {r['default_ms']:.2f} ms on job {k}")
     log(f"per-class autotuning: {time.time() - t_tune:.1f}s")
     plans = [ctx.plan(p, gr, pa) for p, gr, pa in zip(poss, grids, params)]
+    for c, idx in sorted(classes.items()):
+        infos = [plans[i].info() for i in idx]
+        lds = max(i["lds_bytes"] for i in infos)
+        tuned[c].update({"windows": [min(i["window"] for i in infos), max(i["window"] for i in infos)],
+                         "max_lds_bytes": lds, "workgroups_per_cu_by_lds": 163840 // (lds + 1024),
+                         "sinc8_halves": sorted({i["sinc8_halves"] for i in infos})})
+        log(f"class {c}: windows {tuned[c]['windows']}, LDS <= {lds} B "
+            f"({tuned[c]['workgroups_per_cu_by_lds']} workgroups/CU), sinc8 halves {tuned[c]['sinc8_halves']}")
     work = sum(j.work for j in jobs)
 
     def plain_all():
@@ -527,16 +538,21 @@ def served_raw_inputs(W, jobs):
     return W.random_profiles(small_max, 43), (W.random_profiles(large_max, 44) if large_max else None)
 
 
-def leg_served(native, W, args, rate: float, inp, n_jobs: int, raw_inputs=None, ctx=None) -> dict:
+def leg_served(native, W, args, rate: float, inp, n_jobs: int, raw_inputs=None, ctx=None,
+               class_tunings=None) -> dict:
     """The first n_jobs of the stream as Poisson arrivals (rate jobs/s; 0 =
     all at once) through the server.  With raw_inputs the jobs are submitted
-    as raw echoes and range-compressed on the device at admission."""
+    as raw echoes and range-compressed on the device at admission.  The
+    server takes ctx's per-class tunings, then class_tunings ({class:
+    (rows, cols, pulses per stage)}) over them."""
     from sar_cambrianaccelerationcard_amd import device as D
     jobs = inp.jobs[:n_jobs]
     srv = native.Server(interp="sinc8", hbm_budget_bytes=args.hbm_budget,
                         pulses_per_epoch=args.pulses_per_epoch)
     if ctx is not None:
         srv.tune_like(ctx)  # the context's per-class (autotuned) launch configurations
+    for c, t in (class_tunings or {}).items():
+        srv.set_class_tuning(c, *t)
     rng = np.random.default_rng(7)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, len(jobs))) if rate > 0 else np.zeros(len(jobs))
     tag = f"served{' raw' if raw_inputs is not None else ''} @ {rate if rate > 0 else 'burst'}/s"
@@ -675,7 +691,7 @@ def main() -> int:
     ap.add_argument("--class-crop-px", type=int, default=1536,
                     help="autotuner crop (pixels per side budget) of the per-class tuning")
     ap.add_argument("--served-jobs", type=int, default=2000, help="served leg jobs (0: skip)")
-    ap.add_argument("--served-rates", default="16",
+    ap.add_argument("--served-rates", default="20",
                     help="Poisson arrival rates (jobs/s) of the served stream, comma separated; "
                          "0 = every job submitted at once")
     ap.add_argument("--e2e-jobs", type=int, default=100,
@@ -776,12 +792,16 @@ def main() -> int:
 
     rc = leg_range(torch, ctx, native, D, args) if args.range_pulses > 0 else None
     wk = leg_omega_k(torch, ctx, native, args) if args.omega_k else None
-    tuned_classes = (native.size_class(sc.native_grid(), params, sc.pos),) if tune else ()
+    head_cls = native.size_class(sc.native_grid(), params, sc.pos)
+    tuned_classes = (head_cls,) if tune else ()
+    # the headline class's tuning, autotuned on a whole 4096^2 image: the
+    # served stream's large jobs (8192^2, 96% of its work) are of this class
+    head_tuning = {head_cls: ctx.get_class_tuning(I, P, Dt, head_cls)} if tune else {}
     batched = leg_mixed(torch, ctx, native, W, args, tuned_classes) if args.mixed_jobs > 0 else None
     served = None
     if args.served_jobs > 0:
         inp = ServedInputs(W, args)
-        runs = [leg_served(native, W, args, float(r), inp, len(inp.jobs), ctx=ctx)
+        runs = [leg_served(native, W, args, float(r), inp, len(inp.jobs), ctx=ctx, class_tunings=head_tuning)
                 for r in args.served_rates.split(",") if r]
         # the simulated jobs' images of the profile runs, before the raw-echo
         # run (which reuses the first jobs and their image buffers) overwrites them
@@ -794,7 +814,8 @@ def main() -> int:
         if args.served_raw_jobs > 0:
             n_raw = min(args.served_raw_jobs, len(inp.jobs))
             raw_inputs = served_raw_inputs(W, inp.jobs[:n_raw])
-            served["raw_echo_burst"] = leg_served(native, W, args, 0.0, inp, n_raw, raw_inputs, ctx=ctx)
+            served["raw_echo_burst"] = leg_served(native, W, args, 0.0, inp, n_raw, raw_inputs, ctx=ctx,
+                                                  class_tunings=head_tuning)
             del raw_inputs
         e2e = plain_e2e(torch, ctx, native, inp, min(args.e2e_jobs, len(inp.jobs)))
         served["plain_end_to_end"] = e2e

@@ -526,6 +526,36 @@ sarbp_status plan_args(const double* h_pos, const sarbp_grid* g,
   return SARBP_OK;
 }
 
+GroupKey group_key(const BpArgs& a, const sarbp_tuning& t) {
+  const size_t shm = lds_layout(a.kernel_interp, a.win, a.pulses_per_stage).total;
+  return GroupKey{t.cols_per_lane, t.rows_per_lane, a.kernel_interp, a.kernel_prec,
+                  (int)std::max<size_t>(1, kCuLds / (shm + kBatchStaticLds))};
+}
+
+int assign_groups(const GroupKey* k, int n, int max_groups, int* out) {
+  for (int pass = 0; pass < 2; ++pass) {
+    const bool by_occ = pass == 0;
+    int ng = 0;
+    bool over = false;
+    for (int i = 0; i < n && !over; ++i) {
+      int g = -1;
+      for (int j = 0; j < i && g < 0; ++j)
+        if (k[j].px == k[i].px && k[j].py == k[i].py && k[j].kinterp == k[i].kinterp &&
+            k[j].kprec == k[i].kprec && (!by_occ || k[j].occ == k[i].occ))
+          g = out[j];
+      if (g < 0) {
+        if (ng == max_groups)
+          over = true;
+        else
+          g = ng++;
+      }
+      out[i] = g;
+    }
+    if (!over) return ng;
+  }
+  return -1;
+}
+
 sarbp_status launch(const BpArgs& a, const sarbp_params* p, const sarbp_tuning& t, hipStream_t s) {
   if (a.nx == 0 || a.ny == 0) return SARBP_OK;
   hipError_t e = launch_plain(a, a.kernel_interp, a.kernel_prec, p->dtype, t.cols_per_lane, t.rows_per_lane, s);
@@ -796,30 +826,35 @@ sarbp_status sarbp_backproject_batch(sarbp_context* ctx, const sarbp_job* jobs, 
     int kinterp, kprec;
     std::vector<JobDesc> desc;
   };
-  std::vector<Group> groups;
+  std::vector<JobDesc> planned;
+  std::vector<sarbp_tuning> tunes;
+  std::vector<GroupKey> keys;
   for (int32_t j : order) {
     const sarbp_job& jb = jobs[j];
     if (jb.grid.nx == 0 || jb.grid.ny == 0) continue;
     const Fitted f = fit_tuning(get_tuning(ctx, p0.interp, p0.precision, p0.dtype, size_class(&jb.grid, &jb.params, jb.h_pos)),
                                 p0.precision, &jb.grid, &jb.params, jb.h_pos);
-    const sarbp_tuning& t = f.t;
     JobDesc d;
-    sarbp_status st = plan_args(jb.h_pos, &jb.grid, &jb.params, t, f.prec, &d.a, nullptr);
+    sarbp_status st = plan_args(jb.h_pos, &jb.grid, &jb.params, f.t, f.prec, &d.a, nullptr);
     if (st != SARBP_OK) return fail(st, "job %d: %s", j, g_detail);
     d.a.prof = jb.d_profiles;
     d.a.pos = jb.d_pos;
     d.a.img = reinterpret_cast<float2*>(jb.d_image);
     d.item_begin = 0;
-    size_t g = 0;
-    while (g < groups.size() && !(groups[g].tune.cols_per_lane == t.cols_per_lane &&
-                                  groups[g].tune.rows_per_lane == t.rows_per_lane &&
-                                  groups[g].kinterp == d.a.kernel_interp && groups[g].kprec == f.prec))
-      ++g;
-    if (g == groups.size()) groups.push_back(Group{t, d.a.kernel_interp, f.prec, {}});
-    groups[g].desc.push_back(d);
+    planned.push_back(d);
+    tunes.push_back(f.t);
+    keys.push_back(group_key(d.a, f.t));
   }
-  if (groups.size() > sizeof ctx->batch / sizeof ctx->batch[0])
-    return fail(SARBP_ERR_UNSUPPORTED, "%zu tile shapes in one batch", groups.size());
+  const int max_groups = (int)(sizeof ctx->batch / sizeof ctx->batch[0]);
+  std::vector<int> gid(planned.size());
+  const int n_groups = assign_groups(keys.data(), (int)keys.size(), max_groups, gid.data());
+  if (n_groups < 0) return fail(SARBP_ERR_UNSUPPORTED, "more than %d tile shapes in one batch", max_groups);
+  std::vector<Group> groups((size_t)n_groups);
+  for (size_t i = 0; i < planned.size(); ++i) {
+    Group& G = groups[gid[i]];
+    if (G.desc.empty()) G = Group{tunes[i], keys[i].kinterp, keys[i].kprec, {}};
+    G.desc.push_back(planned[i]);
+  }
   for (size_t g = 0; g < groups.size(); ++g) {
     if (!ctx->batch[g] && !(ctx->batch[g] = new (std::nothrow) BatchExec))
       return fail(SARBP_ERR_OUT_OF_MEMORY, "batch executor");
@@ -868,38 +903,42 @@ sarbp_status sarbp_backproject_batch_noalloc(sarbp_context* ctx, const sarbp_job
     size_t shm;
     int64_t items;
   } groups[kNoAllocGroups];
-  int n_groups = 0, n_planned = 0;
-  int8_t group_of[kNoAllocJobs];
+  int n_planned = 0;
+  int group_of[kNoAllocJobs];
+  GroupKey keys[kNoAllocJobs];
+  sarbp_tuning tunes[kNoAllocJobs];
   for (int32_t r = 0; r < n_jobs; ++r) {
     const sarbp_job& jb = jobs[order[r]];
     if (jb.grid.nx == 0 || jb.grid.ny == 0) continue;
     const Fitted f = fit_tuning(get_tuning(ctx, p0.interp, p0.precision, p0.dtype, size_class(&jb.grid, &jb.params, jb.h_pos)),
                                 p0.precision, &jb.grid, &jb.params, jb.h_pos);
-    const sarbp_tuning& t = f.t;
     JobDesc& d = g_noalloc_plan[n_planned];
-    st = plan_args(jb.h_pos, &jb.grid, &jb.params, t, f.prec, &d.a, nullptr);
+    st = plan_args(jb.h_pos, &jb.grid, &jb.params, f.t, f.prec, &d.a, nullptr);
     if (st != SARBP_OK) return fail(st, "job %d: %s", order[r], g_detail);
     d.a.prof = jb.d_profiles;
     d.a.pos = jb.d_pos;
     d.a.img = reinterpret_cast<float2*>(jb.d_image);
-    int g = 0;
-    while (g < n_groups && !(groups[g].tune.cols_per_lane == t.cols_per_lane &&
-                             groups[g].tune.rows_per_lane == t.rows_per_lane &&
-                             groups[g].kinterp == d.a.kernel_interp && groups[g].kprec == f.prec))
-      ++g;
-    if (g == n_groups) {
-      if (n_groups == kNoAllocGroups)
-        return fail(SARBP_ERR_UNSUPPORTED, "more than %d tile shapes in one batch", kNoAllocGroups);
-      groups[n_groups++] = Group{t, d.a.kernel_interp, f.prec, 0, 0, 0};
+    tunes[n_planned] = f.t;
+    keys[n_planned] = group_key(d.a, f.t);
+    ++n_planned;
+  }
+  if (n_planned == 0) return SARBP_OK;
+  const int n_groups = assign_groups(keys, n_planned, kNoAllocGroups, group_of);
+  if (n_groups < 0) return fail(SARBP_ERR_UNSUPPORTED, "more than %d tile shapes in one batch", kNoAllocGroups);
+  for (int g = 0; g < n_groups; ++g) groups[g] = Group{sarbp_tuning{}, 0, 0, 0, 0, 0};
+  for (int i = 0; i < n_planned; ++i) {
+    Group& G = groups[group_of[i]];
+    if (G.n == 0) {
+      G.tune = tunes[i];
+      G.kinterp = keys[i].kinterp;
+      G.kprec = keys[i].kprec;
     }
-    Group& G = groups[g];
+    JobDesc& d = g_noalloc_plan[i];
     d.item_begin = G.items;
     G.items += (int64_t)d.a.tiles_x * d.a.tiles_y;
     G.shm = std::max(G.shm, lds_layout(G.kinterp, d.a.win, d.a.pulses_per_stage).total);
     ++G.n;
-    group_of[n_planned++] = (int8_t)g;
   }
-  if (n_planned == 0) return SARBP_OK;
   // Host image of the scratch: headers (counters zero), then each group's
   // table in planning order.  One copy on the stream writes all of it, so
   // the counters start at zero for this call's launches.

@@ -121,8 +121,12 @@ sarbp_status BatchExec::run(std::vector<JobDesc>& jobs, int interp, int precisio
     shm = std::max(shm, lds_layout(interp, d.a.win, d.a.pulses_per_stage).total);
   }
   if (items == 0) return SARBP_OK;
-  // LDS size classes of 8 KiB keep the number of captured graphs small.
-  const size_t shm_class = std::min<size_t>(kLdsBudget, (shm + 8191) & ~(size_t)8191);
+  // LDS size class: the largest size (256-B steps) that keeps as many
+  // workgroups per CU as the group's largest job needs, so the class costs no
+  // occupancy and at most ~16 graphs are captured per kernel.
+  const size_t per_cu_fit = std::max<size_t>(1, kCuLds / (shm + kBatchStaticLds));
+  const size_t shm_class =
+      std::min(kLdsBudget, std::max(shm, (kCuLds / per_cu_fit - kBatchStaticLds) & ~(size_t)255));
   std::memcpy(h_jobs_, jobs.data(), sizeof(JobDesc) * jobs.size());
   h_hdr_->n_items = items;
   h_hdr_->n_jobs = (int32_t)jobs.size();

@@ -18,6 +18,8 @@
 namespace sarbp {
 
 constexpr size_t kLdsBudget = 64 * 1024;  // dynamic LDS per workgroup
+constexpr size_t kCuLds = 160 * 1024;     // LDS of one gfx950 CU
+constexpr size_t kBatchStaticLds = 1024;  // bound on the batched kernel's static LDS
 
 // Records a detail message for sarbp_last_error_detail and returns st.
 sarbp_status fail(sarbp_status st, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
@@ -65,6 +67,21 @@ sarbp_status plan_args(const double* h_pos, const sarbp_grid* g, const sarbp_par
 // Launches the plain kernel of a planned descriptor (its kernel_interp and
 // kernel_prec, the dtype of p).
 sarbp_status launch(const BpArgs& a, const sarbp_params* p, const sarbp_tuning& t, hipStream_t s);
+
+// Launch group of a planned job in a batched launch.  The jobs of a group
+// share one persistent kernel (tile shape, interpolation kernel, arithmetic)
+// and its LDS size, the largest of theirs; occ (workgroups per CU that the
+// job's LDS allows) keeps jobs of different occupancy apart, so one
+// LDS-hungry job does not lower the occupancy of all the others.
+struct GroupKey {
+  int px, py, kinterp, kprec, occ;
+};
+GroupKey group_key(const BpArgs& a, const sarbp_tuning& t);
+// Assigns keys[0..n) to groups in first-seen order (group indices to out).
+// Keys that differ only in occ share a group when keeping them apart would
+// take more than max_groups groups.  Returns the number of groups, or -1 if
+// that still exceeds max_groups.  Allocates nothing.
+int assign_groups(const GroupKey* keys, int n, int max_groups, int* out);
 
 // Batched persistent launch replayed from a captured hipGraph.  The graph is
 // [counter reset kernel -> persistent BP kernel] and reads the job table and

@@ -34,6 +34,15 @@ enum { kNearest = 0, kLinear = 1, kSinc8 = 2, kSinc8W = 3 };
 __host__ __device__ constexpr int sinc_rows(int interp) {
   return interp == kSinc8 ? kSincHalves * kSincCoef : (interp == kSinc8W ? kSincCoefW : 0);
 }
+// LDS stride of one window position's coefficient block, in 16-B units.
+// Half bins: 96 B (two 48-B half blocks; 48-B blocks of consecutive half
+// positions fall on distinct banks).  Whole bins: the 64-B block padded to
+// 80 B, so 8 consecutive positions start on distinct 16-B bank groups; at
+// 64 B, positions 2 apart share banks and a coarse grid's lanes, spread
+// over many positions, lost ~23% of the CU cycles to bank conflicts.
+__host__ __device__ constexpr int coef_stride16(int interp) {
+  return interp == kSinc8 ? kSincHalves * kSincCoef / 2 : (interp == kSinc8W ? kSincCoefW / 2 + 1 : 0);
+}
 enum { kRef = 0, kF64 = 1, kF32 = 2, kDF = 3 };
 
 // Everything one plain launch needs; passed by value as the kernel argument.
@@ -84,9 +93,10 @@ __host__ __device__ inline int setup_batch(int P) { return P >= 64 ? P : P * (64
 
 // LDS layout of one workgroup (byte offsets), SB = setup_batch(P):
 //   raw   [P][win] float2         staged range windows
-//   coef  [P][win-7][R] float2    sinc8 only: the R = sinc_rows polynomial
-//                                 coefficients of the interpolant at each
-//                                 window position ([2][6] half bins or [8])
+//   coef  [P][win-7] blocks       sinc8 only: the R = sinc_rows polynomial
+//                                 coefficients (float2) of the interpolant at
+//                                 each window position ([2][6] half bins or
+//                                 [8] + 16 B of padding), coef_stride16 x 16 B
 //   par   [SB] 48-B pulse records
 //   rng   [SB] int2: first staged sample, first sinc8 position of each pulse
 //   start [SB] int window starts
@@ -105,7 +115,7 @@ __host__ __device__ inline LdsLayout lds_layout(int interp, int win, int P) {
   L.raw = o;
   o += (size_t)P * win * 8;
   L.coef = o;
-  o += (size_t)P * (win - 7) * sinc_rows(interp) * 8;
+  o += (size_t)P * (win - 7) * coef_stride16(interp) * 16;
   L.par = o;
   o += (size_t)SB * kParBytes;
   L.rng = o;

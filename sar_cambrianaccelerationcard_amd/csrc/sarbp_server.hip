@@ -290,13 +290,17 @@ void sarbp_server::sched_loop() {
   std::deque<Epoch> inflight;
   int k = 0;
   for (;;) {
-    // chunks grouped by tile shape and kernel: one batched launch per group
+    // chunks grouped by tile shape, kernel and occupancy: one batched launch per group
     struct Group {
       sarbp_tuning tune;
       int kinterp, kprec;
       std::vector<JobDesc> desc;
     };
     std::vector<Group> groups;
+    std::vector<JobDesc> chunks;
+    std::vector<sarbp_tuning> tunes;
+    std::vector<GroupKey> keys;
+    bool over_groups = false;
     Epoch ep;
     {
       std::unique_lock<std::mutex> lk(mu);
@@ -332,14 +336,9 @@ void sarbp_server::sched_loop() {
         d.a.pulse_end = (int32_t)std::min<int64_t>(j.args.n_pulses, (int64_t)j.next_pulse + chunk);
         d.a.accumulate = j.next_pulse > 0 ? 1 : 0;
         d.item_begin = 0;
-        size_t g = 0;
-        while (g < groups.size() && !(groups[g].tune.cols_per_lane == j.tune.cols_per_lane &&
-                                      groups[g].tune.rows_per_lane == j.tune.rows_per_lane &&
-                                      groups[g].kinterp == j.args.kernel_interp &&
-                                      groups[g].kprec == j.args.kernel_prec))
-          ++g;
-        if (g == groups.size()) groups.push_back(Group{j.tune, j.args.kernel_interp, j.args.kernel_prec, {}});
-        groups[g].desc.push_back(d);
+        chunks.push_back(d);
+        tunes.push_back(j.tune);
+        keys.push_back(group_key(d.a, j.tune));
         j.next_pulse = d.a.pulse_end;
         j.started = true;
         ++j.inflight;
@@ -351,6 +350,15 @@ void sarbp_server::sched_loop() {
         ep.parts.emplace_back(j.id, d.a.pulse_end);
       }
       ++n_epochs;
+      std::vector<int> gid(chunks.size());
+      const int ng = assign_groups(keys.data(), (int)keys.size(), 8, gid.data());
+      groups.resize(ng < 0 ? 0 : (size_t)ng);
+      for (size_t i = 0; ng >= 0 && i < chunks.size(); ++i) {
+        Group& G = groups[gid[i]];
+        if (G.desc.empty()) G = Group{tunes[i], keys[i].kinterp, keys[i].kprec, {}};
+        G.desc.push_back(chunks[i]);
+      }
+      if (ng < 0) over_groups = true;
     }
     if (inflight.size() >= 2) {
       finish_epoch(inflight.front());
@@ -361,10 +369,10 @@ void sarbp_server::sched_loop() {
     if (e == hipSuccess) e = hipEventCreate(&ep.ev1);
     if (e == hipSuccess) e = hipEventRecord(ep.ev0, s_compute);
     sarbp_status st = e == hipSuccess ? SARBP_OK : hip_fail(e, "epoch events");
+    if (st == SARBP_OK && over_groups) st = fail(SARBP_ERR_UNSUPPORTED, "more than 8 tile shapes in an epoch");
     for (size_t g = 0; st == SARBP_OK && g < groups.size(); ++g)
-      st = g < 8 ? exec[k & 1][g].run(groups[g].desc, groups[g].kinterp, groups[g].kprec, cfg.dtype,
-                                      groups[g].tune, n_cu, s_compute)
-                 : fail(SARBP_ERR_UNSUPPORTED, "more than 8 tile shapes in an epoch");
+      st = exec[k & 1][g].run(groups[g].desc, groups[g].kinterp, groups[g].kprec, cfg.dtype, groups[g].tune, n_cu,
+                              s_compute);
     if (st == SARBP_OK && (e = hipEventRecord(ep.ev1, s_compute)) != hipSuccess) st = hip_fail(e, "hipEventRecord");
     if (st != SARBP_OK) {
       // Nothing was enqueued after ev0 (or the launch failed): fail the parts.
