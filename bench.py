@@ -495,12 +495,26 @@ class ServedInputs:
                 buf[: j.side * j.side].reshape(j.side, j.side))
 
 
-def plain_e2e(torch, ctx, native, inp, n_sample: int) -> dict:
+def plain_e2e(torch, ctx, native, inp, n_sample: int, class_tunings=None) -> dict:
     """The reference host flow per image (SAR_Processor, BPA_CUDA.cpp:51-89):
     copy the job's profiles and positions to the device, back-project with the
     unplanned call, copy the image back, synchronise -- one job after another,
     on a sample of the stream that keeps its share of large jobs (the first
-    ones of each class in stream order)."""
+    ones of each class in stream order).  Launch configurations: the
+    context's per-class tunings with class_tunings over them, as the server
+    of the served leg runs them."""
+    I, Pm, Dt = native.INTERP_SINC8, native.PREC_REF, native.DTYPE_C64
+    saved = {c: ctx.get_class_tuning(I, Pm, Dt, c) for c in (class_tunings or {})}
+    for c, t in (class_tunings or {}).items():
+        ctx.set_class_tuning(I, Pm, Dt, c, *t)
+    try:
+        return _plain_e2e(torch, ctx, inp, n_sample)
+    finally:
+        for c, t in saved.items():
+            ctx.set_class_tuning(I, Pm, Dt, c, *t)
+
+
+def _plain_e2e(torch, ctx, inp, n_sample: int) -> dict:
     jobs = inp.jobs
     n_large = max(1, round(0.1 * n_sample)) if any(j.large for j in jobs) else 0
     ks = [k for k, j in enumerate(jobs) if j.large][:n_large] + \
@@ -817,7 +831,7 @@ def main() -> int:
             served["raw_echo_burst"] = leg_served(native, W, args, 0.0, inp, n_raw, raw_inputs, ctx=ctx,
                                                   class_tunings=head_tuning)
             del raw_inputs
-        e2e = plain_e2e(torch, ctx, native, inp, min(args.e2e_jobs, len(inp.jobs)))
+        e2e = plain_e2e(torch, ctx, native, inp, min(args.e2e_jobs, len(inp.jobs)), class_tunings=head_tuning)
         served["plain_end_to_end"] = e2e
         served["peak_speedup_vs_plain_end_to_end"] = served["peak_W_per_s"] / e2e["W_per_s"]
         if batched:
