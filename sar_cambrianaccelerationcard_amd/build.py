@@ -41,6 +41,12 @@ SARBP_SOURCES = ("sarbp_kernels_nearest.hip", "sarbp_kernels_linear.hip", "sarbp
                  "sarbp_kernels_sinc8w.hip",
                  "sarbp_api.hip", "sarbp_batch.hip", "sarbp_server.hip",
                  "sarbp_range.hip", "sarbp_wk.hip", "sarbp_echo.hip", "sarbp_diag.hip")
+# Host-only sources without HIP runtime calls (planning, the server core):
+# compiled by the host compiler, so the CPU sanitizer builds of the tests
+# compile the same files.
+SARBP_HOST_SOURCES = ("sarbp_plan.cpp", "sarbp_server_core.cpp")
+HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__",
+              f"-I{ROCM}/include", f"-I{INCLUDE}", f"-I{CSRC}"]
 
 
 def _run(cmd: list[str]) -> None:
@@ -63,16 +69,18 @@ def build_sarbp(force: bool = True) -> str:
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libsarbp.so")
     srcs = [os.path.join(CSRC, f) for f in SARBP_SOURCES]
-    deps = srcs + [os.path.join(CSRC, h) for h in ("sarbp_internal.h", "sarbp_host.h", "sarbp_fft.h",
-                                                   "sarbp_kernels.inc")] + \
-        [os.path.join(INCLUDE, "sarbp.h")]
+    hsrcs = [os.path.join(CSRC, f) for f in SARBP_HOST_SOURCES]
+    deps = srcs + hsrcs + [os.path.join(CSRC, h) for h in (
+        "sarbp_internal.h", "sarbp_host.h", "sarbp_fft.h", "sarbp_kernels.inc", "sarbp_rotation.h",
+        "sarbp_server_core.h")] + [os.path.join(INCLUDE, "sarbp.h")]
     if not force and _newer(out, deps):
         return out
     from concurrent.futures import ThreadPoolExecutor
-    objs = [os.path.join(LIB, os.path.basename(s) + ".o") for s in srcs]
-    with ThreadPoolExecutor(max_workers=len(srcs)) as ex:
-        list(ex.map(lambda so: _run([HIPCC, *HIP_FLAGS, "-c", so[0], "-o", so[1]]),
-                    zip(srcs, objs)))
+    objs = [os.path.join(LIB, os.path.basename(s) + ".o") for s in srcs + hsrcs]
+    cmds = [[HIPCC, *HIP_FLAGS, "-c", s, "-o", o] for s, o in zip(srcs, objs)] + \
+           [["g++", *HOST_FLAGS, "-c", s, "-o", o] for s, o in zip(hsrcs, objs[len(srcs):])]
+    with ThreadPoolExecutor(max_workers=len(cmds)) as ex:
+        list(ex.map(_run, cmds))
     tmp = out + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
           "-L" + os.path.join(ROCM, "lib"), "-lhipfft", "-Wl,-rpath," + os.path.join(ROCM, "lib")])

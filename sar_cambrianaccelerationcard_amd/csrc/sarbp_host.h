@@ -40,6 +40,12 @@ struct DeviceGuard {
   DeviceGuard& operator=(const DeviceGuard&) = delete;
 };
 
+// ---- planning (sarbp_plan.cpp; no HIP runtime calls) ----
+bool finite_all(const double* v, int n);
+double norm3(const double* v);
+// Kaiser-sinc tap polynomials of the sinc8 kernels (half bins, whole bins).
+void sinc8_poly(double beta, float* out);
+void sinc8_poly_whole(double beta, float* out);
 sarbp_status validate(const sarbp_grid* g, const sarbp_params* p);
 // Size class of a job (0 .. SARBP_SIZE_CLASSES - 1; sarbp_size_class).
 int size_class(const sarbp_grid* g, const sarbp_params* p, const double* h_pos);
