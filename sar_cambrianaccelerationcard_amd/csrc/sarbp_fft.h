@@ -7,9 +7,10 @@
 //
 // Element i of the transform lives at LDS slot fft::slot<LG>(i): one pad word
 // per 16 for n <= 4096 (buffer fft::lds_words<LG>() complex64 words), an XOR
-// swizzle of bits 1-3 by bits 4-6 for n = 8192 (exactly 64 KiB).  Either
-// turns the power-of-two strides of the Stockham writes into ones whose 16
-// lanes per LDS access hit distinct banks.
+// swizzle of bits 1-3 by bits 4-6 for n >= 8192 (exactly n words: 64 KiB at
+// 8192 points, 128 KiB at 16384).  Either turns the power-of-two strides of
+// the Stockham writes into ones whose 16 lanes per LDS access hit distinct
+// banks.
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -2,10 +2,12 @@
 // upsampling by spectral zero insertion (code/SAR_BP/matlab/BPA_SAR.m:93-100,
 // :106-111), the step that turns raw LFM echoes into the profiles BP reads.
 //
-// One workgroup per pulse (256 threads, up to 1024 for the largest FFTs);
-// every transform runs in LDS (nf complex64 words plus padding, at most
-// 64 KiB for nf = 8192), so a pulse costs one HBM read of its raw echo and
-// one write of its output window.
+// One workgroup per pulse (256 threads, up to 1024 for the largest FFTs).
+// Below 2048 points every transform pass runs in LDS; from 2048 points each
+// thread keeps 8 points in registers (fft::fft_regs) and only the inner
+// passes go through LDS.  The LDS buffer is nf complex64 words plus padding,
+// at most 64 KiB (nf = 8192).  A pulse costs one HBM read of its raw echo
+// and one write of its output window.
 //
 // The L-times upsampled inverse transform is split into L polyphase inverse
 // FFTs of size nf instead of one of size L*nf: with k~ the signed bin of the
@@ -13,7 +15,7 @@
 //   x[L q + r] = sum_k Y[k] e^{j 2 pi k~ (L q + r) / (L nf)}
 //              = IFFT_nf( Y[k] e^{j 2 pi k~ r / (L nf)} )[q],
 // so phase r is an nf-point inverse FFT of the spectrum times a ramp.  The
-// FFTs are the in-LDS radix-8 Stockham transforms of sarbp_fft.h.
+// FFTs are the radix-8 Stockham transforms of sarbp_fft.h.
 //
 // Twiddles, the matched filter conj(FFT(h))/nf and the ramp are FP32 tables
 // computed in FP64 on the host and kept per context.

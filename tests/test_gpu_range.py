@@ -39,6 +39,11 @@ CASES = [
     (1200, 2e-6, 200e6, 4, 5, 4000, 64),   # nf 2048
     (2448, 2e-6, 200e6, 4, 0, 8192, 64),   # nf 4096 (4096-pulse X-band scene)
     (4496, 2e-6, 200e6, 4, 0, 16384, 32),  # nf 8192 (large served job)
+    # the register-FFT path (nf >= 2048) without upsampling and with L = 2
+    (1200, 2e-6, 200e6, 1, 0, 2048, 32),   # nf 2048, L 1
+    (1200, 2e-6, 200e6, 2, 3, 4000, 32),   # nf 2048, L 2, cropped
+    (2448, 2e-6, 200e6, 1, 0, 4096, 16),   # nf 4096, L 1
+    (4496, 2e-6, 200e6, 2, 0, 16384, 16),  # nf 8192, L 2
 ]
 
 

@@ -356,6 +356,25 @@ def test_omega_k_image_at_an_odd_offset(ctx):
     assert buf[0].item() == 0
 
 
+def test_omega_k_echo_at_an_8_byte_offset(ctx):
+    # An echo that starts 8 bytes into its buffer on the power-of-two azimuth
+    # path: the forward azimuth transform then takes its one-column kernel
+    # (the two-column kernel reads 16-byte pairs).  Against the oracle, and
+    # the element before the echo is never written.
+    torch = _torch()
+    echo = _rand((768, 20), 78)
+    wp = O.WkScene(na_in=768, nr_in=20).params()
+    na, nr = W.padded_size(768, 20)[:2]
+    ebuf = torch.zeros(768 * 20 + 1, dtype=torch.complex64, device="cuda:0")
+    ebuf[1:] = torch.from_numpy(np.ascontiguousarray(echo).reshape(-1)).to("cuda:0")
+    before = ebuf[0].item()
+    out = torch.zeros((na, nr), dtype=torch.complex64, device="cuda:0")
+    ctx.omega_k(ebuf.data_ptr() + 8, 768, 20, wp, out.data_ptr())
+    torch.cuda.synchronize()
+    assert _rel_l2(out.cpu().numpy(), _oracle(echo, wp)) < 2e-5
+    assert ebuf[0].item() == before
+
+
 def test_omega_k_calls_on_two_streams(ctx):
     # Calls on one context share its workspace; issued back to back on two
     # streams they are ordered on the device and both images are right.
