@@ -184,10 +184,28 @@ def leg_mixed(torch, ctx, native, W, args, tuned_classes=()) -> dict:
     g = torch.Generator(device=dev).manual_seed(11)
     prof = torch.randn(2 * max_prof, device=dev, generator=g, dtype=torch.float32)
     poss = [j.positions() for j in jobs]
-    dpos = [torch.from_numpy(p).to(dev) for p in poss]
-    imgs = [torch.empty((j.side, j.side), dtype=torch.complex64, device=dev) for j in jobs]
     params = [j.params() for j in jobs]
     grids = [j.grid() for j in jobs]
+    pptr = [prof.data_ptr()] * len(jobs)
+    # a few jobs carry their own simulated scene (point targets over clutter,
+    # as the realistic workload) instead of the shared seeded noise, for the
+    # oracle spot checks of the batched images: the smallest job of each
+    # quarter of the pixel-spacing range
+    sims = {}
+    if args.mixed_sims > 0:
+        qs = np.quantile([j.spacing for j in jobs], np.linspace(0, 1, args.mixed_sims + 1))
+        for a, b in zip(qs[:-1], qs[1:]):
+            cand = [i for i, j in enumerate(jobs) if a <= j.spacing <= b and i not in sims]
+            if not cand:
+                continue
+            i = min(cand, key=lambda k: jobs[k].work)
+            sc = jobs[i].simulate(n_targets=6, n_clutter=512)
+            d = torch.from_numpy(np.ascontiguousarray(sc.profiles)).to(dev)
+            sims[i] = (sc, d)
+            poss[i], params[i], grids[i], pptr[i] = sc.pos, sc.native_params(interp="sinc8"), sc.native_grid(), \
+                d.data_ptr()
+    dpos = [torch.from_numpy(p).to(dev) for p in poss]
+    imgs = [torch.empty((j.side, j.side), dtype=torch.complex64, device=dev) for j in jobs]
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
     # autotune every size class of the mix on its largest job (the classes'
@@ -247,12 +265,12 @@ def leg_mixed(torch, ctx, native, W, args, tuned_classes=()) -> dict:
     work = sum(j.work for j in jobs)
 
     def plain_all():
-        for pl, dp, im in zip(plans, dpos, imgs):
-            pl.execute(prof.data_ptr(), dp.data_ptr(), im.data_ptr(), sp)
+        for pl, pp, dp, im in zip(plans, pptr, dpos, imgs):
+            pl.execute(pp, dp.data_ptr(), im.data_ptr(), sp)
 
     def job(i, im):
         jb = native.Job()
-        jb.d_profiles = prof.data_ptr()
+        jb.d_profiles = pptr[i]
         jb.d_pos = dpos[i].data_ptr()
         jb.h_pos = poss[i].ctypes.data
         jb.grid = grids[i]
@@ -274,15 +292,34 @@ def leg_mixed(torch, ctx, native, W, args, tuned_classes=()) -> dict:
     log(f"mixed {len(jobs)} jobs ({work:.3e} W): plain sequential {t_plain * 1e3:.1f} ms "
         f"({work / t_plain:.3e} W/s), batched {t_batch * 1e3:.1f} ms ({work / t_batch:.3e} W/s), "
         f"x{t_plain / t_batch:.2f}, bitwise equal {same}")
-    del prof, imgs, bimgs, dpos
+    checks = []
+    if sims:
+        import oracle.oracle as O
+        import oracle.spot as SP
+        for i, (sc, _) in sorted(sims.items()):
+            p = O.params(sc.c, sc.fc, sc.r0, sc.dr, "sinc8", sc.kaiser_beta)
+            tp = sc.target_pixels
+            r = SP.spot_check(sc.profiles, sc.pos, sc.grid, p, bimgs[i].cpu().numpy(), [int(tp[0][1])],
+                              [tuple(int(v) for v in tp[0])])
+            r.update({"job": i, "side": jobs[i].side, "n_pulses": jobs[i].n_pulses,
+                      "spacing_m": jobs[i].spacing,
+                      "size_class": native.size_class(grids[i], params[i], poss[i])})
+            checks.append(r)
+            log(f"batched spot check job {i} ({jobs[i].side}^2 at {jobs[i].spacing:.2f} m, {jobs[i].n_pulses} "
+                f"pulses, class {r['size_class']}): rel-L2 {r['rel_l2']:.2e}, "
+                f"peak {'ok' if all(r['chip_peak_match']) else 'MISMATCH'}")
+    del prof, imgs, bimgs, dpos, sims
     torch.cuda.empty_cache()
     return {"workload": f"{len(jobs)} mixed jobs: side logU[256,2048] px at U[0.25,1] m, "
-                        f"pulses U[512,4096], samples U[2048,8192], sinc8",
+                        f"pulses U[512,4096], samples U[2048,8192], sinc8; {len(checks)} of them simulated "
+                        f"scenes (point targets over clutter, oracle spot checks), the rest seeded noise",
             "n_jobs": len(jobs), "work": work,
             "plain_sequential_W_per_s": work / t_plain, "batched_W_per_s": work / t_batch,
             "speedup_vs_plain": t_plain / t_batch, "plain_ms": t_plain * 1e3,
             "batched_ms": t_batch * 1e3, "batched_equals_plain_bitwise": bool(same),
-            "size_classes": tuned}
+            "size_classes": tuned,
+            "spot_checks": checks,
+            "spot_check_max_rel_l2": max((c["rel_l2"] for c in checks), default=None)}
 
 
 def tune_stages(torch, ctx, native, stream, shape, cls, prof, dpos, pos, grid, params, img, keep=True) -> dict:
@@ -699,6 +736,8 @@ def main() -> int:
                     help="timed launches of the unplanned entry point and the DF / F64 / F32 modes (0: skip)")
     ap.add_argument("--mixed-jobs", type=int, default=64, help="batched leg jobs (0: skip)")
     ap.add_argument("--mixed-reps", type=int, default=2)
+    ap.add_argument("--mixed-sims", type=int, default=4,
+                    help="batched jobs given simulated scenes and oracle spot checks (0: none)")
     ap.add_argument("--class-shapes", default="common", choices=("common", "free"),
                     help="per-class tuning of the batched and served legs: the headline's tile shape "
                          "with pulses per stage tuned per class (common), or the autotuner's free choice")
@@ -708,6 +747,9 @@ def main() -> int:
     ap.add_argument("--served-rates", default="20",
                     help="Poisson arrival rates (jobs/s) of the served stream, comma separated; "
                          "0 = every job submitted at once")
+    ap.add_argument("--served-sweep", default="8:400",
+                    help="further points of the rate sweep as rate:jobs (a prefix of the stream), comma "
+                         "separated")
     ap.add_argument("--e2e-jobs", type=int, default=100,
                     help="jobs of the stream run through the reference host flow (baseline)")
     ap.add_argument("--hbm-budget", type=int, default=200 * 10**9)
@@ -825,6 +867,11 @@ def main() -> int:
                               f"samples; others sized as the batched mix), 200 GB HBM budget, sinc8, REF",
                   "runs": runs, "peak_W_per_s": best["W_per_s"],
                   "p50_latency_s": runs[0]["p50_latency_s"], "p99_latency_s": runs[0]["p99_latency_s"]}
+        for spec in (r for r in args.served_sweep.split(",") if r):
+            # a lower-load point of the rate sweep on a prefix of the same stream
+            rate, n = spec.split(":")
+            runs.append(leg_served(native, W, args, float(rate), inp, min(int(n), len(inp.jobs)), ctx=ctx,
+                                   class_tunings=head_tuning))
         if args.served_raw_jobs > 0:
             n_raw = min(args.served_raw_jobs, len(inp.jobs))
             raw_inputs = served_raw_inputs(W, inp.jobs[:n_raw])

@@ -176,6 +176,30 @@ def test_grid_too_coarse_for_ref_runs_double_float(ctx, base, interp, factor):
         assert np.array_equal(out.cpu().numpy().view(np.uint64), plain.view(np.uint64))
 
 
+def test_allocation_free_batch_with_a_double_float_job(ctx, base):
+    """The allocation-free batch groups a DF-fallback job apart from REF jobs
+    and reproduces the allocating batch bit for bit."""
+    import torch
+    sc, ds = base
+    g, _ = _scaled_grid(sc, 3.2)
+    out = {}
+    for name in ("batch", "noalloc"):
+        jobs = []
+        for gg in (sc.grid, g):
+            img = torch.zeros((gg["ny"], gg["nx"]), dtype=torch.complex64, device="cuda")
+            jb = native.Job()
+            jb.d_profiles, jb.d_pos, jb.h_pos = ds.prof.data_ptr(), ds.pos.data_ptr(), sc.pos.ctypes.data
+            jb.grid = native.make_grid(gg["origin"], gg["ux"], gg["uy"], gg["nx"], gg["ny"])
+            jb.params, jb.d_image = sc.native_params(interp="sinc8"), img.data_ptr()
+            jobs.append((jb, img))
+        call = ctx.backproject_batch if name == "batch" else ctx.backproject_batch_noalloc
+        call([j for j, _ in jobs], torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        out[name] = [im.cpu().numpy() for _, im in jobs]
+    for a, b in zip(out["batch"], out["noalloc"]):
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
 @pytest.mark.parametrize("rows", [2, 4, 8])
 def test_double_float_mode_every_shape(ctx, base, rows):
     """SARBP_PREC_DF requested directly, every compiled tile shape, on the
