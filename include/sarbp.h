@@ -667,12 +667,15 @@ sarbp_status sarbp_omega_k(sarbp_context* ctx, const sarbp_cf32* d_echo, int32_t
 /* ---- diagnostics -------------------------------------------------------------- */
 
 /* The 96 coefficients of the SINC8 interpolator's half-bin kernel for Kaiser
- * shape `beta`: out[(h * 6 + k) * 8 + t] multiplies g^k in the weight of tap t
+ * shape `beta`: out[(h * 6 + k) * 8 + t] multiplies e^k in the weight of tap t
  * (sample floor(u) - 3 + t) at fractional bin f = u - floor(u) in half
- * h = floor(2 f) of the bin, g = 2 f - h.  Host-only, no device needed. */
+ * h = floor(2 f) of the bin, e = 2 f - h - 1/2.  The halves mirror each
+ * other exactly: out[((1 - h) * 6 + k) * 8 + 7 - t] = (-1)^k
+ * out[(h * 6 + k) * 8 + t].  Host-only, no device needed. */
 sarbp_status sarbp_sinc8_coefficients(double beta, float* out);
-/* The 64 coefficients of its whole-bin kernel: out[k * 8 + t] multiplies f^k
- * in the weight of tap t.  Host-only. */
+/* The 64 coefficients of its whole-bin kernel: out[k * 8 + t] multiplies e^k,
+ * e = f - 1/2, in the weight of tap t; out[k * 8 + 7 - t] = (-1)^k
+ * out[k * 8 + t].  Host-only. */
 sarbp_status sarbp_sinc8_coefficients_whole(double beta, float* out);
 
 /* Vector-ALU issue-rate probe (roofline calibration; no counterpart in the

@@ -46,6 +46,9 @@ double norm3(const double* v);
 // Kaiser-sinc tap polynomials of the sinc8 kernels (half bins, whole bins).
 void sinc8_poly(double beta, float* out);
 void sinc8_poly_whole(double beta, float* out);
+// Their mirror-sum / mirror-difference rows for a kernel (kSinc8: 48 floats,
+// kSinc8W: 32); nothing for the other interpolators.
+void sinc8_kernel_rows(int kinterp, double beta, float* out);
 sarbp_status validate(const sarbp_grid* g, const sarbp_params* p);
 // Size class of a job (0 .. SARBP_SIZE_CLASSES - 1; sarbp_size_class).
 int size_class(const sarbp_grid* g, const sarbp_params* p, const double* h_pos);

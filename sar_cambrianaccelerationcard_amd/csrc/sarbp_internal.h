@@ -16,10 +16,12 @@ constexpr int kWaves = 4;
 constexpr int kThreads = kTileX * kWaves;
 // Sinc8 comes in two kernels.  kSinc8 (half bins): the weight of tap t
 // (sample i0 - 3 + t) at fractional bin f, in half h = floor(2 f) of the bin,
-// is the degree-5 polynomial sum_k C[h][k][t] g^k of g = 2 f - h.  kSinc8W
-// (whole bins): the degree-7 polynomial sum_k C[k][t] f^k.  Half bins cost a
-// pixel 10 FMA instead of 14 and a window position 192 FMA instead of 128 of
-// coefficient precompute; the planner picks per launch.
+// is the degree-5 polynomial sum_k C[h][k][t] e^k of e = 2 f - h - 1/2.
+// kSinc8W (whole bins): the degree-7 polynomial sum_k C[k][t] e^k of
+// e = f - 1/2.  Half bins cost a pixel 10 FMA instead of 14 and a window
+// position 96 FMA instead of 64 of coefficient precompute (mirror sums and
+// differences of the taps, the polynomials being even or odd in e about the
+// interval's centre); the planner picks per launch.
 constexpr int kSincTaps = 8;
 constexpr int kSincHalves = 2;
 constexpr int kSincCoef = 6;
@@ -29,8 +31,8 @@ constexpr int kParBytes = 48;
 
 enum { kNearest = 0, kLinear = 1, kSinc8 = 2, kSinc8W = 3 };
 
-// Polynomial rows (of kSincTaps coefficients) per window position: the
-// interpolant's complex coefficients D per position, 8 B each.
+// Complex coefficients D (8 B each) of the interpolant per window position;
+// also the number of 4-float rows of the kernel's tap-polynomial table.
 __host__ __device__ constexpr int sinc_rows(int interp) {
   return interp == kSinc8 ? kSincHalves * kSincCoef : (interp == kSinc8W ? kSincCoefW : 0);
 }
@@ -68,8 +70,9 @@ struct BpArgs {
   int32_t accumulate;
   int32_t kernel_interp;  // kNearest, kLinear, kSinc8 or kSinc8W
   int32_t kernel_prec;    // kRef, kF64, kF32 or kDF: the arithmetic launched
-  // kSinc8: C[(h * 6 + k) * 8 + t]; kSinc8W: C[k * 8 + t]
-  float sinc_poly[kSincHalves * kSincCoef * kSincTaps];
+  // tap-polynomial rows of 4 (sinc8_kernel_rows): kSinc8 [6][P, Q] of the
+  // mirror sums and differences, kSinc8W [8] (k even: sums, odd: differences)
+  float sinc_poly[kSincHalves * kSincCoef * 4];
 };
 
 // Batched (persistent) launch: one work item = one tile of one job.
@@ -102,8 +105,8 @@ __host__ __device__ inline int setup_batch(int P) { return P >= 64 ? P : P * (64
 //   start [SB] int window starts
 //   stage [SB/P][3] int per stage: range-expansion order, staged samples and
 //                   sinc8 positions per pulse
-//   poly  [R][8] float  sinc8 only: the tap polynomials C (read as broadcasts;
-//                       too many to stay in scalar registers)
+//   poly  [R][4] float  sinc8 only: the tap-polynomial rows (read as
+//                       broadcasts)
 struct LdsLayout {
   size_t raw, coef, par, rng, start, stage, poly, total;
 };
@@ -126,7 +129,7 @@ __host__ __device__ inline LdsLayout lds_layout(int interp, int win, int P) {
   o += (size_t)(SB / P) * 12;
   o = (o + 15) & ~(size_t)15;
   L.poly = o;
-  o += (size_t)sinc_rows(interp) * kSincTaps * 4;
+  o += (size_t)sinc_rows(interp) * 16;
   L.total = (o + 15) & ~(size_t)15;
   return L;
 }

@@ -52,27 +52,31 @@ double kaiser_sinc8(double x, double beta, double inv_i0b) {
   return s * w;
 }
 
-// Coefficients C[k * 8 + t] of f^k in the weight of sinc8 tap t (sample
-// i0 - 3 + t) at fractional bin f in [0, 1): the degree-7 polynomial that
-// interpolates K(f + 3 - t) at the 8 Chebyshev nodes of [0, 1]
-// (near-minimax; max error ~2e-6 for beta = 6).  Solved as an 8x8 Vandermonde
-// system in FP64 with partial pivoting.
-// Fits the tap polynomials of the sinc8 interpolator: `halves` = 2 gives the
-// half-bin degree-5 set C[(h * 6 + k) * 8 + t] in g = 2 f - h, `halves` = 1
-// the whole-bin degree-7 set C[k * 8 + t] in f.  Interpolation at Chebyshev
-// nodes, FP64 Vandermonde solve with partial pivoting.
+// Tap polynomials of the sinc8 interpolator, in the centred variable e of
+// each interpolation interval.  Whole bins (`halves` = 1): e = f - 1/2 over
+// f in [0, 1), C[k * 8 + t] multiplies e^k in the weight K(f + 3 - t) of tap
+// t (sample i0 - 3 + t), degree 7.  Half bins (`halves` = 2): half h =
+// floor(2 f), e = 2 f - h - 1/2, C[(h * 6 + k) * 8 + t], degree 5.  Each
+// polynomial interpolates its weight at the N Chebyshev nodes of
+// [-1/2, 1/2] (near-minimax; max error ~2e-6 for beta = 6), an N x N
+// Vandermonde solve in FP64 with partial pivoting, into `out` (FP64).
+//
+// K(x) = K(-x) maps (half h, tap t, e) to (half 1 - h, tap 7 - t, -e), so
+// C[1 - h][k][7 - t] = (-1)^k C[h][k][t]: half 0 (whole bins: taps 0..3) is
+// fitted and the rest is set by that identity, exactly, which the kernels'
+// coefficient precompute relies on.
 template <int N>
-void sinc8_poly_fit(double beta, int halves, float* out) {
+void sinc8_poly_fit(double beta, int halves, double* out) {
   const double inv_i0b = 1.0 / bessel_i0(beta);
   double nodes[N];
-  for (int n = 0; n < N; ++n) nodes[n] = 0.5 + 0.5 * std::cos(kPi * (n + 0.5) / N);
-  for (int ht = 0; ht < halves * kSincTaps; ++ht) {
-    const int h = ht / kSincTaps, t = ht % kSincTaps;
+  for (int n = 0; n < N; ++n) nodes[n] = 0.5 * std::cos(kPi * (n + 0.5) / N);
+  const int fitted = halves == 2 ? kSincTaps : kSincTaps / 2;  // taps of half 0 fitted
+  for (int t = 0; t < fitted; ++t) {
     double A[N][N + 1];
     for (int n = 0; n < N; ++n) {
       double v = 1.0;
       for (int k = 0; k < N; ++k, v *= nodes[n]) A[n][k] = v;
-      A[n][N] = kaiser_sinc8((h + nodes[n]) / halves + 3.0 - t, beta, inv_i0b);
+      A[n][N] = kaiser_sinc8((0.5 + nodes[n]) / halves + 3.0 - t, beta, inv_i0b);
     }
     for (int c = 0; c < N; ++c) {
       int piv = c;
@@ -85,12 +89,53 @@ void sinc8_poly_fit(double beta, int halves, float* out) {
         for (int k = c; k <= N; ++k) A[r][k] -= m * A[c][k];
       }
     }
-    for (int k = 0; k < N; ++k) out[(h * N + k) * kSincTaps + t] = (float)(A[k][N] / A[k][k]);
+    for (int k = 0; k < N; ++k) {
+      const double c = A[k][N] / A[k][k];
+      out[k * kSincTaps + t] = c;
+      out[((halves - 1) * N + k) * kSincTaps + (kSincTaps - 1 - t)] = (k & 1) ? -c : c;
+    }
   }
 }
 
-void sinc8_poly(double beta, float* out) { sinc8_poly_fit<kSincCoef>(beta, kSincHalves, out); }
-void sinc8_poly_whole(double beta, float* out) { sinc8_poly_fit<kSincCoefW>(beta, 1, out); }
+void sinc8_poly(double beta, float* out) {
+  double c[kSincHalves * kSincCoef * kSincTaps];
+  sinc8_poly_fit<kSincCoef>(beta, kSincHalves, c);
+  for (int i = 0; i < kSincHalves * kSincCoef * kSincTaps; ++i) out[i] = (float)c[i];
+}
+
+void sinc8_poly_whole(double beta, float* out) {
+  double c[kSincCoefW * kSincTaps];
+  sinc8_poly_fit<kSincCoefW>(beta, 1, c);
+  for (int i = 0; i < kSincCoefW * kSincTaps; ++i) out[i] = (float)c[i];
+}
+
+// The kernels' form of the polynomials: with sigma_t = s_t + s_(7-t) and
+// delta_t = s_t - s_(7-t) (t < 4) the mirror sums and differences of the 8
+// staged samples, the interpolant's coefficients at one window position are
+//   half bins:  D[0][k] = P_k . sigma + Q_k . delta,
+//               D[1][k] = (-1)^k (P_k . sigma - Q_k . delta),
+//               P_k = (C[0][k][t] + C[0][k][7-t]) / 2,
+//               Q_k = (C[0][k][t] - C[0][k][7-t]) / 2;
+//   whole bins: D[k] = C[k][t<4] . sigma (k even), . delta (k odd),
+// 4 FMA per term instead of 8.  Rows of 4: half bins [k][P, Q], whole bins
+// [k].
+void sinc8_kernel_rows(int kinterp, double beta, float* out) {
+  if (kinterp == kSinc8) {
+    double c[kSincHalves * kSincCoef * kSincTaps];
+    sinc8_poly_fit<kSincCoef>(beta, kSincHalves, c);
+    for (int k = 0; k < kSincCoef; ++k)
+      for (int t = 0; t < 4; ++t) {
+        const double a = c[k * kSincTaps + t], b = c[k * kSincTaps + kSincTaps - 1 - t];
+        out[(2 * k) * 4 + t] = (float)(0.5 * (a + b));
+        out[(2 * k + 1) * 4 + t] = (float)(0.5 * (a - b));
+      }
+  } else if (kinterp == kSinc8W) {
+    double c[kSincCoefW * kSincTaps];
+    sinc8_poly_fit<kSincCoefW>(beta, 1, c);
+    for (int k = 0; k < kSincCoefW; ++k)
+      for (int t = 0; t < 4; ++t) out[k * 4 + t] = (float)c[k * kSincTaps + t];
+  }
+}
 
 bool finite_all(const double* v, int n) {
   for (int i = 0; i < n; ++i)
@@ -425,8 +470,7 @@ sarbp_status plan_args(const double* h_pos, const sarbp_grid* g,
   a->order_tol = 1e-5;
   a->kernel_interp = kinterp;
   a->kernel_prec = kprec;
-  if (kinterp == kSinc8) sinc8_poly(p->kaiser_beta, a->sinc_poly);
-  if (kinterp == kSinc8W) sinc8_poly_whole(p->kaiser_beta, a->sinc_poly);
+  sinc8_kernel_rows(kinterp, p->kaiser_beta, a->sinc_poly);
   if (info) {
     info->tiles_x = a->tiles_x;
     info->tiles_y = a->tiles_y;
