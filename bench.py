@@ -892,13 +892,14 @@ def main() -> int:
 
     fw = F_W[interp]
     achieved = wps * fw / 1e12
-    traffic = None
+    traffic = traffic_tuning = None
     tpath = os.path.join(ROOT, "profiles", "bp_sinc8_4096x4096_4096pulses_traffic.json")
     if os.path.exists(tpath):
         t = json.load(open(tpath))
         if (t["workload"], t["interp"], t["precision"], t["dtype"]) == \
                 (cfg["name"], interp, args.precision, args.dtype):
             traffic = t["traffic_bytes"]
+            traffic_tuning = t["tuning"]  # the launch the counters measured (tile shape sets the re-fetch)
     line = {
         "metric": "pixel·pulse back-projections/s at rel-L2≤1e-4 vs FP64; p50/p99 job latency",
         "value": wps,
@@ -922,7 +923,7 @@ def main() -> int:
                    "profile_dtype": args.dtype, "plan": info, "autotune": tune},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_OPS / 1e12,
                      "unit": "Top/s", "frac": achieved / (PEAK_OPS / 1e12),
-                     "ops_per_W": fw, "traffic": traffic,
+                     "ops_per_W": fw, "traffic": traffic, "traffic_tuning": traffic_tuning,
                      "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE, "
                                      "copy-calibrated; profiles/)",
                      "flop_convention": {"achieved_tflops": wps * FLOP_W[interp] / 1e12,

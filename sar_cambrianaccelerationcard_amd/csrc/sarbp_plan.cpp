@@ -176,22 +176,22 @@ sarbp_status validate(const sarbp_grid* g, const sarbp_params* p) {
 }
 
 sarbp_tuning default_tuning(int interp, int cls, int precision) {
-  // Built-in defaults: the best one-column points of (tile shape x
-  // pulses/stage) sweeps on MI355X with the project autotuner.  Sinc8, per
-  // size class (range windows of 72, 96-108, 144, 212-236 and 332 samples:
-  // X band at 5 km, 0.25 m pixels under 2048- and 4096-8192-pulse apertures,
-  // 0.5, 0.7 and 1 m pixels): wider windows fit fewer pulses in a stage at
-  // the occupancy that wins.  Two-column tiles win on some grids but lose up
-  // to 37% under long apertures (an 8192-pulse aperture at 0.25 m: 2 x 4
-  // 0.76 s, 1 x 8 0.556 s), where their width adds range spread; the
-  // autotuner picks them where they pay.  Linear and nearest stage 32 pulses
-  // of their 8-byte windows.
+  // Built-in defaults: per size class (range windows of a 64 x 32 tile of
+  // < 80, < 120, < 176, < 272 and >= 272 samples), the (tile shape x pulses
+  // per stage) point of MI355X sweeps that is closest to the best on every
+  // swept geometry -- 2048 pulses over 2048^2 pixels and 8192 pulses over
+  // 1536^2, 0.25 / 0.35 / 0.5 / 0.7 / 1 m pixels at X band, 5 km (within 6%
+  // of each geometry's best; class 4 within 7%).  Wider windows fit fewer
+  // pulses in a stage at the occupancy that wins.  Two-column tiles win on
+  // some grids but lose up to 37% under long apertures, where their width
+  // adds range spread; the autotuner picks them where they pay.  Linear and
+  // nearest stage 32 pulses of their 8-byte windows.
   sarbp_tuning t;
   t.cols_per_lane = 1;
   t.rows_per_lane = 4;
   t.pulses_per_stage = 32;
   if (interp == SARBP_INTERP_SINC8) {
-    static const int kShape[SARBP_SIZE_CLASSES][3] = {{1, 4, 6}, {1, 8, 4}, {1, 8, 2}, {1, 4, 4}, {1, 4, 2}};
+    static const int kShape[SARBP_SIZE_CLASSES][3] = {{1, 4, 8}, {1, 4, 6}, {1, 4, 4}, {1, 4, 2}, {1, 8, 1}};
     const int c = std::min(std::max(cls, 0), SARBP_SIZE_CLASSES - 1);
     t.cols_per_lane = kShape[c][0];
     t.rows_per_lane = kShape[c][1];
