@@ -534,12 +534,14 @@ class ServedInputs:
 
 def plain_e2e(torch, ctx, native, inp, n_sample: int, class_tunings=None) -> dict:
     """The reference host flow per image (SAR_Processor, BPA_CUDA.cpp:51-89):
-    copy the job's profiles and positions to the device, back-project with the
-    unplanned call, copy the image back, synchronise -- one job after another,
-    on a sample of the stream that keeps its share of large jobs (the first
-    ones of each class in stream order).  Launch configurations: the
-    context's per-class tunings with class_tunings over them, as the server
-    of the served leg runs them."""
+    copy the job's profiles and positions to the device, back-project, copy
+    the image back, synchronise -- one job after another, on a sample of the
+    stream that keeps its share of large jobs (the first ones of each class in
+    stream order).  The back-projection is the strongest one-job call: a plan
+    made from the host positions (the reference's host holds them before it
+    copies them), planning time included, as the server plans each job.
+    Launch configurations: the context's per-class tunings with class_tunings
+    over them, as the server of the served leg runs them."""
     I, Pm, Dt = native.INTERP_SINC8, native.PREC_REF, native.DTYPE_C64
     saved = {c: ctx.get_class_tuning(I, Pm, Dt, c) for c in (class_tunings or {})}
     for c, t in (class_tunings or {}).items():
@@ -569,7 +571,9 @@ def _plain_e2e(torch, ctx, inp, n_sample: int) -> dict:
         j = jobs[k]
         d_prof[: j.n_pulses * j.n_samples].copy_(torch.from_numpy(np.ascontiguousarray(prof).reshape(-1)))
         d_pos = torch.from_numpy(pos).to(dev)
-        ctx.backproject(d_prof.data_ptr(), d_pos.data_ptr(), grid, params, d_img.data_ptr(), stream)
+        pl = ctx.plan(pos, grid, params)
+        pl.execute(d_prof.data_ptr(), d_pos.data_ptr(), d_img.data_ptr(), stream)
+        pl.close()
         out = d_img[: j.side * j.side].cpu()
         del out
     torch.cuda.synchronize()

@@ -279,7 +279,13 @@ sarbp_status sarbp_get_tuning(sarbp_context* ctx, int32_t interp,
                               sarbp_tuning* tuning);
 /* Per size class: set (all-zero removes the class's entry) and get the
  * configuration in effect (the class's entry, else the all-class one, else
- * the built-in default). */
+ * the built-in default) for calls that give host positions (plans, batch
+ * jobs and server requests with h_pos).  Calls without them (the unplanned
+ * sarbp_backproject, plans and jobs with h_pos NULL) plan every window from
+ * the grid alone -- a disc of the tile's radius, other classes and other
+ * best tunings -- and keep their own per-class entries: the built-in
+ * position-free defaults, or what sarbp_autotune without h_pos stored; an
+ * all-class sarbp_set_tuning applies to both. */
 sarbp_status sarbp_set_class_tuning(sarbp_context* ctx, int32_t interp,
                                     int32_t precision, int32_t dtype,
                                     int32_t size_class, const sarbp_tuning* tuning);
@@ -306,7 +312,9 @@ typedef struct sarbp_autotune_result {
  * d_scratch receives the crop images and must hold scratch_pixels >=
  * crop_nx * crop_ny complex64 values.  The built-in default of the job's
  * size class stays unless a candidate is more than 2% faster.  The result is
- * stored for the job's size class (sarbp_size_class) only.  Synchronous.
+ * stored for the job's size class (sarbp_size_class) only, for calls with
+ * host positions if h_pos is given, else for calls without (see
+ * sarbp_set_class_tuning).  Synchronous.
  */
 sarbp_status sarbp_autotune(sarbp_context* ctx, const void* d_profiles,
                             const double* d_pos, const double* h_pos,

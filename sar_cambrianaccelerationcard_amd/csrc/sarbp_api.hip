@@ -24,11 +24,14 @@ sarbp_status hip_fail(hipError_t e, const char* what) {
               "%s: %s", what, hipGetErrorString(e));
 }
 
-// Tuning cache key: (interp, precision, dtype, size class); class kAllClasses
-// holds a tuning set for every class at once (sarbp_set_tuning).
+// Tuning cache key: (interp, precision, dtype, size class, whether the call
+// has host positions); class kAllClasses holds a tuning set for every class
+// at once (sarbp_set_tuning), with and without positions.  Without positions
+// a tile's window covers the disc of its radius, so the two kinds of calls
+// keep separate per-class entries.
 constexpr int kAllClasses = SARBP_SIZE_CLASSES;
-int tuning_key(int interp, int prec, int dtype, int cls) {
-  return ((interp * 10 + prec) * 10 + dtype) * 10 + cls;
+int tuning_key(int interp, int prec, int dtype, int cls, bool positions = true) {
+  return ((((interp * 10 + prec) * 10 + dtype) * 10 + cls) << 1) | (positions || cls == kAllClasses ? 0 : 1);
 }
 
 // ---- device scratch of the allocation-free batched launch ----------------
@@ -86,12 +89,14 @@ struct sarbp_plan {
 
 namespace sarbp {
 
-sarbp_tuning get_tuning(sarbp_context* ctx, int interp, int prec, int dtype, int cls) {
-  auto it = ctx->tuning.find(tuning_key(interp, prec, dtype, cls));
+// The context's tuning of a class, else its all-classes tuning, else the
+// built-in default (`positions`: for a plan that knows the antenna positions).
+sarbp_tuning get_tuning(sarbp_context* ctx, int interp, int prec, int dtype, int cls, bool positions = true) {
+  auto it = ctx->tuning.find(tuning_key(interp, prec, dtype, cls, positions));
   if (it != ctx->tuning.end()) return it->second;
   it = ctx->tuning.find(tuning_key(interp, prec, dtype, kAllClasses));
   if (it != ctx->tuning.end()) return it->second;
-  return default_tuning(interp, cls, prec);
+  return default_tuning(interp, cls, prec, positions);
 }
 
 sarbp_status launch(const BpArgs& a, const sarbp_params* p, const sarbp_tuning& t, hipStream_t s) {
@@ -169,7 +174,10 @@ sarbp_status sarbp_set_tuning(sarbp_context* ctx, int32_t interp, int32_t precis
   if (interp < 0 || interp > 2 || precision < 0 || precision > 3 || dtype < 0 || dtype > 1)
     return fail(SARBP_ERR_INVALID_ARGUMENT, "bad (interp, precision, dtype)");
   std::lock_guard<std::mutex> lk(ctx->mu);
-  for (int c = 0; c <= kAllClasses; ++c) ctx->tuning.erase(tuning_key(interp, precision, dtype, c));
+  for (int c = 0; c <= kAllClasses; ++c) {
+    ctx->tuning.erase(tuning_key(interp, precision, dtype, c, true));
+    ctx->tuning.erase(tuning_key(interp, precision, dtype, c, false));
+  }
   if (t->rows_per_lane == 0 && t->cols_per_lane == 0 && t->pulses_per_stage == 0) return SARBP_OK;
   sarbp_status st = check_tuning(precision, t);
   if (st != SARBP_OK) return st;
@@ -229,7 +237,8 @@ sarbp_status sarbp_backproject(sarbp_context* ctx, const void* d_profiles, const
   DeviceGuard dg(ctx->device);
   if (dg.err != hipSuccess) return hip_fail(dg.err, "hipSetDevice");
   const Fitted fit =
-      fit_tuning(get_tuning(ctx, params->interp, params->precision, params->dtype, size_class(grid, params, nullptr)),
+      fit_tuning(get_tuning(ctx, params->interp, params->precision, params->dtype, size_class(grid, params, nullptr),
+                            false),
                  params->precision, grid, params, nullptr);
   BpArgs a;
   st = plan_args(nullptr, grid, params, fit.t, fit.prec, &a, nullptr);
@@ -268,7 +277,8 @@ sarbp_status sarbp_plan_create(sarbp_context* ctx, const double* h_pos, const sa
     return hip_fail(dg.err, "hipSetDevice");
   }
   const Fitted fit =
-      fit_tuning(get_tuning(ctx, params->interp, params->precision, params->dtype, size_class(grid, params, h_pos)),
+      fit_tuning(get_tuning(ctx, params->interp, params->precision, params->dtype, size_class(grid, params, h_pos),
+                            h_pos != nullptr),
                  params->precision, grid, params, h_pos);
   const sarbp_tuning& tune = fit.t;
   std::memset(&pl->info, 0, sizeof pl->info);
@@ -370,7 +380,8 @@ sarbp_status sarbp_backproject_batch(sarbp_context* ctx, const sarbp_job* jobs, 
   for (int32_t j : order) {
     const sarbp_job& jb = jobs[j];
     if (jb.grid.nx == 0 || jb.grid.ny == 0) continue;
-    const Fitted f = fit_tuning(get_tuning(ctx, p0.interp, p0.precision, p0.dtype, size_class(&jb.grid, &jb.params, jb.h_pos)),
+    const Fitted f = fit_tuning(get_tuning(ctx, p0.interp, p0.precision, p0.dtype, size_class(&jb.grid, &jb.params, jb.h_pos),
+                                           jb.h_pos != nullptr),
                                 p0.precision, &jb.grid, &jb.params, jb.h_pos);
     JobDesc d;
     sarbp_status st = plan_args(jb.h_pos, &jb.grid, &jb.params, f.t, f.prec, &d.a, nullptr);
@@ -448,7 +459,8 @@ sarbp_status sarbp_backproject_batch_noalloc(sarbp_context* ctx, const sarbp_job
   for (int32_t r = 0; r < n_jobs; ++r) {
     const sarbp_job& jb = jobs[order[r]];
     if (jb.grid.nx == 0 || jb.grid.ny == 0) continue;
-    const Fitted f = fit_tuning(get_tuning(ctx, p0.interp, p0.precision, p0.dtype, size_class(&jb.grid, &jb.params, jb.h_pos)),
+    const Fitted f = fit_tuning(get_tuning(ctx, p0.interp, p0.precision, p0.dtype, size_class(&jb.grid, &jb.params, jb.h_pos),
+                                           jb.h_pos != nullptr),
                                 p0.precision, &jb.grid, &jb.params, jb.h_pos);
     JobDesc& d = g_noalloc_plan[n_planned];
     st = plan_args(jb.h_pos, &jb.grid, &jb.params, f.t, f.prec, &d.a, nullptr);
@@ -577,7 +589,7 @@ sarbp_status sarbp_autotune(sarbp_context* ctx, const void* d_profiles, const do
   };
   // default first (built-in, ignoring any stored tuning)
   const int cls = size_class(grid, params, h_pos);
-  const sarbp_tuning def = default_tuning(p.interp, cls, p.precision);
+  const sarbp_tuning def = default_tuning(p.interp, cls, p.precision, h_pos != nullptr);
   int eff = 0;
   double def_ms = INFINITY;
   st = time_one(def, &def_ms, &eff);
@@ -623,7 +635,7 @@ sarbp_status sarbp_autotune(sarbp_context* ctx, const void* d_profiles, const do
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (st != SARBP_OK) return st;
-  ctx->tuning[tuning_key(p.interp, p.precision, p.dtype, cls)] = best;
+  ctx->tuning[tuning_key(p.interp, p.precision, p.dtype, cls, h_pos != nullptr)] = best;
   out->best = best;
   out->best_ms = best_ms;
   out->default_ms = def_ms;

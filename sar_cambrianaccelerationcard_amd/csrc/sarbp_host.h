@@ -53,7 +53,10 @@ sarbp_status validate(const sarbp_grid* g, const sarbp_params* p);
 // Size class of a job (0 .. SARBP_SIZE_CLASSES - 1; sarbp_size_class).
 int size_class(const sarbp_grid* g, const sarbp_params* p, const double* h_pos);
 // Built-in tuning of an interpolator for a size class and precision mode.
-sarbp_tuning default_tuning(int interp, int cls, int precision);
+// Built-in tuning of a size class; `positions`: the plan knows the antenna
+// positions (without them a tile's window covers a disc of its radius, and
+// other tunings win).
+sarbp_tuning default_tuning(int interp, int cls, int precision, bool positions = true);
 // SARBP_ERR_INVALID_ARGUMENT unless the shape is compiled and 1 <= P <= 256.
 sarbp_status check_tuning(int precision, const sarbp_tuning* t);
 // A tuning fitted to one grid and the arithmetic (kRef, kF64, kF32, kDF) its

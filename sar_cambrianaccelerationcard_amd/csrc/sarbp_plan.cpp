@@ -175,7 +175,7 @@ sarbp_status validate(const sarbp_grid* g, const sarbp_params* p) {
   return SARBP_OK;
 }
 
-sarbp_tuning default_tuning(int interp, int cls, int precision) {
+sarbp_tuning default_tuning(int interp, int cls, int precision, bool positions) {
   // Built-in defaults: per size class (range windows of a 64 x 32 tile of
   // < 80, < 120, < 176, < 272 and >= 272 samples), the (tile shape x pulses
   // per stage) point of MI355X sweeps that is closest to the best on every
@@ -184,18 +184,24 @@ sarbp_tuning default_tuning(int interp, int cls, int precision) {
   // of each geometry's best; class 4 within 7%).  Wider windows fit fewer
   // pulses in a stage at the occupancy that wins.  Two-column tiles win on
   // some grids but lose up to 37% under long apertures, where their width
-  // adds range spread; the autotuner picks them where they pay.  Linear and
-  // nearest stage 32 pulses of their 8-byte windows.
+  // adds range spread; the autotuner picks them where they pay.  Without
+  // positions a tile's window covers the disc of its radius, so the class of
+  // a grid is higher and the squarer 64 x 32 tile (most pixels per radius)
+  // wins in every class: 4 / 4 / 3 / 2 / 2 pulses per stage, within 2-10% of
+  // each swept geometry's best (the planner lowers P where the LDS budget
+  // needs it).  Linear and nearest stage 32 pulses of their 8-byte windows.
   sarbp_tuning t;
   t.cols_per_lane = 1;
   t.rows_per_lane = 4;
   t.pulses_per_stage = 32;
   if (interp == SARBP_INTERP_SINC8) {
     static const int kShape[SARBP_SIZE_CLASSES][3] = {{1, 4, 8}, {1, 4, 6}, {1, 4, 4}, {1, 4, 2}, {1, 8, 1}};
+    static const int kShapeNoPos[SARBP_SIZE_CLASSES][3] = {{1, 8, 4}, {1, 8, 4}, {1, 8, 3}, {1, 8, 2}, {1, 8, 2}};
     const int c = std::min(std::max(cls, 0), SARBP_SIZE_CLASSES - 1);
-    t.cols_per_lane = kShape[c][0];
-    t.rows_per_lane = kShape[c][1];
-    t.pulses_per_stage = kShape[c][2];
+    const int* sh = positions ? kShape[c] : kShapeNoPos[c];
+    t.cols_per_lane = sh[0];
+    t.rows_per_lane = sh[1];
+    t.pulses_per_stage = sh[2];
   }
   // the F64 and F32 modes compile one column per lane
   if (!tile_shape_supported(precision, t.cols_per_lane, t.rows_per_lane)) t.cols_per_lane = 1;
