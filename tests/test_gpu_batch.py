@@ -176,3 +176,42 @@ def test_size_class_is_the_planned_window_class(ctx, jobs):
                 assert native.size_class(g, pa, sc.pos) == sum(info["window"] >= b for b in bounds)
     finally:
         ctx.set_tuning(*key, 0, 0, 0)
+
+
+def test_tunings_with_and_without_positions_are_separate(ctx, jobs):
+    """Calls without host positions plan windows over each tile's radius, so
+    they keep their own per-class tunings: a class tuning (for calls with
+    positions) does not reach them, an all-class tuning reaches both, and the
+    autotuner run without positions stores only theirs."""
+    import torch
+    scs, dss = jobs
+    sc, ds = scs[1], dss[1]
+    key = (native.INTERP_SINC8, native.PREC_REF, native.DTYPE_C64)
+    g, p = sc.native_grid(), sc.native_params(interp="sinc8")
+    tile = lambda pl: (pl.info()["cols_per_lane"], pl.info()["rows_per_lane"])
+    c_pos, c_free = native.size_class(g, p, sc.pos), native.size_class(g, p)
+    # built-in defaults (rows, cols, pulses per stage) of calls with positions
+    defaults = {0: (4, 1, 8), 1: (4, 1, 6), 2: (4, 1, 4), 3: (4, 1, 2), 4: (8, 1, 1)}
+    print(f"  class with positions {c_pos}, without {c_free}", flush=True)
+    try:
+        ctx.set_tuning(*key, 0, 0, 0)
+        assert ctx.get_class_tuning(*key, c_pos) == defaults[c_pos]
+        assert tile(ctx.plan(None, g, p)) == (1, 8)  # the position-free default: 64 x 32 tiles
+        ctx.set_class_tuning(*key, c_pos, 2, 2, 5)
+        ctx.set_class_tuning(*key, c_free, 2, 2, 5)
+        assert tile(ctx.plan(sc.pos, g, p)) == (2, 2)
+        assert tile(ctx.plan(None, g, p)) == (1, 8)
+        ctx.set_tuning(*key, 2, 2, 5)  # all classes, both kinds of call
+        assert tile(ctx.plan(None, g, p)) == (2, 2)
+        assert tile(ctx.plan(sc.pos, g, p)) == (2, 2)
+        ctx.set_tuning(*key, 0, 0, 0)
+        scratch = torch.empty(sc.grid["nx"] * sc.grid["ny"], dtype=torch.complex64, device="cuda")
+        stream = torch.cuda.current_stream().cuda_stream
+        r = ctx.autotune(ds.profiles_ptr("c64"), ds.pos.data_ptr(), None, g, p, scratch.data_ptr(),
+                         scratch.numel(), 256, stream)
+        print(f"  autotuned without positions: {r}", flush=True)
+        assert tile(ctx.plan(None, g, p)) == (r["cols_per_lane"], r["rows_per_lane"])
+        assert ctx.get_class_tuning(*key, c_pos) == defaults[c_pos]  # the positioned entry untouched
+        assert tile(ctx.plan(sc.pos, g, p)) == (1, defaults[c_pos][0])
+    finally:
+        ctx.set_tuning(*key, 0, 0, 0)
