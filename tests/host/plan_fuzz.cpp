@@ -114,14 +114,16 @@ void one_case(std::mt19937_64& rng, bool brute) {
   if (validate(&g, &p) != SARBP_OK) return;
   const int cls = size_class(&g, &p, h_pos);
   CHECK(cls >= 0 && cls < SARBP_SIZE_CLASSES, "size class %d", cls);
-  sarbp_tuning t = default_tuning(p.interp, cls, p.precision);
+  // the built-in default of the call's kind: with positions, or the
+  // position-free table (windows over the disc of each tile's radius)
+  sarbp_tuning t = default_tuning(p.interp, cls, p.precision, h_pos != nullptr);
   if (U(rng) < 0.5) {  // a random compiled shape and stage depth
     static const int shapes[][2] = {{1, 2}, {1, 4}, {1, 8}, {2, 2}, {2, 4}, {2, 8}, {4, 2}, {4, 4}};
     const int* sh = shapes[(int)(U(rng) * 8)];
     t.cols_per_lane = sh[0];
     t.rows_per_lane = sh[1];
     t.pulses_per_stage = 1 + (int)(U(rng) * 64);
-    if (check_tuning(p.precision, &t) != SARBP_OK) t = default_tuning(p.interp, cls, p.precision);
+    if (check_tuning(p.precision, &t) != SARBP_OK) t = default_tuning(p.interp, cls, p.precision, h_pos != nullptr);
   }
   const Fitted f = fit_tuning(t, p.precision, &g, &p, h_pos);
   CHECK(tile_shape_supported(f.prec, f.t.cols_per_lane, f.t.rows_per_lane), "shape %dx%d not compiled for %d",
