@@ -95,7 +95,7 @@ struct BatchHeader {
 __host__ __device__ inline int setup_batch(int P) { return P >= 64 ? P : P * (64 / P); }
 
 // LDS layout of one workgroup (byte offsets), SB = setup_batch(P):
-//   raw   [P][win] float2         staged range windows
+//   raw   [P][win] float2         staged range windows (nearest, linear)
 //   coef  [P][win-7] blocks       sinc8 only: the R = sinc_rows polynomial
 //                                 coefficients (float2) of the interpolant at
 //                                 each window position ([2][6] half bins or
@@ -116,7 +116,7 @@ __host__ __device__ inline LdsLayout lds_layout(int interp, int win, int P) {
   const int SB = setup_batch(P);
   size_t o = 0;
   L.raw = o;
-  o += (size_t)P * win * 8;
+  if (sinc_rows(interp) == 0) o += (size_t)P * win * 8;  // sinc8 reads its taps from the profile
   L.coef = o;
   o += (size_t)P * (win - 7) * coef_stride16(interp) * 16;
   L.par = o;
