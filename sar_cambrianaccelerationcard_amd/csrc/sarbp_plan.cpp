@@ -180,23 +180,23 @@ sarbp_tuning default_tuning(int interp, int cls, int precision, bool positions) 
   // < 80, < 120, < 176, < 272 and >= 272 samples), the (tile shape x pulses
   // per stage) point of MI355X sweeps that is closest to the best on every
   // swept geometry -- 2048 pulses over 2048^2 pixels and 8192 pulses over
-  // 1536^2, 0.25 / 0.35 / 0.5 / 0.7 / 1 m pixels at X band, 5 km (within 6%
-  // of each geometry's best; class 4 within 7%).  Wider windows fit fewer
-  // pulses in a stage at the occupancy that wins.  Two-column tiles win on
-  // some grids but lose up to 37% under long apertures, where their width
-  // adds range spread; the autotuner picks them where they pay.  Without
-  // positions a tile's window covers the disc of its radius, so the class of
-  // a grid is higher and the squarer 64 x 32 tile (most pixels per radius)
-  // wins in every class: 4 / 4 / 3 / 2 / 2 pulses per stage, within 2-10% of
-  // each swept geometry's best (the planner lowers P where the LDS budget
-  // needs it).  Linear and nearest stage 32 pulses of their 8-byte windows.
+  // 1536^2, 0.25 / 0.35 / 0.5 / 0.7 / 1 m pixels at X band, 5 km: within 0-9%
+  // of each geometry's best.  Wider windows fit fewer pulses in a stage at
+  // the occupancy that wins.  Two-column tiles win on some grids but lose up
+  // to 37% under long apertures, where their width adds range spread; the
+  // autotuner picks them where they pay.  Without positions a tile's window
+  // covers the disc of its radius, so the class of a grid is higher and the
+  // squarer 64 x 32 tile (most pixels per radius) wins in most classes
+  // (within 0-8% of the best of its swept geometries; at 0.25 m under 8192
+  // pulses, 18%).  The planner lowers P where the LDS budget needs it.
+  // Linear and nearest stage 32 pulses of their 8-byte windows.
   sarbp_tuning t;
   t.cols_per_lane = 1;
   t.rows_per_lane = 4;
   t.pulses_per_stage = 32;
   if (interp == SARBP_INTERP_SINC8) {
-    static const int kShape[SARBP_SIZE_CLASSES][3] = {{1, 4, 8}, {1, 4, 6}, {1, 4, 4}, {1, 4, 2}, {1, 8, 1}};
-    static const int kShapeNoPos[SARBP_SIZE_CLASSES][3] = {{1, 8, 4}, {1, 8, 4}, {1, 8, 3}, {1, 8, 2}, {1, 8, 2}};
+    static const int kShape[SARBP_SIZE_CLASSES][3] = {{1, 4, 8}, {1, 4, 6}, {1, 4, 4}, {1, 4, 3}, {1, 4, 2}};
+    static const int kShapeNoPos[SARBP_SIZE_CLASSES][3] = {{1, 8, 4}, {1, 8, 3}, {1, 4, 3}, {1, 8, 3}, {1, 8, 2}};
     const int c = std::min(std::max(cls, 0), SARBP_SIZE_CLASSES - 1);
     const int* sh = positions ? kShape[c] : kShapeNoPos[c];
     t.cols_per_lane = sh[0];

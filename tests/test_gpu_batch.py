@@ -191,7 +191,7 @@ def test_tunings_with_and_without_positions_are_separate(ctx, jobs):
     tile = lambda pl: (pl.info()["cols_per_lane"], pl.info()["rows_per_lane"])
     c_pos, c_free = native.size_class(g, p, sc.pos), native.size_class(g, p)
     # built-in defaults (rows, cols, pulses per stage) of calls with positions
-    defaults = {0: (4, 1, 8), 1: (4, 1, 6), 2: (4, 1, 4), 3: (4, 1, 2), 4: (8, 1, 1)}
+    defaults = {0: (4, 1, 8), 1: (4, 1, 6), 2: (4, 1, 4), 3: (4, 1, 3), 4: (4, 1, 2)}
     print(f"  class with positions {c_pos}, without {c_free}", flush=True)
     try:
         ctx.set_tuning(*key, 0, 0, 0)
