@@ -396,9 +396,12 @@ int size_class(const sarbp_grid* g, const sarbp_params* p, const double* h_pos) 
 // LDS range window (samples) of a tile of px x py pixels per lane and the
 // interpolation kernel (kernel_interp) it runs; 0 if the window exceeds 8192
 // samples.  Sinc8 runs half bins unless the tile is coarse against the range
-// bins: per pixel-pulse the half-bin kernel issues ~4 VALU instructions fewer
-// in the pixel loop and ~0.6 * 76 / (tile pixels) more per window position
-// (about 60% of the window's positions are touched per pulse).
+// bins: per pixel-pulse the half-bin kernel issues 4 FMA fewer in the pixel
+// loop and ~0.6 * 60 / (tile pixels) more issue slots per window position
+// (about 60% of the window's positions are touched per pulse; 60 slots is
+// where measured grids cross over: 64 x 16 tiles at 0.5 m, 100-120-sample
+// windows, run 1.5-1.7% faster in half bins, at 0.7 m, 152 samples, 9%
+// slower).
 int plan_window(const sarbp_grid* g, const sarbp_params* p, int px, int py, const double* h_pos,
                 int* kinterp) {
   const Extent ext = tile_extent(g, px, py, h_pos, p->n_pulses);
@@ -409,7 +412,7 @@ int plan_window(const sarbp_grid* g, const sarbp_params* p, int px, int py, cons
   const int win = ((2 * (int)half_bins + 2) + 3) & ~3;  // multiple of 4 samples
   if (p->interp == SARBP_INTERP_SINC8) {
     const double tile_px = (double)kTileX * px * kWaves * py;
-    *kinterp = 0.6 * 76.0 * (win - 7) < 4.0 * tile_px ? kSinc8 : kSinc8W;
+    *kinterp = 0.6 * 60.0 * (win - 7) < 4.0 * tile_px ? kSinc8 : kSinc8W;
   }
   return win;
 }
