@@ -22,7 +22,8 @@ Extra legs (reported in the same JSON line):
             64 sinc8, 33 linear; peak 78.6 Top/s = 256 CU x 64 lanes x 2 x
             2.4 GHz) and the conventional FLOP view (157.3 TFLOP/s)
   cpu_baseline  the FP64 CPU oracle on a bounded row sample of the image
-  accuracy  rel-L2 of GPU rows against those oracle rows
+  accuracy  rel-L2 of GPU rows against those oracle rows, and on 64 x 64
+            chips around four point targets (peak pixel, phase, PSLR/ISLR)
   range_compression  the step before BP on the device: raw echoes of the
             same scene size (4096 pulses x 2448 samples) -> 8192 4x-upsampled
             samples, against the HBM roofline (8 TB/s), with the FP64 oracle
@@ -376,6 +377,38 @@ def median_time(fn, runs: int = 3) -> float:
     return float(np.median(ts))
 
 
+def headline_chips(sc, gpu_img, p, nthr, n_chips: int = 4, half: int = 32) -> dict:
+    """The GPU image against the FP64 oracle on 2half x 2half chips centred on
+    the first n_chips point targets at least `half` pixels inside the image:
+    combined rel-L2, and per chip its rel-L2, whether the brightest pixel is
+    the oracle's, the phase error at that peak and the largest PSLR / ISLR
+    difference along the peak's cuts."""
+    import oracle.oracle as O
+    import oracle.spot as SP
+    from sar_cambrianaccelerationcard_amd import metrics as M
+    nx, ny = sc.grid["nx"], sc.grid["ny"]
+    tgt = [(int(tx), int(ty)) for tx, ty in sc.target_pixels
+           if half <= tx < nx - half and half <= ty < ny - half][:n_chips]
+    num = den = 0.0
+    chips = []
+    for tx, ty in tgt:
+        x0, y0 = tx - half, ty - half
+        ref = O.backproject(sc.profiles, sc.pos, SP.chip_grid(sc.grid, x0, y0, 2 * half), p, n_threads=nthr)
+        got = gpu_img[y0:y0 + 2 * half, x0:x0 + 2 * half]
+        num += float(np.sum(np.abs(got.astype(np.complex128) - ref) ** 2))
+        den += float(np.sum(np.abs(ref) ** 2))
+        rec = M.compare_targets(got, ref, np.array([[half, half]]))[0]
+        chips.append({"target": [tx, ty], "rel_l2": M.rel_l2(got, ref), "peak_match": bool(rec["peak_match"]),
+                      "phase_err_deg": rec["phase_err_deg"], "max_pslr_islr_diff_db": rec["max_quality_diff_db"]})
+    out = {"rel_l2_chips": float(np.sqrt(num / den)) if den > 0 else None, "chips": chips}
+    if chips:
+        log(f"GPU target chips ({len(chips)} x {2 * half}^2 px) vs FP64 oracle: rel-L2 {out['rel_l2_chips']:.3e}, "
+            f"peaks {'exact' if all(c['peak_match'] for c in chips) else 'MISMATCH'}, max |phase err| "
+            f"{max(abs(c['phase_err_deg']) for c in chips):.2e} deg, max PSLR/ISLR diff "
+            f"{max(c['max_pslr_islr_diff_db'] for c in chips):.2e} dB")
+    return out
+
+
 def leg_cpu(sc, gpu_img, interp, args):
     """The reference's CPU BP semantics (BPA_SAR.m:178-197) restated in C
     (oracle/sarbp_oracle.c): FP64 and FP32, all host threads and one thread,
@@ -396,6 +429,7 @@ def leg_cpu(sc, gpu_img, interp, args):
     ref = O.backproject(sc.profiles, sc.pos, sc.grid, p, rows=(r_lo, r_hi), n_threads=nthr)
     t_warm = time.perf_counter() - tc
     acc = {"rel_l2_rows": M.rel_l2(gpu_img[r_lo:r_hi], ref), "rows": [r_lo, r_hi]}
+    acc.update(headline_chips(sc, gpu_img, p, nthr))
     t64 = median_time(lambda: O.backproject(sc.profiles, sc.pos, sc.grid, p, rows=(r_lo, r_hi),
                                             n_threads=nthr))
     f32 = O.backproject_f32(sc.profiles, sc.pos, sc.grid, p, rows=(r_lo, r_hi), n_threads=nthr)
