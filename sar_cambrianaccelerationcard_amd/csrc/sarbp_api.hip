@@ -392,7 +392,7 @@ sarbp_status sarbp_backproject_batch(sarbp_context* ctx, const sarbp_job* jobs, 
     d.item_begin = 0;
     planned.push_back(d);
     tunes.push_back(f.t);
-    keys.push_back(group_key(d.a, f.t));
+    keys.push_back(group_key(d.a, f.t, p0.dtype));
   }
   const int max_groups = (int)(sizeof ctx->batch / sizeof ctx->batch[0]);
   std::vector<int> gid(planned.size());
@@ -469,7 +469,7 @@ sarbp_status sarbp_backproject_batch_noalloc(sarbp_context* ctx, const sarbp_job
     d.a.pos = jb.d_pos;
     d.a.img = reinterpret_cast<float2*>(jb.d_image);
     tunes[n_planned] = f.t;
-    keys[n_planned] = group_key(d.a, f.t);
+    keys[n_planned] = group_key(d.a, f.t, p0.dtype);
     ++n_planned;
   }
   if (n_planned == 0) return SARBP_OK;

@@ -2,6 +2,9 @@
 // per configuration into a hipGraph and replayed over a device job table.
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "sarbp_host.h"
 
@@ -52,6 +55,27 @@ int batch_blocks_per_cu(int interp, int precision, int dtype, int px, int py, si
     default: return 1;
   }
 }
+
+namespace {
+// The occupancy source of planning (set_occupancy_source): the HIP query for
+// the batched kernel, cached per kernel and LDS size (a plan asks for a few
+// stage depths; every gfx950 device gives the same answer).
+int cached_occupancy(int kinterp, int kprec, int dtype, int px, int py, size_t shm) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int, int, int, size_t>, int> cache;
+  const auto key = std::make_tuple(kinterp, kprec, dtype, px, py, shm);
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    const auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+  }
+  const int n = batch_blocks_per_cu(kinterp, kprec, dtype, px, py, shm);
+  std::lock_guard<std::mutex> lk(mu);
+  cache.emplace(key, n);
+  return n;
+}
+const bool g_occupancy_registered = (set_occupancy_source(&cached_occupancy), true);
+}  // namespace
 
 BatchExec::~BatchExec() {
   if (done_) {
@@ -123,10 +147,14 @@ sarbp_status BatchExec::run(std::vector<JobDesc>& jobs, int interp, int precisio
   if (items == 0) return SARBP_OK;
   // LDS size class: the largest size (256-B steps) that keeps as many
   // workgroups per CU as the group's largest job needs, so the class costs no
-  // occupancy and at most ~16 graphs are captured per kernel.
+  // occupancy and at most ~16 graphs are captured per kernel.  The estimate
+  // bounds the static LDS from above; the occupancy query confirms it, else
+  // the job's own size is the class.
+  const int per_cu_job = cached_occupancy(interp, precision, dtype, px, py, shm);
   const size_t per_cu_fit = std::max<size_t>(1, kCuLds / (shm + kBatchStaticLds));
-  const size_t shm_class =
-      std::min(kLdsBudget, std::max(shm, (kCuLds / per_cu_fit - kBatchStaticLds) & ~(size_t)255));
+  size_t shm_class = std::min(kLdsBudget, std::max(shm, (kCuLds / per_cu_fit - kBatchStaticLds) & ~(size_t)255));
+  if (shm_class != shm && cached_occupancy(interp, precision, dtype, px, py, shm_class) < per_cu_job)
+    shm_class = shm;
   std::memcpy(h_jobs_, jobs.data(), sizeof(JobDesc) * jobs.size());
   h_hdr_->n_items = items;
   h_hdr_->n_jobs = (int32_t)jobs.size();

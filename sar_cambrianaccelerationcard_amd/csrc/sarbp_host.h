@@ -72,8 +72,18 @@ struct Fitted {
 // radius limit) with one column per lane.
 Fitted fit_tuning(sarbp_tuning t, int precision, const sarbp_grid* g, const sarbp_params* p,
                   const double* h_pos);
+// Workgroups per CU of the batched BP kernel of interpolator kinterp,
+// arithmetic kprec, profile dtype and tile shape px x py with `shm` bytes of
+// dynamic LDS (registers, LDS and its allocation granularity counted), as the
+// device library's registered source (the HIP occupancy query) reports it; 0
+// if no source is registered (host-only builds).
+using OccupancyFn = int (*)(int kinterp, int kprec, int dtype, int px, int py, size_t shm);
+void set_occupancy_source(OccupancyFn fn);
+int kernel_occupancy(int kinterp, int kprec, int dtype, int px, int py, size_t shm);
 // Fills a launch descriptor (device pointers left NULL) for one image, with
-// kernel arithmetic kprec.
+// kernel arithmetic kprec.  The tuning's pulses per stage is an upper bound:
+// while one pulse fewer per stage lets more workgroups share a CU
+// (kernel_occupancy), the stage is made one pulse shallower.
 sarbp_status plan_args(const double* h_pos, const sarbp_grid* g, const sarbp_params* p,
                        const sarbp_tuning& tune, int kprec, BpArgs* a, sarbp_plan_info* info);
 // Launches the plain kernel of a planned descriptor (its kernel_interp and
@@ -88,7 +98,7 @@ sarbp_status launch(const BpArgs& a, const sarbp_params* p, const sarbp_tuning& 
 struct GroupKey {
   int px, py, kinterp, kprec, occ;
 };
-GroupKey group_key(const BpArgs& a, const sarbp_tuning& t);
+GroupKey group_key(const BpArgs& a, const sarbp_tuning& t, int dtype);
 // Assigns keys[0..n) to groups in first-seen order (group indices to out).
 // Keys that differ only in occ share a group when keeping them apart would
 // take more than max_groups groups.  Returns the number of groups, or -1 if

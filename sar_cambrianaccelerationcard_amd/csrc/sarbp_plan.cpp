@@ -8,6 +8,7 @@
 #include "sarbp_host.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -32,6 +33,15 @@ sarbp_status fail(sarbp_status st, const char* fmt, ...) {
 }
 
 constexpr double kPi = 3.14159265358979323846;
+
+std::atomic<OccupancyFn> g_occupancy_source{nullptr};
+
+void set_occupancy_source(OccupancyFn fn) { g_occupancy_source.store(fn); }
+
+int kernel_occupancy(int kinterp, int kprec, int dtype, int px, int py, size_t shm) {
+  const OccupancyFn fn = g_occupancy_source.load();
+  return fn ? fn(kinterp, kprec, dtype, px, py, shm) : 0;
+}
 
 double bessel_i0(double x) {
   double q = 0.25 * x * x, term = 1.0, sum = 1.0;
@@ -447,6 +457,15 @@ sarbp_status plan_args(const double* h_pos, const sarbp_grid* g,
   while (P > 1 && lds_layout(kinterp, win, P).total > kLdsBudget) --P;
   if (lds_layout(kinterp, win, P).total > kLdsBudget)
     return fail(SARBP_ERR_UNSUPPORTED, "range window of %d samples does not fit LDS", win);
+  // A tuning's stage depth is set for its class's typical window; a longer
+  // window can push the stage's LDS over a workgroups-per-CU step (measured on
+  // an 84-sample window, 1 x 4 px per lane: 6 pulses per stage at 3 workgroups
+  // per CU 541 ms, 5 pulses at 4 per CU 495 ms).  Take the shallower stage
+  // while it gains a workgroup per CU.
+  auto per_cu = [&](int q) {
+    return kernel_occupancy(kinterp, kprec, p->dtype, px, py, lds_layout(kinterp, win, q).total);
+  };
+  while (P > 1 && per_cu(P - 1) > per_cu(P)) --P;
 
   std::memset(a, 0, sizeof *a);
   for (int k = 0; k < 3; ++k) {
@@ -500,10 +519,11 @@ sarbp_status plan_args(const double* h_pos, const sarbp_grid* g,
   return SARBP_OK;
 }
 
-GroupKey group_key(const BpArgs& a, const sarbp_tuning& t) {
+GroupKey group_key(const BpArgs& a, const sarbp_tuning& t, int dtype) {
   const size_t shm = lds_layout(a.kernel_interp, a.win, a.pulses_per_stage).total;
-  return GroupKey{t.cols_per_lane, t.rows_per_lane, a.kernel_interp, a.kernel_prec,
-                  (int)std::max<size_t>(1, kCuLds / (shm + kBatchStaticLds))};
+  int occ = kernel_occupancy(a.kernel_interp, a.kernel_prec, dtype, t.cols_per_lane, t.rows_per_lane, shm);
+  if (occ <= 0) occ = (int)std::max<size_t>(1, kCuLds / (shm + kBatchStaticLds));
+  return GroupKey{t.cols_per_lane, t.rows_per_lane, a.kernel_interp, a.kernel_prec, occ};
 }
 
 int assign_groups(const GroupKey* k, int n, int max_groups, int* out) {

@@ -331,7 +331,7 @@ void sarbp_server::sched_loop() {
         d.item_begin = 0;
         chunks.push_back(d);
         tunes.push_back(j.tune);
-        keys.push_back(group_key(d.a, j.tune));
+        keys.push_back(group_key(d.a, j.tune, cfg.dtype));
         j.next_pulse = d.a.pulse_end;
         j.started = true;
         ++j.inflight;
