@@ -235,8 +235,11 @@ const char* sarbp_last_error_detail(void);
 /* Kernel configuration knobs chosen by the autotuner.  A workgroup's tile is
  * (64 cols_per_lane) x (4 rows_per_lane) pixels: each lane accumulates
  * cols_per_lane columns (64 apart) of rows_per_lane rows.  pulses_per_stage
- * is the number of pulses staged in LDS per barrier interval (reduced by the
- * planner until the stage fits 64 KiB of LDS).  Shapes: rows 2, 4 or 8 with
+ * is the number of pulses staged in LDS per barrier interval, an upper bound:
+ * the planner reduces it until the stage fits 64 KiB of LDS, then by one
+ * pulse at a time while one pulse fewer lets one more workgroup share a CU
+ * (a job's window sets the LDS per pulse; sarbp_plan_get_info reports the
+ * depth planned).  Shapes: rows 2, 4 or 8 with
  * cols 1 or 2; rows 2 or 4 with cols 4 (SARBP_PREC_REF); cols 1 otherwise. */
 typedef struct sarbp_tuning {
   int32_t rows_per_lane;

@@ -215,3 +215,39 @@ def test_tunings_with_and_without_positions_are_separate(ctx, jobs):
         assert tile(ctx.plan(sc.pos, g, p)) == (1, defaults[c_pos][0])
     finally:
         ctx.set_tuning(*key, 0, 0, 0)
+
+
+def test_stage_depth_fitted_to_occupancy(ctx, jobs):
+    """A tuning's pulses per stage is an upper bound: the planned depth never
+    exceeds it and grows with it, a tuned depth whose stage LDS would cost a
+    workgroup per CU plans shallower, and every depth gives the same image bit
+    for bit (each pixel adds its pulses in pulse order whatever the stages)."""
+    scs, dss = jobs
+    sc, ds = scs[0], dss[0]
+    key = (native.INTERP_SINC8, native.PREC_REF, native.DTYPE_C64)
+    g, p = sc.native_grid(), sc.native_params(interp="sinc8")
+    try:
+        planned = {}
+        for P in range(1, 13):
+            ctx.set_tuning(*key, 4, 1, P)
+            i = ctx.plan(sc.pos, g, p).info()
+            assert (i["cols_per_lane"], i["rows_per_lane"]) == (1, 4)
+            planned[P] = (i["pulses_per_stage"], i["lds_bytes"])
+        print(f"  window {i['window']}: tuned -> planned (depth, LDS) {planned}", flush=True)
+        depths = [planned[P][0] for P in range(1, 13)]
+        assert all(1 <= d <= P for P, d in zip(range(1, 13), depths))
+        assert depths == sorted(depths)
+        assert any(d < P for P, d in zip(range(1, 13), depths))
+        # a planned depth is a fixed point: tuning it plans it again
+        for d, lds in set(planned.values()):
+            assert planned[d] == (d, lds)
+        ref = None
+        for P in (1, 3, 12):
+            ctx.set_tuning(*key, 4, 1, P)
+            img = D.backproject(ctx, ds, interp="sinc8", plan=True).cpu().numpy()
+            if ref is None:
+                ref = img
+            else:
+                assert np.array_equal(img.view(np.uint64), ref.view(np.uint64))
+    finally:
+        ctx.set_tuning(*key, 0, 0, 0)
