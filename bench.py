@@ -891,12 +891,16 @@ def main() -> int:
 
     rc = leg_range(torch, ctx, native, D, args) if args.range_pulses > 0 else None
     wk = leg_omega_k(torch, ctx, native, args) if args.omega_k else None
-    head_cls = native.size_class(sc.native_grid(), params, sc.pos)
-    tuned_classes = (head_cls,) if tune else ()
-    # the headline class's tuning, autotuned on a whole 4096^2 image: the
-    # served stream's large jobs (8192^2, 96% of its work) are of this class
-    head_tuning = {head_cls: ctx.get_class_tuning(I, P, Dt, head_cls)} if tune else {}
-    batched = leg_mixed(torch, ctx, native, W, args, tuned_classes) if args.mixed_jobs > 0 else None
+    # The legs below start from the built-in per-class tunings: the served
+    # stream's large jobs (8192^2 from 8192 pulses, 96% of its work) share the
+    # headline's size class but not its best tile (the autotuner's pick on the
+    # 4096-pulse image can be 1 x 8 px per lane, whose 64 x 32 tiles need
+    # 108-sample windows on them); the class default runs them with the stage
+    # depth the planner fits to their 84-sample windows, and the mixed leg
+    # tunes every class of its own jobs.
+    ctx.set_tuning(I, P, Dt, 0, 0, 0)
+    head_tuning = {}
+    batched = leg_mixed(torch, ctx, native, W, args) if args.mixed_jobs > 0 else None
     served = None
     if args.served_jobs > 0:
         inp = ServedInputs(W, args)
@@ -941,8 +945,12 @@ def main() -> int:
         t = json.load(open(tpath))
         if (t["workload"], t["interp"], t["precision"], t["dtype"]) == \
                 (cfg["name"], interp, args.precision, args.dtype):
-            traffic = t["traffic_bytes"]
-            traffic_tuning = t["tuning"]  # the launch the counters measured (tile shape sets the re-fetch)
+            # the counters of the launch the headline ran (its tile shape and
+            # stage depth set the re-fetch), if they were measured
+            planned = {k: info[k] for k in ("cols_per_lane", "rows_per_lane", "pulses_per_stage")}
+            for launch in t["launches"]:
+                if launch["tuning"] == planned:
+                    traffic, traffic_tuning = launch["traffic_bytes"], launch["tuning"]
     line = {
         "metric": "pixel·pulse back-projections/s at rel-L2≤1e-4 vs FP64; p50/p99 job latency",
         "value": wps,

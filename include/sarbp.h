@@ -238,8 +238,10 @@ const char* sarbp_last_error_detail(void);
  * is the number of pulses staged in LDS per barrier interval, an upper bound:
  * the planner reduces it until the stage fits 64 KiB of LDS, then by one
  * pulse at a time while one pulse fewer lets one more workgroup share a CU
- * (a job's window sets the LDS per pulse; sarbp_plan_get_info reports the
- * depth planned).  Shapes: rows 2, 4 or 8 with
+ * (a job's window sets the LDS per pulse), down to 4 pulses for a plain
+ * launch (sarbp_plan_get_info reports the depth planned) and to 1 for a job
+ * of a batch or a server, whose persistent launch gains more from occupancy.
+ * The depth changes no bit of an image.  Shapes: rows 2, 4 or 8 with
  * cols 1 or 2; rows 2 or 4 with cols 4 (SARBP_PREC_REF); cols 1 otherwise. */
 typedef struct sarbp_tuning {
   int32_t rows_per_lane;

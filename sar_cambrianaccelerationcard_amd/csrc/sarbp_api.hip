@@ -384,7 +384,7 @@ sarbp_status sarbp_backproject_batch(sarbp_context* ctx, const sarbp_job* jobs, 
                                            jb.h_pos != nullptr),
                                 p0.precision, &jb.grid, &jb.params, jb.h_pos);
     JobDesc d;
-    sarbp_status st = plan_args(jb.h_pos, &jb.grid, &jb.params, f.t, f.prec, &d.a, nullptr);
+    sarbp_status st = plan_args(jb.h_pos, &jb.grid, &jb.params, f.t, f.prec, &d.a, nullptr, true);
     if (st != SARBP_OK) return fail(st, "job %d: %s", j, last_detail());
     d.a.prof = jb.d_profiles;
     d.a.pos = jb.d_pos;
@@ -463,7 +463,7 @@ sarbp_status sarbp_backproject_batch_noalloc(sarbp_context* ctx, const sarbp_job
                                            jb.h_pos != nullptr),
                                 p0.precision, &jb.grid, &jb.params, jb.h_pos);
     JobDesc& d = g_noalloc_plan[n_planned];
-    st = plan_args(jb.h_pos, &jb.grid, &jb.params, f.t, f.prec, &d.a, nullptr);
+    st = plan_args(jb.h_pos, &jb.grid, &jb.params, f.t, f.prec, &d.a, nullptr, true);
     if (st != SARBP_OK) return fail(st, "job %d: %s", order[r], last_detail());
     d.a.prof = jb.d_profiles;
     d.a.pos = jb.d_pos;

@@ -83,9 +83,11 @@ int kernel_occupancy(int kinterp, int kprec, int dtype, int px, int py, size_t s
 // Fills a launch descriptor (device pointers left NULL) for one image, with
 // kernel arithmetic kprec.  The tuning's pulses per stage is an upper bound:
 // while one pulse fewer per stage lets more workgroups share a CU
-// (kernel_occupancy), the stage is made one pulse shallower.
+// (kernel_occupancy), the stage is made one pulse shallower, down to 4 pulses
+// for a plain launch and to 1 for a batched one (`batched`).
 sarbp_status plan_args(const double* h_pos, const sarbp_grid* g, const sarbp_params* p,
-                       const sarbp_tuning& tune, int kprec, BpArgs* a, sarbp_plan_info* info);
+                       const sarbp_tuning& tune, int kprec, BpArgs* a, sarbp_plan_info* info,
+                       bool batched = false);
 // Launches the plain kernel of a planned descriptor (its kernel_interp and
 // kernel_prec, the dtype of p).
 sarbp_status launch(const BpArgs& a, const sarbp_params* p, const sarbp_tuning& t, hipStream_t s);

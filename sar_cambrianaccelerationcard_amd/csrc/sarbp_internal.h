@@ -103,8 +103,9 @@ __host__ __device__ inline int setup_batch(int P) { return P >= 64 ? P : P * (64
 //   par   [SB] 48-B pulse records
 //   rng   [SB] int2: first staged sample, first sinc8 position of each pulse
 //   start [SB] int window starts
-//   stage [SB/P][3] int per stage: range-expansion order, staged samples and
-//                   sinc8 positions per pulse
+//   stage [SB/P][3] int per stage: the set of its pulses' range-expansion
+//                   orders (bit o), staged samples and sinc8 positions per
+//                   pulse
 //   poly  [R][4] float  sinc8 only: the tap-polynomial rows (read as
 //                       broadcasts)
 struct LdsLayout {

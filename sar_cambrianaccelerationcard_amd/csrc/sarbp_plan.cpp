@@ -438,7 +438,7 @@ double df_quant(const sarbp_grid* g, int px, int py) {
 // Fills a launch descriptor.  Caller holds ctx->mu and the device guard.
 sarbp_status plan_args(const double* h_pos, const sarbp_grid* g,
                        const sarbp_params* p, const sarbp_tuning& tune, int kprec, BpArgs* a,
-                       sarbp_plan_info* info) {
+                       sarbp_plan_info* info, bool batched) {
   const int px = tune.cols_per_lane, py = tune.rows_per_lane;
   if (!tile_shape_supported(kprec, px, py))
     return fail(SARBP_ERR_INVALID_ARGUMENT, "unsupported tile shape: %d cols x %d rows per lane", px, py);
@@ -461,11 +461,19 @@ sarbp_status plan_args(const double* h_pos, const sarbp_grid* g,
   // window can push the stage's LDS over a workgroups-per-CU step (measured on
   // an 84-sample window, 1 x 4 px per lane: 6 pulses per stage at 3 workgroups
   // per CU 541 ms, 5 pulses at 4 per CU 495 ms).  Take the shallower stage
-  // while it gains a workgroup per CU.
+  // while it gains a workgroup per CU.  A plain launch stops at 4 pulses:
+  // below that its barrier per stage costs more than the occupancy gains (a
+  // 152-sample window, 2048^2 x 2048 pulses: 3 pulses at 3 per CU 11.4 ms, 2
+  // at 4 per CU 12.3 ms; a 248-sample window: 2 pulses 16.7 ms, 1 pulse
+  // 18.0 ms).  A batched launch's persistent workgroups gain more from
+  // occupancy and fit down to 1 pulse (64 mixed jobs: 191.7 ms that way,
+  // 217.0 ms stopping at 4, same box).  The depth changes no result bit:
+  // each pixel adds its pulses in pulse order whatever the stages.
+  const int min_stage = batched ? 1 : 4;
   auto per_cu = [&](int q) {
     return kernel_occupancy(kinterp, kprec, p->dtype, px, py, lds_layout(kinterp, win, q).total);
   };
-  while (P > 1 && per_cu(P - 1) > per_cu(P)) --P;
+  while (P > min_stage && per_cu(P - 1) > per_cu(P)) --P;
 
   std::memset(a, 0, sizeof *a);
   for (int k = 0; k < 3; ++k) {

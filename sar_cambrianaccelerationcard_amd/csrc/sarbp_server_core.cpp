@@ -514,7 +514,8 @@ sarbp_status sarbp::server_submit(sarbp_server* s, const sarbp_request* req, con
   }
   const Fitted fit = fit_tuning(base, p.precision, &req->grid, &p, req->h_pos);
   j->tune = fit.t;
-  if ((st = plan_args(req->h_pos, &req->grid, &p, j->tune, fit.prec, &j->args, nullptr)) != SARBP_OK) return st;
+  if ((st = plan_args(req->h_pos, &req->grid, &p, j->tune, fit.prec, &j->args, nullptr, true)) != SARBP_OK)
+    return st;
   j->prof_bytes = (size_t)p.n_pulses * p.n_samples * (p.dtype == SARBP_DTYPE_C64 ? 8 : 4);
   j->pos_bytes = (size_t)p.n_pulses * 24;
   j->img_bytes = npix * 8;

@@ -241,13 +241,19 @@ def test_stage_depth_fitted_to_occupancy(ctx, jobs):
         # a planned depth is a fixed point: tuning it plans it again
         for d, lds in set(planned.values()):
             assert planned[d] == (d, lds)
-        ref = None
-        for P in (1, 3, 12):
-            ctx.set_tuning(*key, 4, 1, P)
-            img = D.backproject(ctx, ds, interp="sinc8", plan=True).cpu().numpy()
-            if ref is None:
-                ref = img
-            else:
-                assert np.array_equal(img.view(np.uint64), ref.view(np.uint64))
+        # the same grid, and coarser ones whose tiles cross from the second-
+        # to the exact range expansion over the aperture (pulses of one stage
+        # with different orders)
+        grids = [None] + [dict(sc.grid, ux=[f * v for v in sc.grid["ux"]], uy=[f * v for v in sc.grid["uy"]])
+                          for f in (1.5, 1.6, 1.7)]
+        for grid in grids:
+            ref = None
+            for P in (1, 2, 3, 5, 12):
+                ctx.set_tuning(*key, 4, 1, P)
+                img = D.backproject(ctx, ds, interp="sinc8", plan=True, grid=grid).cpu().numpy()
+                if ref is None:
+                    ref = img
+                else:
+                    assert np.array_equal(img.view(np.uint64), ref.view(np.uint64))
     finally:
         ctx.set_tuning(*key, 0, 0, 0)
