@@ -57,20 +57,22 @@ int batch_blocks_per_cu(int interp, int precision, int dtype, int px, int py, si
 }
 
 namespace {
+// Held across every graph capture of a batched launch, and across the
+// occupancy queries of planning: a query can load a kernel's code object,
+// which must not overlap another thread's capture (a server plans jobs on its
+// client and admission threads while its scheduler thread captures).
+std::mutex g_capture_mu;
+
 // The occupancy source of planning (set_occupancy_source): the HIP query for
 // the batched kernel, cached per kernel and LDS size (a plan asks for a few
 // stage depths; every gfx950 device gives the same answer).
 int cached_occupancy(int kinterp, int kprec, int dtype, int px, int py, size_t shm) {
-  static std::mutex mu;
   static std::map<std::tuple<int, int, int, int, int, size_t>, int> cache;
   const auto key = std::make_tuple(kinterp, kprec, dtype, px, py, shm);
-  {
-    std::lock_guard<std::mutex> lk(mu);
-    const auto it = cache.find(key);
-    if (it != cache.end()) return it->second;
-  }
+  std::lock_guard<std::mutex> lk(g_capture_mu);
+  const auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
   const int n = batch_blocks_per_cu(kinterp, kprec, dtype, px, py, shm);
-  std::lock_guard<std::mutex> lk(mu);
   cache.emplace(key, n);
   return n;
 }
@@ -175,8 +177,9 @@ sarbp_status BatchExec::run(std::vector<JobDesc>& jobs, int interp, int precisio
     hipStream_t cs = nullptr;
     if ((e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking)) != hipSuccess)
       return hip_fail(e, "hipStreamCreate(capture)");
-    const int per_cu = batch_blocks_per_cu(interp, precision, dtype, px, py, shm_class);
+    const int per_cu = cached_occupancy(interp, precision, dtype, px, py, shm_class);
     const int n_blocks = std::max(1, n_cu * per_cu);
+    std::lock_guard<std::mutex> cap(g_capture_mu);
     if ((e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal)) != hipSuccess) {
       (void)hipStreamDestroy(cs);
       return hip_fail(e, "hipStreamBeginCapture");

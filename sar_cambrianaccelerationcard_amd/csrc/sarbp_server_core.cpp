@@ -11,6 +11,7 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdio>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -101,6 +102,7 @@ struct Job {
   bool has_mem = false;
   int32_t state = SARBP_JOB_QUEUED;
   sarbp_status status = SARBP_OK;
+  char detail[192] = {};  // the failing call's detail text (sarbp_server_wait reports it)
   int32_t next_pulse = 0;
   bool started = false;
   int inflight = 0;
@@ -109,6 +111,9 @@ struct Job {
   double t_submit = 0, t_admit = -1, t_first = -1, t_done = -1;
   double work = 0;
 };
+
+// Keeps the calling thread's last error detail (the call that failed the job).
+void set_detail(Job& j) { std::snprintf(j.detail, sizeof j.detail, "%s", last_detail()); }
 
 struct Epoch {
   std::vector<std::pair<uint64_t, int32_t>> parts;  // (job, pulse_end of its chunk)
@@ -205,6 +210,7 @@ void sarbp_server::admit_loop() {
     if (st != SARBP_OK) {
       j->state = SARBP_JOB_FAILED;
       j->status = st;
+      set_detail(*j);
       j->t_done = now();
       ++n_failed;
       release_job(*j);
@@ -253,6 +259,7 @@ void sarbp_server::finish_epoch(Epoch& ep) {
       j.state = SARBP_JOB_FAILED;
       j.status = est != SARBP_OK ? est
                                  : fail(SARBP_ERR_TIMEOUT, "epoch exceeded the %.3g s watchdog", cfg.epoch_watchdog_s);
+      set_detail(j);
       j.t_done = now();
       ++n_failed;
       cv_done.notify_all();
@@ -375,6 +382,7 @@ void sarbp_server::sched_loop() {
         --j.inflight;
         j.state = SARBP_JOB_FAILED;
         j.status = st;
+        set_detail(j);
         j.t_done = now();
         ++n_failed;
         remove_active(j.id);
@@ -413,6 +421,7 @@ void sarbp_server::complete_loop() {
     if (st != SARBP_OK) {
       j->state = SARBP_JOB_FAILED;
       j->status = st;
+      set_detail(*j);
       ++n_failed;
     } else {
       j->state = SARBP_JOB_DONE;
@@ -592,7 +601,7 @@ sarbp_status sarbp_server_wait(sarbp_server* s, uint64_t id, double timeout_s, s
     return fail(SARBP_ERR_TIMEOUT, "job %llu not finished after %.3g s", (unsigned long long)id, timeout_s);
   }
   fill_info(*j, info);
-  if (j->state == SARBP_JOB_FAILED) return j->status;
+  if (j->state == SARBP_JOB_FAILED) return fail(j->status, "job %llu: %s", (unsigned long long)id, j->detail);
   if (j->state == SARBP_JOB_CANCELLED) return fail(SARBP_ERR_CANCELLED, "job %llu cancelled", (unsigned long long)id);
   return SARBP_OK;
 }
