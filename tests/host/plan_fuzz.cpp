@@ -10,8 +10,14 @@
 //     interpolator's taps -- checked by brute force over all pixels and
 //     pulses of small cases, in FP64;
 //   * the DF quantum makes alpha_h di + beta_h dj exact for the tile;
+//   * the stage depth fitted to occupancy (a stand-in occupancy source:
+//     registers allow 4 workgroups per CU, 6 for 1 x 2 tiles; LDS in 512-B
+//     blocks of 160 KiB) is at most the tuned one, never costs a workgroup
+//     per CU, stops at 4 pulses for a plain launch and goes no deeper for a
+//     batched one, with the same window;
 // and the batch grouping must put equal keys together and stay within its
 // limit.  Exit status 0 when every check passes.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <random>
@@ -24,7 +30,7 @@ using namespace sarbp;
 
 namespace {
 
-int g_failures = 0, g_planned = 0, g_df = 0, g_refused = 0;
+int g_failures = 0, g_planned = 0, g_df = 0, g_refused = 0, g_fit_plain = 0, g_fit_batch = 0;
 
 #define CHECK(cond, ...)                                      \
   do {                                                        \
@@ -37,6 +43,13 @@ int g_failures = 0, g_planned = 0, g_df = 0, g_refused = 0;
   } while (0)
 
 double dot(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// Stand-in for the device library's occupancy query.
+int stand_in_occupancy(int, int, int, int px, int py, size_t shm) {
+  const int regs = px == 1 && py == 2 ? 6 : 4;
+  const size_t blocks = (shm + 16 + 511) / 512;
+  return std::max(1, std::min<int>(regs, (int)((160 * 1024 / 512) / blocks)));
+}
 
 // Every window-relative bin of every (tile, pulse, pixel) lies where the
 // interpolator reads inside the window: [lo, hi] in window samples.
@@ -149,6 +162,29 @@ void one_case(std::mt19937_64& rng, bool brute) {
   const double span = std::sqrt(dot(g.ux, g.ux)) * (TX / 2) + std::sqrt(dot(g.uy, g.uy)) * (TY / 2) + 1.0;
   CHECK(span * a.s_quant <= 8388608.0, "quantum 2^e = %g too fine for span %g", a.s_quant, span);
   if (brute && with_pos) check_window(a, pos, f.t.cols_per_lane, f.t.rows_per_lane);
+
+  // stage depth: P0 is the tuned depth fitted to the LDS budget alone
+  int P0 = std::max(1, f.t.pulses_per_stage);
+  while (P0 > 1 && lds_layout(a.kernel_interp, a.win, P0).total > kLdsBudget) --P0;
+  auto occ = [&](int q) {
+    return stand_in_occupancy(0, 0, 0, f.t.cols_per_lane, f.t.rows_per_lane,
+                              lds_layout(a.kernel_interp, a.win, q).total);
+  };
+  BpArgs b{};
+  CHECK(plan_args(h_pos, &g, &p, f.t, f.prec, &b, nullptr, true) == SARBP_OK, "batched plan refused");
+  const int Pp = a.pulses_per_stage, Pb = b.pulses_per_stage;
+  CHECK(b.win == a.win && b.kernel_interp == a.kernel_interp, "batched plan's window differs");
+  CHECK(Pp <= P0 && Pb <= Pp && Pb >= 1, "depths: tuned %d, plain %d, batched %d", P0, Pp, Pb);
+  CHECK(Pp >= std::min(P0, 4), "plain depth %d below 4 (tuned %d)", Pp, P0);
+  CHECK(occ(Pp) >= occ(P0) && occ(Pb) >= occ(Pp), "occupancy lost: %d %d %d", occ(P0), occ(Pp), occ(Pb));
+  // each pulse taken off gained a workgroup per CU, and the stop is where the
+  // next one would not (or at the floor)
+  g_fit_plain += Pp < P0;
+  g_fit_batch += Pb < Pp;
+  for (int q = Pp; q < P0; ++q) CHECK(occ(q) > occ(q + 1), "plain step %d -> %d gained nothing", q + 1, q);
+  CHECK(Pp <= 4 || occ(Pp - 1) <= occ(Pp), "plain fitting stopped early at %d", Pp);
+  for (int q = Pb; q < Pp; ++q) CHECK(occ(q) > occ(q + 1), "batched step %d -> %d gained nothing", q + 1, q);
+  CHECK(Pb == 1 || occ(Pb - 1) <= occ(Pb), "batched fitting stopped early at %d", Pb);
 }
 
 void group_cases(std::mt19937_64& rng) {
@@ -187,11 +223,13 @@ void group_cases(std::mt19937_64& rng) {
 }  // namespace
 
 int main() {
+  set_occupancy_source(&stand_in_occupancy);
   std::mt19937_64 rng(20261203);
   for (int i = 0; i < 3000; ++i) one_case(rng, false);
   for (int i = 0; i < 400; ++i) one_case(rng, true);
   group_cases(rng);
-  std::printf("planned %d (DF fallback or request %d), refused %d\n", g_planned, g_df, g_refused);
+  std::printf("planned %d (DF fallback or request %d), refused %d; stage depth fitted %d plain, %d more batched\n",
+              g_planned, g_df, g_refused, g_fit_plain, g_fit_batch);
   if (g_failures) {
     std::printf("%d checks failed\n", g_failures);
     return 1;
