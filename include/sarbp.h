@@ -460,7 +460,8 @@ sarbp_status sarbp_server_submit_raw(sarbp_server* server, const sarbp_raw_reque
 sarbp_status sarbp_server_poll(sarbp_server* server, uint64_t job_id,
                                sarbp_job_info* info);
 /* Waits until the job is done, failed or cancelled (SARBP_ERR_TIMEOUT after
- * timeout_s; timeout_s < 0 waits forever). */
+ * timeout_s; timeout_s < 0 waits forever).  A failed job returns its failure
+ * status, and sarbp_last_error_detail then names the call that failed it. */
 sarbp_status sarbp_server_wait(sarbp_server* server, uint64_t job_id,
                                double timeout_s, sarbp_job_info* info);
 sarbp_status sarbp_server_cancel(sarbp_server* server, uint64_t job_id);
